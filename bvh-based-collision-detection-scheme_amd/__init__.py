@@ -1,0 +1,388 @@
+"""Python host binding of the gfx950 BVH / BVTT-front collision-detection library.
+
+Thin ctypes layer over ``include/bvhcd.h`` (``libbvhcd.so`` built in this directory). The
+object model follows the reference's two plugins:
+
+* ``CollisionContext.step(bvh_op, front_op)`` = ``LBvhFixedDeformable::maintain`` followed by
+  ``BvttFrontLooseIntra::maintain`` (collision/lbvh/LBvhFixedDeformable.cu:68-87,
+  collision/bvtt_front/BvttFrontLooseIntra.cu:69-126 in Source/Project/Mine/ of
+  littlemine/BVH-based-Collision-Detection-Scheme);
+* ``CollisionContext.pairs()`` = ``applyCpResults`` (BvttFrontLooseIntra.cu:63-67), returning the
+  narrow-phase filtered pairs by default;
+* ``CollisionContext.stats()`` = ``inspectResults`` (BvttFrontLooseIntra.cu:50-61).
+
+The library must be built (``make`` in this directory, or ``__graft_entry__.build()``); there is
+no fallback path: importing works without it, but creating a context raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbvhcd.so")
+
+OK, E_INVALID, E_OOM, E_CAPACITY, E_DEVICE, E_NOT_BUILT = 0, -1, -2, -3, -4, -5
+MEM_HOST, MEM_DEVICE = 0, 1
+BVH_BUILD, BVH_REFIT, BVH_RESTRUCTURE, BVH_NONE = 0, 1, 2, 3
+FRONT_PURE_BVH_CD, FRONT_GENERATE, FRONT_UPDATE, FRONT_REORDER, FRONT_KEEP = 0, 1, 2, 3, 4
+SCHED_PLAIN, SCHED_LEARNED, SCHED_GENERATE = 0, 1, 2
+SCENE_TWO_SPHERES, SCENE_CLOTH, SCENE_LAYERS, SCENE_SHEET_SPHERE = 0, 1, 2, 3
+MLP_NPARAM = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
+BVH_OP_NAMES = {0: "BUILD", 1: "REFIT", 2: "RESTRUCTURE", 3: "NONE"}
+FRONT_OP_NAMES = {0: "PURE_BVH_CD", 1: "GENERATE", 2: "UPDATE", 3: "REORDER", 4: "KEEP"}
+
+
+class BvhcdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bvhcd error {code}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("max_tris", C.c_int64), ("max_verts", C.c_int64),
+        ("int_front_cap", C.c_int64), ("ext_front_cap", C.c_int64), ("pair_cap", C.c_int64),
+        ("narrow_phase", C.c_int32), ("scheduler", C.c_int32), ("enable_restructure", C.c_int32),
+        ("bvh_cycle", C.c_int32), ("front_cycle", C.c_int32), ("device", C.c_int32),
+        ("stream", C.c_void_p),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("frame", C.c_int64), ("last_bvh_op", C.c_int32), ("last_front_op", C.c_int32),
+        ("int_front", C.c_int64), ("ext_front", C.c_int64), ("bp_pairs", C.c_int64), ("np_pairs", C.c_int64),
+        ("required_int_front", C.c_int64), ("required_ext_front", C.c_int64), ("required_pairs", C.c_int64),
+        ("node_visits", C.c_int64), ("restructured_subtrees", C.c_int64), ("restructured_leaves", C.c_int64),
+        ("stage_ms", C.c_double * 8), ("stage_bytes", C.c_double * 8),
+    ]
+
+
+class SceneParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("n", C.c_int32), ("frames", C.c_int32), ("layers", C.c_int32),
+        ("subdiv", C.c_int32), ("amplitude", C.c_float), ("compress", C.c_float), ("sigma", C.c_float),
+        ("radius", C.c_float), ("spacing", C.c_float), ("speed", C.c_float),
+    ]
+
+
+_lib = None
+
+
+def library_built() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    """Load libbvhcd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BvhcdError(E_INVALID, f"{LIB_PATH} is missing: build it first (make -C {_HERE})")
+    L = C.CDLL(LIB_PATH)
+    P, I64, I32, U64 = C.c_void_p, C.c_int64, C.c_int32, C.c_uint64
+    sig = {
+        "bvhcd_create": (C.c_int, [C.POINTER(Config), C.POINTER(P)]),
+        "bvhcd_destroy": (C.c_int, [P]),
+        "bvhcd_last_error": (C.c_char_p, [P]),
+        "bvhcd_set_topology": (C.c_int, [P, P, I64, I64, C.c_int]),
+        "bvhcd_update_vertices": (C.c_int, [P, P, I64, C.c_int]),
+        "bvhcd_step": (C.c_int, [P, C.c_int, C.c_int]),
+        "bvhcd_step_scheduled": (C.c_int, [P]),
+        "bvhcd_get_pairs": (C.c_int, [P, P, I64, C.POINTER(I64), C.c_int, C.c_int]),
+        "bvhcd_pair_digest": (C.c_int, [P, C.c_int, C.POINTER(U64), C.POINTER(I64)]),
+        "bvhcd_get_stats": (C.c_int, [P, C.POINTER(Stats)]),
+        "bvhcd_set_timing": (C.c_int, [P, C.c_int]),
+        "bvhcd_export_bvh": (C.c_int, [P] + [P] * 14),
+        "bvhcd_export_front": (C.c_int, [P, C.c_int, P, I64, C.POINTER(I64)]),
+        "bvhcd_export_scene_box": (C.c_int, [P, P]),
+        "bvhcd_mlp_set_weights": (C.c_int, [P, P, I64]),
+        "bvhcd_mlp_get_weights": (C.c_int, [P, P, I64]),
+        "bvhcd_mlp_features": (C.c_int, [P, P, P, I64, C.POINTER(I64)]),
+        "bvhcd_mlp_forward": (C.c_int, [P, P, I64, P, C.c_int]),
+        "bvhcd_mlp_train_step": (C.c_int, [P, P, P, I64, C.c_float, C.POINTER(C.c_float), C.c_int]),
+        "bvhcd_mlp_grad": (C.c_int, [P, P, P, I64, P, C.POINTER(C.c_float), C.c_int]),
+        "bvhcd_batch_create": (C.c_int, [C.POINTER(Config), I32, P, P, C.POINTER(P)]),
+        "bvhcd_batch_destroy": (C.c_int, [P]),
+        "bvhcd_batch_set_topology": (C.c_int, [P, P, C.c_int]),
+        "bvhcd_batch_update_vertices": (C.c_int, [P, P, C.c_int]),
+        "bvhcd_batch_step": (C.c_int, [P, C.c_int, C.c_int]),
+        "bvhcd_batch_get_pairs": (C.c_int, [P, I32, P, I64, C.POINTER(I64)]),
+        "bvhcd_batch_pair_counts": (C.c_int, [P, P]),
+        "bvhcd_batch_last_error": (C.c_char_p, [P]),
+        "bvhcd_autotune": (C.c_int, [P]),
+        "bvhcd_autotune_report": (C.c_int, [P, I64]),
+        "bvhcd_scene_create": (C.c_int, [C.c_int, C.POINTER(SceneParams), C.POINTER(P)]),
+        "bvhcd_scene_destroy": (None, [P]),
+        "bvhcd_scene_num_tris": (I64, [P]),
+        "bvhcd_scene_num_verts": (I64, [P]),
+        "bvhcd_scene_faces": (C.c_int, [P, P]),
+        "bvhcd_scene_frame": (C.c_int, [P, I32, P, I32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ----------------------------------------------------------------------------- scenes
+@dataclass
+class SceneSpec:
+    kind: int
+    seed: int = 1
+    n: int = 0
+    frames: int = 1
+    layers: int = 1
+    subdiv: int = 5
+    amplitude: float = 0.0
+    compress: float = 0.0
+    sigma: float = 0.0
+    radius: float = 1.0
+    spacing: float = 0.0
+    speed: float = 0.0
+
+
+# The five workloads of the benchmark suite (seeded synthetic deforming meshes).
+PRESETS = {
+    # two icospheres (20,480 tris each), centres 2.5 apart, closing 0.01/frame, N(0,1e-3) jitter
+    "spheres": SceneSpec(SCENE_TWO_SPHERES, seed=1, frames=200, subdiv=5, radius=1.0, sigma=1e-3,
+                         spacing=2.5, speed=0.01),
+    # 256x256 folding cloth (130,050 tris), 300 frames
+    "cloth256": SceneSpec(SCENE_CLOTH, seed=2, n=256, frames=300, amplitude=0.15, compress=0.5),
+    # 1024x1024 folding cloth (2,093,058 tris), 400 frames
+    "cloth1024": SceneSpec(SCENE_CLOTH, seed=4, n=1024, frames=400, amplitude=0.15, compress=0.5),
+    # 4 stacked 1024x1024 cloths 1e-3 apart (8,372,232 tris), 200 frames
+    "layers4": SceneSpec(SCENE_LAYERS, seed=5, n=1024, frames=200, layers=4, amplitude=0.05,
+                         compress=0.5, spacing=1e-3),
+}
+
+
+class Scene:
+    """Seeded synthetic deforming mesh (host generator in the library)."""
+
+    def __init__(self, spec: SceneSpec):
+        L = lib()
+        p = SceneParams(spec.seed, spec.n, spec.frames, spec.layers, spec.subdiv, spec.amplitude,
+                        spec.compress, spec.sigma, spec.radius, spec.spacing, spec.speed)
+        h = C.c_void_p()
+        st = L.bvhcd_scene_create(spec.kind, C.byref(p), C.byref(h))
+        if st != OK:
+            raise BvhcdError(st, "scene_create")
+        self._h = h
+        self.spec = spec
+        self.num_tris = int(L.bvhcd_scene_num_tris(h))
+        self.num_verts = int(L.bvhcd_scene_num_verts(h))
+        self.frames = spec.frames
+
+    @classmethod
+    def preset(cls, name: str, **over):
+        spec = PRESETS[name]
+        spec = SceneSpec(**{**spec.__dict__, **over})
+        return cls(spec)
+
+    def faces(self) -> np.ndarray:
+        f = np.empty((self.num_tris, 3), dtype=np.int32)
+        st = lib().bvhcd_scene_faces(self._h, _ptr(f))
+        if st != OK:
+            raise BvhcdError(st, "scene_faces")
+        return f
+
+    def frame(self, f: int, out: np.ndarray | None = None, threads: int = 0) -> np.ndarray:
+        if out is None:
+            out = np.empty((self.num_verts, 3), dtype=np.float32)
+        st = lib().bvhcd_scene_frame(self._h, int(f), _ptr(out), int(threads))
+        if st != OK:
+            raise BvhcdError(st, f"scene_frame({f})")
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.bvhcd_scene_destroy(h)
+            self._h = None
+
+
+# ----------------------------------------------------------------------------- context
+class CollisionContext:
+    """One BVH + BVTT front over a fixed-topology triangle soup on one GPU."""
+
+    def __init__(self, max_tris, max_verts, *, int_front_cap=0, ext_front_cap=0, pair_cap=0,
+                 narrow_phase=True, scheduler=SCHED_PLAIN, enable_restructure=True, bvh_cycle=12,
+                 front_cycle=6, device=0, stream=None):
+        L = lib()
+        cfg = Config(int(max_tris), int(max_verts), int(int_front_cap), int(ext_front_cap), int(pair_cap),
+                     1 if narrow_phase else 0, int(scheduler), 1 if enable_restructure else 0,
+                     int(bvh_cycle), int(front_cycle), int(device), stream)
+        h = C.c_void_p()
+        st = L.bvhcd_create(C.byref(cfg), C.byref(h))
+        if st != OK:
+            raise BvhcdError(st, "bvhcd_create failed (no GPU, or out of memory)")
+        self._h = h
+        self.T = 0
+        self.V = 0
+
+    def _chk(self, st, what):
+        if st != OK:
+            msg = lib().bvhcd_last_error(self._h)
+            raise BvhcdError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bvhcd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_topology(self, faces, num_verts, where=MEM_HOST):
+        if where == MEM_HOST:
+            f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+            self._chk(lib().bvhcd_set_topology(self._h, _ptr(f), f.shape[0], int(num_verts), MEM_HOST), "set_topology")
+            self.T = f.shape[0]
+        else:  # device pointer (int), faces = (ptr, T)
+            ptr, T = faces
+            self._chk(lib().bvhcd_set_topology(self._h, C.c_void_p(ptr), int(T), int(num_verts), MEM_DEVICE),
+                      "set_topology")
+            self.T = int(T)
+        self.V = int(num_verts)
+
+    def update_vertices(self, xyz, where=MEM_HOST):
+        if where == MEM_HOST:
+            v = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+            self._chk(lib().bvhcd_update_vertices(self._h, _ptr(v), v.shape[0], MEM_HOST), "update_vertices")
+        else:
+            self._chk(lib().bvhcd_update_vertices(self._h, C.c_void_p(int(xyz)), self.V, MEM_DEVICE),
+                      "update_vertices")
+
+    def step(self, bvh_op, front_op):
+        self._chk(lib().bvhcd_step(self._h, int(bvh_op), int(front_op)), "step")
+
+    def step_scheduled(self):
+        self._chk(lib().bvhcd_step_scheduled(self._h), "step_scheduled")
+
+    def set_timing(self, on=True):
+        self._chk(lib().bvhcd_set_timing(self._h, 1 if on else 0), "set_timing")
+
+    def pairs(self, filtered=True) -> np.ndarray:
+        n = C.c_int64()
+        self._chk(lib().bvhcd_get_pairs(self._h, None, 0, C.byref(n), MEM_HOST, 1 if filtered else 0), "get_pairs")
+        out = np.empty((n.value, 2), dtype=np.int32)
+        if n.value:
+            self._chk(lib().bvhcd_get_pairs(self._h, _ptr(out), n.value, C.byref(n), MEM_HOST,
+                                            1 if filtered else 0), "get_pairs")
+        return out
+
+    def digest(self, filtered=True):
+        d = C.c_uint64()
+        n = C.c_int64()
+        self._chk(lib().bvhcd_pair_digest(self._h, 1 if filtered else 0, C.byref(d), C.byref(n)), "pair_digest")
+        return int(d.value), int(n.value)
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._chk(lib().bvhcd_get_stats(self._h, C.byref(s)), "get_stats")
+        out = {f: getattr(s, f) for f, _ in Stats._fields_ if f not in ("stage_ms", "stage_bytes")}
+        out["stage_ms"] = list(s.stage_ms)
+        out["stage_bytes"] = list(s.stage_bytes)
+        return out
+
+    def scene_box(self) -> np.ndarray:
+        b = np.empty(6, dtype=np.float32)
+        self._chk(lib().bvhcd_export_scene_box(self._h, _ptr(b)), "export_scene_box")
+        return b
+
+    def export_bvh(self) -> dict:
+        n, I = self.T, max(self.T - 1, 0)
+        a = {
+            "codes": np.empty(n, np.uint32), "prim_idx": np.empty(n, np.int32), "prim_map": np.empty(n, np.int32),
+            "int_lc": np.empty(I, np.int32), "int_rc": np.empty(I, np.int32), "int_par": np.empty(I, np.int32),
+            "int_mark": np.empty(I, np.uint32), "int_rx": np.empty(I, np.int32), "int_ry": np.empty(I, np.int32),
+            "leaf_par": np.empty(n, np.int32), "leaf_mark": np.empty(n, np.uint32),
+            "leaf_lca": np.empty(n + 1, np.int32),
+            "int_box": np.empty((I, 6), np.float32), "leaf_box": np.empty((n, 6), np.float32),
+        }
+        order = ["codes", "prim_idx", "prim_map", "int_lc", "int_rc", "int_par", "int_mark", "int_rx", "int_ry",
+                 "leaf_par", "leaf_mark", "leaf_lca", "int_box", "leaf_box"]
+        self._chk(lib().bvhcd_export_bvh(self._h, *[_ptr(a[k]) for k in order]), "export_bvh")
+        return a
+
+    def export_front(self, which) -> np.ndarray:
+        n = C.c_int64()
+        self._chk(lib().bvhcd_export_front(self._h, int(which), None, 0, C.byref(n)), "export_front")
+        out = np.empty((n.value, 2), dtype=np.int32)
+        if n.value:
+            self._chk(lib().bvhcd_export_front(self._h, int(which), _ptr(out), n.value, C.byref(n)), "export_front")
+        return out
+
+    # ---- learned scheduler
+    def mlp_set_weights(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        self._chk(lib().bvhcd_mlp_set_weights(self._h, _ptr(p), p.size), "mlp_set_weights")
+
+    def mlp_get_weights(self):
+        p = np.empty(MLP_NPARAM, np.float32)
+        self._chk(lib().bvhcd_mlp_get_weights(self._h, _ptr(p), p.size), "mlp_get_weights")
+        return p
+
+    def mlp_features(self, cap=1 << 20):
+        f = np.empty((cap, 16), np.float32)
+        r = np.empty(cap, np.int32)
+        rows = C.c_int64()
+        self._chk(lib().bvhcd_mlp_features(self._h, _ptr(f), _ptr(r), cap, C.byref(rows)), "mlp_features")
+        return f[: rows.value].copy(), r[: rows.value].copy()
+
+    def mlp_forward(self, feats):
+        x = np.ascontiguousarray(feats, dtype=np.float32)
+        out = np.empty(x.shape[0], np.float32)
+        self._chk(lib().bvhcd_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(out), MEM_HOST), "mlp_forward")
+        return out
+
+    def mlp_train_step(self, feats, targets, lr):
+        x = np.ascontiguousarray(feats, dtype=np.float32)
+        y = np.ascontiguousarray(targets, dtype=np.float32)
+        loss = C.c_float()
+        self._chk(lib().bvhcd_mlp_train_step(self._h, _ptr(x), _ptr(y), x.shape[0], float(lr), C.byref(loss),
+                                             MEM_HOST), "mlp_train_step")
+        return float(loss.value)
+
+    def mlp_grad(self, feats, targets):
+        x = np.ascontiguousarray(feats, dtype=np.float32)
+        y = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        self._chk(lib().bvhcd_mlp_grad(self._h, _ptr(x), _ptr(y), x.shape[0], _ptr(g), C.byref(loss), MEM_HOST),
+                  "mlp_grad")
+        return g, float(loss.value)
+
+
+def plain_schedule(frame, bvh_cycle=12, front_cycle=6):
+    """BenchmarkLogic::maintainScheme, STATIC_MANDATORY with divergent mark
+    (system/Logic/BenchmarkLogic.hpp:102-117)."""
+    bvh = BVH_REFIT if frame % bvh_cycle else BVH_BUILD
+    fr = frame % front_cycle
+    if fr == 0:
+        front = FRONT_GENERATE if frame == 0 else FRONT_UPDATE
+    elif fr == front_cycle - 1:
+        front = FRONT_REORDER
+    else:
+        front = FRONT_KEEP
+    if bvh == BVH_BUILD:
+        front = FRONT_GENERATE
+    if (frame + 1) % bvh_cycle == 0:
+        front = FRONT_KEEP
+    return bvh, front

@@ -1,0 +1,768 @@
+// C ABI of the gfx950 collision-detection pipeline: context, per-frame maintenance, scheduler.
+//
+// Mirrors LBvhFixedDeformable::maintain (collision/lbvh/LBvhFixedDeformable.cu:68-87) and
+// BvttFrontLooseIntra::maintain (collision/bvtt_front/BvttFrontLooseIntra.cu:69-126) of
+// littlemine/BVH-based-Collision-Detection-Scheme (Source/Project/Mine/), with the plain
+// schedule of BenchmarkLogic::maintainScheme (system/Logic/BenchmarkLogic.hpp:102-117).
+// All work is stream-ordered on the context stream; only get_pairs, pair_digest, get_stats,
+// the export calls, timing read-back and the restructure decision wait for the device.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "bvhcd_internal.hpp"
+#include "kernels.hpp"
+#include "learned.hpp"
+
+using namespace bvhcd;
+
+namespace {
+// stage ids of bvhcd_stats.stage_ms / stage_bytes
+enum Stage { S_BUILD = 0, S_REFIT = 1, S_RESTR = 2, S_FRONT = 3, S_ORDER = 4, S_NARROW = 5, S_SCHED = 6, S_TOTAL = 7 };
+constexpr int kEv = 16;   // begin/end event per stage
+constexpr int kLog = 8;   // device log words per frame
+}  // namespace
+
+struct bvhcd_ctx {
+    bvhcd_config cfg{};
+    DevState d;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    int64_t T = 0, V = 0;
+    bool built = false, front_valid = false, restr_pending = false;
+    int64_t frame = 0;
+    int last_bvh = -1, last_front = -1;
+    int64_t restr_sub = 0, restr_leaf = 0;
+    // timing: kEv events per recorded frame, device counter log, executed ops
+    int64_t tcap = 0, tn = 0;
+    std::vector<hipEvent_t> tev;
+    std::vector<uint8_t> tused;
+    uint64_t* dlog = nullptr;
+    std::vector<int32_t> tops;
+    std::vector<int64_t> trestr;
+    std::vector<void*> allocs;
+    LearnedState* learned = nullptr;
+};
+
+namespace {
+
+int fail(bvhcd_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail((c), BVHCD_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+bool dalloc(bvhcd_ctx* c, T*& p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) return false;
+    c->allocs.push_back(q);
+    p = static_cast<T*>(q);
+    return true;
+}
+
+void free_all(bvhcd_ctx* c) {
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+}
+
+int allocate(bvhcd_ctx* c) {
+    DevState& d = c->d;
+    const int64_t T = std::max<int64_t>(c->cfg.max_tris, 1), V = std::max<int64_t>(c->cfg.max_verts, 1);
+    d.maxT = T;
+    d.maxV = V;
+    bool ok = true;
+    ok &= dalloc(c, d.faces, 3 * T);
+    ok &= dalloc(c, d.verts, 3 * V);
+    ok &= dalloc(c, d.codes, T);
+    ok &= dalloc(c, d.keys_tmp, T);
+    ok &= dalloc(c, d.keys_tmp2, T);
+    ok &= dalloc(c, d.prim_idx, T);
+    ok &= dalloc(c, d.vals_tmp, T);
+    ok &= dalloc(c, d.prim_map, T);
+    ok &= dalloc(c, d.inode, T);
+    ok &= dalloc(c, d.leaf, T);
+    ok &= dalloc(c, d.int_lc, T);
+    ok &= dalloc(c, d.int_rc, T);
+    ok &= dalloc(c, d.int_par, T);
+    ok &= dalloc(c, d.int_mark, T);
+    ok &= dalloc(c, d.int_rx, T);
+    ok &= dalloc(c, d.int_ry, T);
+    ok &= dalloc(c, d.int_flag, T);
+    ok &= dalloc(c, d.leaf_par, T);
+    ok &= dalloc(c, d.leaf_mark, T);
+    ok &= dalloc(c, d.leaf_lca, T + 1);
+    ok &= dalloc(c, d.s_a, T);
+    ok &= dalloc(c, d.s_b, T);
+    ok &= dalloc(c, d.s_par, T);
+    ok &= dalloc(c, d.s_lc, T);
+    ok &= dalloc(c, d.s_rc, T);
+    ok &= dalloc(c, d.s_lpar, T);
+    ok &= dalloc(c, d.s_top, T);
+    ok &= dalloc(c, d.depths, T + 1);
+    ok &= dalloc(c, d.offsets, T + 1);
+    ok &= dalloc(c, d.tkmap, T);
+    // scan block sums: enough for the largest scanned array (digit table of the sort)
+    const int64_t tiles = (T + 4095) / 4096;
+    d.sort_hist_n = (size_t)(256 * tiles);
+    const int64_t max_scan = std::max<int64_t>((int64_t)d.sort_hist_n, T + 2);
+    d.scan_tmp_n = (size_t)((max_scan + 4095) / 4096 + 1);
+    ok &= dalloc(c, d.scan_tmp, d.scan_tmp_n);
+    ok &= dalloc(c, d.sort_hist, d.sort_hist_n);
+    ok &= dalloc(c, d.scene_box, 8);
+    ok &= dalloc(c, d.scene_box_u, 8);
+    ok &= dalloc(c, d.fsize, 4);
+    ok &= dalloc(c, d.intcnt, T + 1);
+    ok &= dalloc(c, d.extcnt, T + 2);
+    ok &= dalloc(c, d.intbeg, T + 2);
+    ok &= dalloc(c, d.extbeg, T + 2);
+    ok &= dalloc(c, d.intback, T + 1);
+    ok &= dalloc(c, d.cursor, T + 1);
+    ok &= dalloc(c, d.snapshot, T);
+    ok &= dalloc(c, d.intrange, T + 1);
+    ok &= dalloc(c, d.rtroot, T + 1);
+    ok &= dalloc(c, d.intmark, T + 1);
+    ok &= dalloc(c, d.rt_of_leaf, T + 1);
+    ok &= dalloc(c, d.r_leaf_lo, T + 1);
+    ok &= dalloc(c, d.r_leaf_hi, T + 1);
+    ok &= dalloc(c, d.r_split_lo, T);
+    ok &= dalloc(c, d.r_split_hi, T);
+    ok &= dalloc(c, d.r_base, T);
+    ok &= dalloc(c, d.r_keep_par, T);
+    ok &= dalloc(c, d.r_keep_mark, T);
+    ok &= dalloc(c, d.r_keys, T);
+    ok &= dalloc(c, d.r_keys2, T);
+    ok &= dalloc(c, d.ctr, 32);
+    if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fixed arrays)");
+    // capacities: explicit, or a multiple of T bounded by free HBM and the 32-bit counters
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    const int64_t lim = 0x7FFFFFFFll;
+    int64_t ci = c->cfg.int_front_cap, ce = c->cfg.ext_front_cap, cp = c->cfg.pair_cap;
+    if (ci <= 0) ci = 48 * T + (1 << 20);
+    if (ce <= 0) ce = 24 * T + (1 << 20);
+    if (cp <= 0) cp = 24 * T + (1 << 20);
+    ci = std::min(ci, lim); ce = std::min(ce, lim); cp = std::min(cp, lim);
+    // 2 buffers x 8 B for each front, 2 x 8 B for pairs
+    const double need = 16.0 * (double)(ci + ce) + 16.0 * (double)cp;
+    const double budget = 0.7 * (double)free_b;
+    if (need > budget && (c->cfg.int_front_cap <= 0 || c->cfg.ext_front_cap <= 0 || c->cfg.pair_cap <= 0)) {
+        const double f = budget / need;
+        if (c->cfg.int_front_cap <= 0) ci = (int64_t)(ci * f);
+        if (c->cfg.ext_front_cap <= 0) ce = (int64_t)(ce * f);
+        if (c->cfg.pair_cap <= 0) cp = (int64_t)(cp * f);
+    }
+    d.front_cap[0] = std::max<int64_t>(ci, 1);
+    d.front_cap[1] = std::max<int64_t>(ce, 1);
+    d.pair_cap = std::max<int64_t>(cp, 1);
+    ok &= dalloc(c, d.front[0][0], d.front_cap[0]);
+    ok &= dalloc(c, d.front[1][0], d.front_cap[0]);
+    ok &= dalloc(c, d.front[0][1], d.front_cap[1]);
+    ok &= dalloc(c, d.front[1][1], d.front_cap[1]);
+    ok &= dalloc(c, d.bp, d.pair_cap);
+    ok &= dalloc(c, d.np, d.pair_cap);
+    if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
+    HIPCHK(c, hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(d.fsize, 0, 4 * sizeof(uint32_t), c->stream));
+    return BVHCD_OK;
+}
+
+// ---------------------------------------------------------------- timing
+bool timing_on(const bvhcd_ctx* c) { return c->tcap > 0 && c->tn < c->tcap; }
+void mark(bvhcd_ctx* c, int stage, bool end) {
+    if (!timing_on(c)) return;
+    const int64_t k = c->tn * kEv + 2 * stage + (end ? 1 : 0);
+    (void)hipEventRecord(c->tev[k], c->stream);
+    c->tused[k] = 1;
+}
+void begin(bvhcd_ctx* c, int stage) { mark(c, stage, false); }
+void end(bvhcd_ctx* c, int stage) { mark(c, stage, true); }
+
+__global__ void k_log_frame(DevState d, int cur, uint64_t* out) {
+    if (threadIdx.x != 0) return;
+    out[0] = ctr64(d.ctr)[C64_VISITS];
+    out[1] = d.ctr[C_BP];
+    out[2] = d.ctr[C_NP];
+    out[3] = d.fsize[2 * cur];
+    out[4] = d.fsize[2 * cur + 1];
+    out[5] = d.ctr[C_OVF];
+}
+
+void log_frame(bvhcd_ctx* c, int bvh_op, int front_op) {
+    if (!timing_on(c)) return;
+    hipLaunchKernelGGL(k_log_frame, dim3(1), dim3(64), 0, c->stream, c->d, c->d.cur, c->dlog + kLog * c->tn);
+    c->tops.push_back(bvh_op);
+    c->tops.push_back(front_op);
+    c->trestr.push_back(c->restr_leaf);
+    c->tn++;
+}
+
+// Algorithmic HBM bytes per stage of one frame:
+//   B_build  = 24V + 16T (box + codes) + 4T + 16kT (k = 4 sort passes) + 4T + 24I + 12T + 24(T+I)
+//   B_refit  = 12V + 40T + 88I
+//   B_restr  = refit + 16 (k+3) L_r over the restructured leaves L_r (keys, sort, emission)
+//   B_front  = 32 N_in + 36 N_visit + 12 N_out + 8 P_bp
+//   B_order  = 16 N_out + 16T
+//   B_narrow = 104 P_bp + 8 P_np
+void frame_bytes(double T, double V, int bvh_op, int front_op, double visits, double fi0, double fe0, double fi1,
+                 double fe1, double bp, double np, double restr_leaves, double* out) {
+    const double I = T > 1 ? T - 1 : 0;
+    for (int i = 0; i < 8; ++i) out[i] = 0;
+    if (bvh_op == BVHCD_BVH_BUILD)
+        out[S_BUILD] = 24 * V + 16 * T + (4 * T + 16.0 * 4 * T) + (4 * T + 24 * I + 12 * T + 24 * (T + I));
+    if (bvh_op == BVHCD_BVH_REFIT || bvh_op == BVHCD_BVH_BUILD) out[S_REFIT] = 12 * V + 40 * T + 88 * I;
+    if (bvh_op == BVHCD_BVH_RESTRUCTURE) out[S_RESTR] = 12 * V + 40 * T + 88 * I + 16.0 * 7 * restr_leaves;
+    const bool whole = front_op == BVHCD_FRONT_PURE_BVH_CD || front_op == BVHCD_FRONT_GENERATE;
+    const double nin = whole ? T : fi0 + fe0;
+    const double nout = (front_op == BVHCD_FRONT_KEEP || front_op == BVHCD_FRONT_PURE_BVH_CD) ? 0 : fi1 + fe1;
+    out[S_FRONT] = 32 * nin + 36 * visits + 12 * nout + 8 * bp;
+    if (front_op == BVHCD_FRONT_GENERATE || front_op == BVHCD_FRONT_UPDATE || front_op == BVHCD_FRONT_REORDER)
+        out[S_ORDER] = 16 * nout + 16 * T;
+    out[S_NARROW] = 104 * bp + 8 * np;
+    double tot = 0;
+    for (int i = 0; i < 7; ++i) tot += out[i];
+    out[S_TOTAL] = tot;
+}
+
+// ---------------------------------------------------------------- BVH maintenance
+int do_build(bvhcd_ctx* c) {
+    DevState& d = c->d;
+    const int32_t n = (int32_t)c->T;
+    hipStream_t s = c->stream;
+    launch_scene_box(d, n, s);
+    launch_codes(d, n, d.codes, d.prim_idx, s);
+    radix_sort_u32(d, d.codes, d.prim_idx, d.keys_tmp, d.vals_tmp, n, 30, s);
+    launch_prim_map(d, n, s);
+    if (n >= 2) {
+        launch_emit_full(d, n, s);
+    } else {
+        // single leaf: no internal nodes; lca = {leaf 0, end}
+        const int32_t lca[2] = {1, -1};
+        HIPCHK(c, hipMemcpyAsync(d.leaf_lca, lca, sizeof(lca), hipMemcpyHostToDevice, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        HIPCHK(c, hipMemsetAsync(d.leaf_par, 0xff, sizeof(int32_t), s));
+        fill_u32(d.leaf_mark, 1, 3u, s);
+    }
+    return BVHCD_OK;
+}
+
+int build_and_refit(bvhcd_ctx* c) {
+    begin(c, S_BUILD);
+    int st = do_build(c);
+    if (st) return st;
+    end(c, S_BUILD);
+    begin(c, S_REFIT);
+    launch_refit(c->d, (int32_t)c->T, c->stream);
+    end(c, S_REFIT);
+    return BVHCD_OK;
+}
+
+// Restructure frame: select subtrees (reference heuristic, or the learned scheduler), then
+// rebuild (> I/2 flagged internal nodes), refit (< I/8) or restructure the selection.
+int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
+    DevState& d = c->d;
+    const int32_t n = (int32_t)c->T;
+    const int64_t I = n - 1;
+    begin(c, S_SCHED);
+    int64_t nsub = 0, nint = 0, nleaf = 0;
+    bool learned = c->cfg.scheduler == BVHCD_SCHED_LEARNED && learned_ready(c->learned);
+    if (learned) {
+        int st = learned_select(c->learned, d, n, c->frame, c->stream, &nsub, &nint, &nleaf, &c->err);
+        if (st < 0) return st;
+    } else {
+        launch_restr_decision(d, n, c->stream);
+        uint32_t cnt[3];
+        HIPCHK(c, hipMemcpyAsync(cnt, d.ctr + C_RT_SUB, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        nsub = cnt[0]; nint = cnt[1]; nleaf = cnt[2];
+    }
+    end(c, S_SCHED);
+    if (!learned && nint > (I >> 1)) {
+        int st = build_and_refit(c);
+        if (st) return st;
+        bvh_op = BVHCD_BVH_BUILD;
+        topo_changed = true;
+    } else if ((!learned && nint < (I >> 3)) || nsub == 0) {
+        begin(c, S_REFIT);
+        launch_refit(d, n, c->stream);
+        end(c, S_REFIT);
+        bvh_op = BVHCD_BVH_REFIT;
+    } else {
+        begin(c, S_RESTR);
+        int st = restructure_subtrees(d, n, c->stream);
+        if (st) return fail(c, st, "restructure failed");
+        end(c, S_RESTR);
+        c->restr_sub = nsub;
+        c->restr_leaf = nleaf;
+        topo_changed = true;
+    }
+    return BVHCD_OK;
+}
+
+int run_bvh(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
+    topo_changed = false;
+    if (bvh_op == BVHCD_BVH_RESTRUCTURE && (!c->built || !c->restr_pending)) bvh_op = BVHCD_BVH_REFIT;
+    if (bvh_op != BVHCD_BVH_BUILD && bvh_op != BVHCD_BVH_NONE && !c->built)
+        return fail(c, BVHCD_E_NOT_BUILT, "no BVH yet: first step must BUILD");
+    if (bvh_op == BVHCD_BVH_NONE && !c->built) return fail(c, BVHCD_E_NOT_BUILT, "no BVH yet");
+    c->restr_sub = c->restr_leaf = 0;
+    switch (bvh_op) {
+        case BVHCD_BVH_BUILD: {
+            int st = build_and_refit(c);
+            if (st) return st;
+            c->built = true;
+            topo_changed = true;
+            break;
+        }
+        case BVHCD_BVH_REFIT:
+            begin(c, S_REFIT);
+            launch_refit(c->d, (int32_t)c->T, c->stream);
+            end(c, S_REFIT);
+            break;
+        case BVHCD_BVH_RESTRUCTURE: {
+            int st = run_restructure(c, bvh_op, topo_changed);
+            if (st) return st;
+            break;
+        }
+        case BVHCD_BVH_NONE:
+            break;
+        default:
+            return fail(c, BVHCD_E_INVALID, "bad bvh_op");
+    }
+    return BVHCD_OK;
+}
+
+int run_front(bvhcd_ctx* c, int& front_op) {
+    DevState& d = c->d;
+    const int32_t n = (int32_t)c->T;
+    hipStream_t s = c->stream;
+    if (front_op != BVHCD_FRONT_PURE_BVH_CD && !c->front_valid) front_op = BVHCD_FRONT_GENERATE;
+    c->restr_pending = false;  // the quality flag lives for one frame
+    const bool quality = c->cfg.enable_restructure != 0;
+    switch (front_op) {
+        case BVHCD_FRONT_PURE_BVH_CD:
+            begin(c, S_FRONT);
+            launch_pure_cd(d, n, s);
+            end(c, S_FRONT);
+            break;
+        case BVHCD_FRONT_GENERATE:
+            begin(c, S_FRONT);
+            launch_generate(d, n, s);  // traversal + ordering
+            end(c, S_FRONT);
+            begin(c, S_ORDER);
+            if (quality) launch_snapshot(d, n, s);
+            end(c, S_ORDER);
+            c->front_valid = true;
+            break;
+        case BVHCD_FRONT_UPDATE:
+            begin(c, S_FRONT);
+            launch_update(d, n, quality, s);  // prune/sprout + quality + ordering
+            end(c, S_FRONT);
+            if (quality) c->restr_pending = true;
+            break;
+        case BVHCD_FRONT_REORDER:
+            begin(c, S_FRONT);
+            launch_reorder(d, n, s);
+            end(c, S_FRONT);
+            break;
+        case BVHCD_FRONT_KEEP:
+            begin(c, S_FRONT);
+            launch_keep(d, n, s);
+            end(c, S_FRONT);
+            break;
+        default:
+            return fail(c, BVHCD_E_INVALID, "bad front_op");
+    }
+    begin(c, S_NARROW);
+    if (c->cfg.narrow_phase) launch_narrow(d, s);
+    else (void)hipMemsetAsync(d.ctr + C_NP, 0, sizeof(uint32_t), s);
+    end(c, S_NARROW);
+    return BVHCD_OK;
+}
+
+int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
+    if (c->T <= 0) return fail(c, BVHCD_E_INVALID, "set_topology first");
+    if (bvh_op < 0 || bvh_op > BVHCD_BVH_NONE || front_op < 0 || front_op > BVHCD_FRONT_KEEP)
+        return fail(c, BVHCD_E_INVALID, "bad op");
+    begin(c, S_TOTAL);
+    bool topo = false;
+    int st = run_bvh(c, bvh_op, topo);
+    if (st) return st;
+    if (topo) c->front_valid = false;
+    st = run_front(c, front_op);
+    if (st) return st;
+    end(c, S_TOTAL);
+    c->last_bvh = bvh_op;
+    c->last_front = front_op;
+    log_frame(c, bvh_op, front_op);
+    c->frame++;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, BVHCD_E_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return BVHCD_OK;
+}
+
+// BenchmarkLogic::maintainScheme, STATIC_MANDATORY with divergent mark.
+void plain_schedule(int64_t f, int bc, int fc, int& bvh_op, int& front_op) {
+    bvh_op = (f % bc) ? BVHCD_BVH_REFIT : BVHCD_BVH_BUILD;
+    const int64_t fr = f % fc;
+    if (fr == 0) front_op = f == 0 ? BVHCD_FRONT_GENERATE : BVHCD_FRONT_UPDATE;
+    else if (fr == fc - 1) front_op = BVHCD_FRONT_REORDER;
+    else front_op = BVHCD_FRONT_KEEP;
+    if (bvh_op == BVHCD_BVH_BUILD) front_op = BVHCD_FRONT_GENERATE;
+    if ((f + 1) % bc == 0) front_op = BVHCD_FRONT_KEEP;
+}
+
+int check_overflow(bvhcd_ctx* c) {
+    uint32_t ovf = 0;
+    HIPCHK(c, hipMemcpyAsync(&ovf, c->d.ctr + C_OVF, sizeof(ovf), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (ovf) {
+        char buf[160];
+        if (ovf & 16u) return fail(c, BVHCD_E_DEVICE, "traversal step bound exceeded (corrupted tree)");
+        snprintf(buf, sizeof(buf), "capacity exceeded (flags 0x%x: 1 int front, 2 ext front, 4 bp pairs, 8 np pairs)",
+                 ovf);
+        return fail(c, BVHCD_E_CAPACITY, buf);
+    }
+    return BVHCD_OK;
+}
+
+void timing_free(bvhcd_ctx* c) {
+    for (auto& e : c->tev) (void)hipEventDestroy(e);
+    c->tev.clear();
+    c->tused.clear();
+    if (c->dlog) {
+        (void)hipFree(c->dlog);
+        c->dlog = nullptr;
+    }
+    c->tcap = c->tn = 0;
+    c->tops.clear();
+    c->trestr.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int bvhcd_create(const bvhcd_config* cfg, bvhcd_ctx** out) {
+    if (!cfg || !out || cfg->max_tris < 1 || cfg->max_verts < 1 || cfg->max_tris > (1ll << 30) ||
+        cfg->max_verts > (1ll << 30))
+        return BVHCD_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || cfg->device < 0 || cfg->device >= ndev)
+        return BVHCD_E_DEVICE;
+    bvhcd_ctx* c = new (std::nothrow) bvhcd_ctx();
+    if (!c) return BVHCD_E_OOM;
+    c->cfg = *cfg;
+    if (c->cfg.bvh_cycle <= 0) c->cfg.bvh_cycle = 12;
+    if (c->cfg.front_cycle <= 0) c->cfg.front_cycle = 6;
+    if (hipSetDevice(cfg->device) != hipSuccess) {
+        delete c;
+        return BVHCD_E_DEVICE;
+    }
+    if (cfg->stream) {
+        c->stream = static_cast<hipStream_t>(cfg->stream);
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return BVHCD_E_DEVICE;
+        }
+        c->own_stream = true;
+    }
+    int st = allocate(c);
+    if (st) {
+        free_all(c);
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return st;
+    }
+    *out = c;
+    return BVHCD_OK;
+}
+
+int bvhcd_destroy(bvhcd_ctx* c) {
+    if (!c) return BVHCD_E_INVALID;
+    (void)hipStreamSynchronize(c->stream);
+    learned_destroy(c->learned);
+    timing_free(c);
+    free_all(c);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BVHCD_OK;
+}
+
+const char* bvhcd_last_error(const bvhcd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V, int where) {
+    if (!c) return BVHCD_E_INVALID;
+    if (!faces || T < 1 || V < 1 || T > c->d.maxT || V > c->d.maxV) return fail(c, BVHCD_E_INVALID, "bad topology size");
+    HIPCHK(c, hipMemcpyAsync(c->d.faces, faces, sizeof(int32_t) * 3 * T,
+                             where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->T = T;
+    c->V = V;
+    c->built = c->front_valid = c->restr_pending = false;
+    c->frame = 0;
+    c->d.cur = 0;
+    HIPCHK(c, hipMemsetAsync(c->d.fsize, 0, 4 * sizeof(uint32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
+    return BVHCD_OK;
+}
+
+int bvhcd_update_vertices(bvhcd_ctx* c, const float* xyz, int64_t V, int where) {
+    if (!c) return BVHCD_E_INVALID;
+    if (!xyz || V != c->V) return fail(c, BVHCD_E_INVALID, "vertex count differs from set_topology");
+    HIPCHK(c, hipMemcpyAsync(c->d.verts, xyz, sizeof(float) * 3 * V,
+                             where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BVHCD_OK;
+}
+
+int bvhcd_step(bvhcd_ctx* c, int bvh_op, int front_op) {
+    if (!c) return BVHCD_E_INVALID;
+    return do_step(c, bvh_op, front_op);
+}
+
+int bvhcd_step_scheduled(bvhcd_ctx* c) {
+    if (!c) return BVHCD_E_INVALID;
+    int bvh_op, front_op;
+    if (c->cfg.scheduler == BVHCD_SCHED_GENERATE) return do_step(c, BVHCD_BVH_BUILD, BVHCD_FRONT_PURE_BVH_CD);
+    plain_schedule(c->frame, c->cfg.bvh_cycle, c->cfg.front_cycle, bvh_op, front_op);
+    if (bvh_op == BVHCD_BVH_REFIT && c->cfg.enable_restructure && c->restr_pending) bvh_op = BVHCD_BVH_RESTRUCTURE;
+    return do_step(c, bvh_op, front_op);
+}
+
+int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int where, int filtered) {
+    if (!c || !num) return BVHCD_E_INVALID;
+    uint32_t cnt = 0;
+    HIPCHK(c, hipMemcpyAsync(&cnt, c->d.ctr + (filtered ? C_NP : C_BP), sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    int st = check_overflow(c);
+    *num = cnt;
+    if (st) return st;
+    if (out && cap > 0) {
+        const int64_t k = std::min<int64_t>(cap, cnt);
+        if (k > 0)
+            HIPCHK(c, hipMemcpyAsync(out, filtered ? c->d.np : c->d.bp, sizeof(int2) * k,
+                                     where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                     c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return BVHCD_OK;
+}
+
+int bvhcd_pair_digest(bvhcd_ctx* c, int filtered, uint64_t* digest, int64_t* num) {
+    if (!c || !digest) return BVHCD_E_INVALID;
+    launch_digest(c->d, filtered, c->stream);
+    uint64_t dg = 0;
+    uint32_t cnt = 0;
+    HIPCHK(c, hipMemcpyAsync(&dg, ctr64(c->d.ctr) + C64_DIGEST, sizeof(dg), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&cnt, c->d.ctr + (filtered ? C_NP : C_BP), sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    int st = check_overflow(c);
+    *digest = dg;
+    if (num) *num = cnt;
+    return st;
+}
+
+int bvhcd_set_timing(bvhcd_ctx* c, int frames) {
+    if (!c || frames < 0) return BVHCD_E_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    timing_free(c);
+    if (frames == 0) return BVHCD_OK;
+    c->tev.resize((size_t)frames * kEv);
+    for (auto& e : c->tev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(c, BVHCD_E_DEVICE, "hipEventCreate");
+    c->tused.assign((size_t)frames * kEv, 0);
+    if (hipMalloc(&c->dlog, sizeof(uint64_t) * kLog * (size_t)frames) != hipSuccess)
+        return fail(c, BVHCD_E_OOM, "timing log");
+    HIPCHK(c, hipMemsetAsync(c->dlog, 0, sizeof(uint64_t) * kLog * (size_t)frames, c->stream));
+    c->tcap = frames;
+    c->tn = 0;
+    return BVHCD_OK;
+}
+
+int bvhcd_timing_frames(bvhcd_ctx* c, int64_t cap, double* ms, double* bytes, int32_t* ops, int64_t* n) {
+    if (!c || !n) return BVHCD_E_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *n = c->tn;
+    if (c->tn == 0) return BVHCD_OK;
+    std::vector<uint64_t> log((size_t)kLog * c->tn);
+    HIPCHK(c, hipMemcpy(log.data(), c->dlog, sizeof(uint64_t) * log.size(), hipMemcpyDeviceToHost));
+    double fi0 = 0, fe0 = 0;
+    for (int64_t f = 0; f < c->tn && f < cap; ++f) {
+        const uint64_t* L = &log[kLog * f];
+        const int bo = c->tops[2 * f], fo = c->tops[2 * f + 1];
+        if (ms) {
+            for (int s = 0; s < 8; ++s) {
+                const int64_t k = f * kEv + 2 * s;
+                float t = 0;
+                if (c->tused[k] && c->tused[k + 1]) (void)hipEventElapsedTime(&t, c->tev[k], c->tev[k + 1]);
+                ms[8 * f + s] = t;
+            }
+        }
+        if (bytes)
+            frame_bytes((double)c->T, (double)c->V, bo, fo, (double)L[0], fi0, fe0, (double)L[3], (double)L[4],
+                        (double)L[1], (double)L[2], (double)c->trestr[f], bytes + 8 * f);
+        if (ops) { ops[2 * f] = bo; ops[2 * f + 1] = fo; }
+        fi0 = (double)L[3];
+        fe0 = (double)L[4];
+    }
+    return BVHCD_OK;
+}
+
+int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
+    if (!c || !out) return BVHCD_E_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    uint32_t sz[4] = {0, 0, 0, 0}, ctr[16];
+    uint64_t c64[2];
+    HIPCHK(c, hipMemcpyAsync(sz, c->d.fsize, sizeof(sz), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c64, ctr64(c->d.ctr), sizeof(c64), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out->frame = c->frame;
+    out->last_bvh_op = c->last_bvh;
+    out->last_front_op = c->last_front;
+    out->int_front = sz[2 * c->d.cur];
+    out->ext_front = sz[2 * c->d.cur + 1];
+    out->bp_pairs = ctr[C_BP];
+    out->np_pairs = ctr[C_NP];
+    out->required_int_front = out->int_front;
+    out->required_ext_front = out->ext_front;
+    out->required_pairs = ctr[C_BP];
+    out->node_visits = (int64_t)c64[C64_VISITS];
+    out->restructured_subtrees = c->restr_sub;
+    out->restructured_leaves = c->restr_leaf;
+    if (c->tcap > 0 && c->tn > 0) {
+        const int64_t f = c->tn - 1;
+        for (int s = 0; s < 8; ++s) {
+            const int64_t k = f * kEv + 2 * s;
+            float t = 0;
+            if (c->tused[k] && c->tused[k + 1]) (void)hipEventElapsedTime(&t, c->tev[k], c->tev[k + 1]);
+            out->stage_ms[s] = t;
+        }
+    }
+    frame_bytes((double)c->T, (double)c->V, c->last_bvh, c->last_front, (double)c64[C64_VISITS], 0, 0,
+                (double)out->int_front, (double)out->ext_front, (double)out->bp_pairs, (double)out->np_pairs,
+                (double)c->restr_leaf, out->stage_bytes);
+    if (ctr[C_OVF]) {
+        if (ctr[C_OVF] & 16u) return fail(c, BVHCD_E_DEVICE, "traversal step bound exceeded (corrupted tree)");
+        out->required_int_front = std::max<int64_t>(out->int_front, c->d.front_cap[0] + 1);
+        return fail(c, BVHCD_E_CAPACITY, "capacity exceeded");
+    }
+    return BVHCD_OK;
+}
+
+int bvhcd_export_scene_box(bvhcd_ctx* c, float* box6) {
+    if (!c || !box6) return BVHCD_E_INVALID;
+    HIPCHK(c, hipMemcpyAsync(box6, c->d.scene_box, 6 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BVHCD_OK;
+}
+
+int bvhcd_export_bvh(bvhcd_ctx* c, uint32_t* codes, int32_t* prim_idx, int32_t* prim_map, int32_t* int_lc,
+                     int32_t* int_rc, int32_t* int_par, uint32_t* int_mark, int32_t* int_rx, int32_t* int_ry,
+                     int32_t* leaf_par, uint32_t* leaf_mark, int32_t* leaf_lca, float* int_box, float* leaf_box) {
+    if (!c) return BVHCD_E_INVALID;
+    if (!c->built) return fail(c, BVHCD_E_NOT_BUILT, "no BVH");
+    const int64_t n = c->T, I = n - 1;
+    DevState& d = c->d;
+    hipStream_t s = c->stream;
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        if (!dst || bytes == 0) return hipSuccess;
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    };
+    HIPCHK(c, cp(codes, d.codes, 4 * n));
+    HIPCHK(c, cp(prim_idx, d.prim_idx, 4 * n));
+    HIPCHK(c, cp(prim_map, d.prim_map, 4 * n));
+    HIPCHK(c, cp(int_lc, d.int_lc, 4 * I));
+    HIPCHK(c, cp(int_rc, d.int_rc, 4 * I));
+    HIPCHK(c, cp(int_par, d.int_par, 4 * I));
+    HIPCHK(c, cp(int_mark, d.int_mark, 4 * I));
+    HIPCHK(c, cp(int_rx, d.int_rx, 4 * I));
+    HIPCHK(c, cp(int_ry, d.int_ry, 4 * I));
+    HIPCHK(c, cp(leaf_par, d.leaf_par, 4 * n));
+    HIPCHK(c, cp(leaf_mark, d.leaf_mark, 4 * n));
+    HIPCHK(c, cp(leaf_lca, d.leaf_lca, 4 * (n + 1)));
+    std::vector<NodeRec> tmp;
+    auto boxes = [&](float* dst, const NodeRec* src, int64_t m) -> int {
+        if (!dst || m <= 0) return BVHCD_OK;
+        tmp.resize(m);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), src, sizeof(NodeRec) * m, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int64_t i = 0; i < m; ++i) {
+            dst[6 * i + 0] = tmp[i].lo.x; dst[6 * i + 1] = tmp[i].lo.y; dst[6 * i + 2] = tmp[i].lo.z;
+            dst[6 * i + 3] = tmp[i].hi.x; dst[6 * i + 4] = tmp[i].hi.y; dst[6 * i + 5] = tmp[i].hi.z;
+        }
+        return BVHCD_OK;
+    };
+    int st = boxes(int_box, d.inode, I);
+    if (st) return st;
+    st = boxes(leaf_box, d.leaf, n);
+    if (st) return st;
+    HIPCHK(c, hipStreamSynchronize(s));
+    return BVHCD_OK;
+}
+
+int bvhcd_export_front(bvhcd_ctx* c, int which, int32_t* out, int64_t cap, int64_t* num) {
+    if (!c || !num || which < 0 || which > 1) return BVHCD_E_INVALID;
+    uint32_t sz[4];
+    HIPCHK(c, hipMemcpyAsync(sz, c->d.fsize, sizeof(sz), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t n = sz[2 * c->d.cur + which];
+    *num = n;
+    if (out && cap > 0) {
+        const int64_t k = std::min<int64_t>(std::min<int64_t>(cap, n), c->d.front_cap[which]);
+        if (k > 0)
+            HIPCHK(c, hipMemcpyAsync(out, c->d.front[c->d.cur][which], sizeof(int2) * k, hipMemcpyDeviceToHost,
+                                     c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return check_overflow(c);
+}
+
+// ---- learned scheduler entry points (learned.hip holds the kernels)
+int bvhcd_mlp_set_weights(bvhcd_ctx* c, const float* p, int64_t n) {
+    if (!c || !p || n != BVHCD_MLP_NPARAM) return BVHCD_E_INVALID;
+    if (!c->learned) c->learned = learned_create(c->stream);
+    if (!c->learned) return fail(c, BVHCD_E_OOM, "learned state");
+    return learned_set_weights(c->learned, p, c->stream, &c->err);
+}
+int bvhcd_mlp_get_weights(bvhcd_ctx* c, float* p, int64_t n) {
+    if (!c || !p || n != BVHCD_MLP_NPARAM) return BVHCD_E_INVALID;
+    if (!c->learned) return fail(c, BVHCD_E_INVALID, "no weights");
+    return learned_get_weights(c->learned, p, c->stream, &c->err);
+}
+int bvhcd_mlp_features(bvhcd_ctx* c, float* feats, int32_t* roots, int64_t cap, int64_t* rows) {
+    if (!c || !rows) return BVHCD_E_INVALID;
+    if (!c->built || !c->front_valid) return fail(c, BVHCD_E_NOT_BUILT, "features need a BVH and a front");
+    if (!c->learned) c->learned = learned_create(c->stream);
+    if (!c->learned) return fail(c, BVHCD_E_OOM, "learned state");
+    return learned_features(c->learned, c->d, (int32_t)c->T, c->frame, feats, roots, cap, rows, c->stream, &c->err);
+}
+int bvhcd_mlp_forward(bvhcd_ctx* c, const float* feats, int64_t rows, float* out, int where) {
+    if (!c || !feats || !out || rows < 0) return BVHCD_E_INVALID;
+    if (!c->learned) return fail(c, BVHCD_E_INVALID, "no weights");
+    return learned_forward(c->learned, feats, rows, out, where, c->stream, &c->err);
+}
+int bvhcd_mlp_train_step(bvhcd_ctx* c, const float* feats, const float* targets, int64_t rows, float lr, float* loss,
+                         int where) {
+    if (!c || !feats || !targets || rows <= 0) return BVHCD_E_INVALID;
+    if (!c->learned) return fail(c, BVHCD_E_INVALID, "no weights");
+    return learned_train_step(c->learned, feats, targets, rows, lr, loss, where, c->stream, &c->err);
+}
+int bvhcd_mlp_grad(bvhcd_ctx* c, const float* feats, const float* targets, int64_t rows, float* grad, float* loss,
+                   int where) {
+    if (!c || !feats || !targets || !grad || rows <= 0) return BVHCD_E_INVALID;
+    if (!c->learned) return fail(c, BVHCD_E_INVALID, "no weights");
+    return learned_grad(c->learned, feats, targets, rows, grad, loss, where, c->stream, &c->err);
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// Internal declarations of the gfx950 collision-detection pipeline (HIP + C++ host).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/bvhcd.h"
+
+namespace bvhcd {
+
+// ------------------------------------------------------------------ node records (HBM layout)
+// One 32-byte record per node, read with two 16-byte loads by the traversal kernels:
+//   internal node n (preorder): lo = (minx, miny, minz, rangex), hi = (maxx, maxy, maxz, escape)
+//   leaf j (sorted slot):       lo = (minx, miny, minz, parent), hi = (maxx, maxy, maxz, escape)
+// escape = encoded lca of the first leaf after the node's range (lca[rangey + 1], -1 at the end):
+// (internal id << 1) or (leaf << 1 | 1). The w words hold int bits.
+struct alignas(16) NodeRec {
+    float4 lo, hi;
+};
+
+__host__ __device__ inline int32_t f2i(float f) { return __builtin_bit_cast(int32_t, f); }
+__host__ __device__ inline float i2f(int32_t i) { return __builtin_bit_cast(float, i); }
+
+struct Box {
+    float mnx, mny, mnz, mxx, mxy, mxz;
+};
+
+__device__ inline Box empty_box() { return {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX}; }
+__device__ inline void box_add(Box& b, float x, float y, float z) {
+    b.mnx = fminf(b.mnx, x); b.mny = fminf(b.mny, y); b.mnz = fminf(b.mnz, z);
+    b.mxx = fmaxf(b.mxx, x); b.mxy = fmaxf(b.mxy, y); b.mxz = fmaxf(b.mxz, z);
+}
+__device__ inline Box box_merge(const Box& a, const Box& b) {
+    return {fminf(a.mnx, b.mnx), fminf(a.mny, b.mny), fminf(a.mnz, b.mnz),
+            fmaxf(a.mxx, b.mxx), fmaxf(a.mxy, b.mxy), fmaxf(a.mxz, b.mxz)};
+}
+// Closed-interval overlap; false as soon as one axis separates (base/Bv.h:40-45).
+__device__ inline bool box_overlap(const Box& a, float4 lo, float4 hi) {
+    return !(lo.x > a.mxx || hi.x < a.mnx || lo.y > a.mxy || hi.y < a.mny || lo.z > a.mxz || hi.z < a.mnz);
+}
+__device__ inline bool box_overlap(const Box& a, const Box& b) {
+    return !(b.mnx > a.mxx || b.mxx < a.mnx || b.mny > a.mxy || b.mxy < a.mny || b.mnz > a.mxz || b.mxz < a.mnz);
+}
+__device__ inline Box rec_box(const float4& lo, const float4& hi) { return {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z}; }
+
+__device__ inline Box tri_box(const int32_t* __restrict__ faces, const float* __restrict__ verts, int32_t t) {
+    const int32_t a = faces[3 * (int64_t)t], b = faces[3 * (int64_t)t + 1], c = faces[3 * (int64_t)t + 2];
+    Box bx = empty_box();
+    box_add(bx, verts[3 * (int64_t)a], verts[3 * (int64_t)a + 1], verts[3 * (int64_t)a + 2]);
+    box_add(bx, verts[3 * (int64_t)b], verts[3 * (int64_t)b + 1], verts[3 * (int64_t)b + 2]);
+    box_add(bx, verts[3 * (int64_t)c], verts[3 * (int64_t)c + 1], verts[3 * (int64_t)c + 2]);
+    return bx;
+}
+
+// ------------------------------------------------------------------ Morton (30-bit)
+__host__ __device__ inline uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+__device__ inline uint32_t morton3d(float x, float y, float z) {
+    x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+    y = fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
+    z = fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
+    return expand_bits((uint32_t)x) * 4u + expand_bits((uint32_t)y) * 2u + expand_bits((uint32_t)z);
+}
+// Centre = float sum halved exactly; offset / extent with IEEE division (base/Bv.h:65-67).
+__device__ inline uint32_t box_code(const Box& b, const float* __restrict__ scene) {
+    const float cx = (float)((double)(b.mnx + b.mxx) * 0.5);
+    const float cy = (float)((double)(b.mny + b.mxy) * 0.5);
+    const float cz = (float)((double)(b.mnz + b.mxz) * 0.5);
+    const float qx = (cx - scene[0]) / (scene[3] - scene[0]);
+    const float qy = (cy - scene[1]) / (scene[4] - scene[1]);
+    const float qz = (cz - scene[2]) / (scene[5] - scene[2]);
+    return morton3d(qx, qy, qz);
+}
+
+// ------------------------------------------------------------------ wave64 helpers
+__device__ inline int lane_id() { return __lane_id(); }
+// Wave-aggregated append: returns this lane's slot; one atomic per wave.
+__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (!pred) return 0;
+    const int lane = lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// ------------------------------------------------------------------ device buffers
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+// Per-context device state. All arrays are owned by the context.
+struct DevState {
+    int64_t maxT = 0, maxV = 0;
+    // inputs
+    int32_t* faces = nullptr;   // 3T
+    float* verts = nullptr;     // 3V
+    // BVH
+    uint32_t* codes = nullptr;      // T sorted codes
+    uint32_t* keys_tmp = nullptr;   // T
+    uint32_t* keys_tmp2 = nullptr;  // T
+    int32_t* prim_idx = nullptr;    // T slot -> triangle
+    int32_t* vals_tmp = nullptr;    // T
+    int32_t* prim_map = nullptr;    // T triangle -> slot
+    NodeRec* inode = nullptr;       // T-1 (preorder)
+    NodeRec* leaf = nullptr;        // T
+    int32_t* int_lc = nullptr;      // T-1
+    int32_t* int_rc = nullptr;
+    int32_t* int_par = nullptr;
+    uint32_t* int_mark = nullptr;
+    int32_t* int_rx = nullptr;
+    int32_t* int_ry = nullptr;
+    uint32_t* int_flag = nullptr;   // T-1 refit arrival counters
+    int32_t* leaf_par = nullptr;    // T
+    uint32_t* leaf_mark = nullptr;  // T
+    int32_t* leaf_lca = nullptr;    // T+1
+    // emission scratch (split-id space)
+    int32_t* s_a = nullptr;         // T range start
+    int32_t* s_b = nullptr;         // T range end
+    int32_t* s_par = nullptr;       // T parent split id (-1 root); leaves use s_lpar
+    int32_t* s_lc = nullptr;        // T child (id << 1 | is_leaf)
+    int32_t* s_rc = nullptr;
+    int32_t* s_lpar = nullptr;      // T leaf parent split id (sign bit clear), right-child flag in bit 30
+    int32_t* s_top = nullptr;       // T topmost split node with leftmost leaf x (-1 none)
+    uint32_t* depths = nullptr;     // T
+    uint32_t* offsets = nullptr;    // T+1
+    int32_t* tkmap = nullptr;       // T split id -> preorder id
+    // scan / sort scratch
+    uint32_t* scan_tmp = nullptr;   // block sums
+    size_t scan_tmp_n = 0;
+    uint32_t* sort_hist = nullptr;  // digit histograms
+    size_t sort_hist_n = 0;
+    float* scene_box = nullptr;     // 6 floats (+ pad)
+    uint32_t* scene_box_u = nullptr;// ordered-int reduction scratch, 6 words
+    // fronts (double buffered) + logs
+    int2* front[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buf][int/ext]
+    int64_t front_cap[2] = {0, 0};  // int, ext
+    int cur = 0;                    // current buffer index
+    uint32_t* fsize = nullptr;      // [buf*2 + which] sizes (4 words)
+    uint32_t* intcnt = nullptr;     // T
+    uint32_t* extcnt = nullptr;     // T+1
+    uint32_t* intbeg = nullptr;     // T+1
+    uint32_t* extbeg = nullptr;     // T+2
+    uint32_t* intback = nullptr;    // T
+    uint32_t* cursor = nullptr;     // T (REORDER front cursor)
+    float* snapshot = nullptr;      // T
+    int32_t* intrange = nullptr;    // T+1
+    int32_t* rtroot = nullptr;      // T
+    int32_t* intmark = nullptr;     // T
+    // restructure scratch
+    int32_t* rt_of_leaf = nullptr;  // T+1
+    int32_t* r_leaf_lo = nullptr;   // T
+    int32_t* r_leaf_hi = nullptr;
+    int32_t* r_split_lo = nullptr;
+    int32_t* r_split_hi = nullptr;
+    int32_t* r_base = nullptr;
+    int32_t* r_keep_par = nullptr;
+    uint32_t* r_keep_mark = nullptr;
+    uint64_t* r_keys = nullptr;     // T
+    uint64_t* r_keys2 = nullptr;    // T
+    // pairs
+    int2* bp = nullptr;             // broad-phase pairs
+    int2* np = nullptr;             // narrow-phase hits
+    int64_t pair_cap = 0;
+    // counters block: see Counter enum
+    uint32_t* ctr = nullptr;        // 32 words
+};
+
+enum Counter : int {
+    C_BP = 0,          // broad-phase pair count
+    C_NP = 1,          // narrow-phase hit count
+    C_OVF = 2,         // overflow bits: 1 int front, 2 ext front, 4 bp pairs, 8 np pairs
+    C_VISIT_LO = 3,    // node tests (low 32 bits, wraps; C_VISIT_HI counts wraps)
+    C_VISIT_HI = 4,
+    C_RT_SUB = 5,      // restructure: subtree count
+    C_RT_INT = 6,      // restructure: internal node count
+    C_RT_LEAF = 7,     // restructure: leaf count
+    C_WORK = 8,        // dynamic work-fetch cursor of the running front kernel
+    C_NUM = 16         // words 16..31 hold 64-bit counters (see Counter64)
+};
+enum Counter64 : int {
+    C64_VISITS = 0,    // node tests of the last step
+    C64_DIGEST = 1,    // pair-set digest
+    C64_NUM = 8
+};
+__host__ __device__ inline uint64_t* ctr64(uint32_t* ctr) { return reinterpret_cast<uint64_t*>(ctr + 16); }
+
+// Launch helpers
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace bvhcd

@@ -1,0 +1,666 @@
+// LBVH construction and refit kernels for gfx950.
+//
+// Reference behaviour (Source/Project/Mine/ of littlemine/BVH-based-Collision-Detection-Scheme):
+//   scene box + Morton          collision/lbvh/LBvhBuildKernels.cu:55-91, utility/CudaDeviceUtils.cu:82-130
+//   sort_by_key + prim map      collision/lbvh/LBvhFixedDeformable.cu:355-363
+//   split metric                collision/lbvh/BvhExtNode.cu:128-134
+//   topology (Apetrei)          collision/lbvh/LBvhBuildKernels.cu:135-172
+//   DFS renumbering             collision/lbvh/LBvhFixedDeformable.cu:365-380, LBvhBuildKernels.cu:174-217
+//   refit                       collision/lbvh/LBvhUpdateKernels.cu:35-74
+//
+// Design here:
+//  * The topology rule (a node [a,b] is the right child of split a-1 if delta(a-1) < delta(b),
+//    else the left child of split b) makes the tree the Cartesian tree of delta with the
+//    leftmost maximum on top. Each split node's range therefore follows from two prefix
+//    searches over the sorted codes (nearest delta >= d on the left, nearest delta > d on
+//    the right), so emission is one independent thread per node: no atomics, no climbing,
+//    no memory-ordering hazards.
+//  * Sorting is an LSD radix sort with 8-bit digits over an LDS-staged tile of 4096 keys:
+//    per-tile digit counts, a device-wide exclusive scan of the digit-major count table,
+//    then a stable wave64 multisplit ranking in LDS and coalesced scatter.
+//  * Refit climbs from the leaves with one agent-scope acquire/release counter per node.
+#include "bvhcd_internal.hpp"
+#include "kernels.hpp"
+
+namespace bvhcd {
+
+// ======================================================================= scene box + codes
+// Order-preserving float <-> uint map for atomic min/max (boxes here never hold NaN:
+// fminf/fmaxf against the FLT_MAX start value discard NaN vertex coordinates).
+__device__ inline uint32_t f2o(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float o2f(uint32_t u) {
+    u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    return __builtin_bit_cast(float, u);
+}
+
+__global__ void k_scene_box_init(uint32_t* box_u) {
+    if (threadIdx.x < 3) box_u[threadIdx.x] = 0xFFFFFFFFu;       // min slots
+    else if (threadIdx.x < 6) box_u[threadIdx.x] = 0u;           // max slots
+}
+
+__global__ void __launch_bounds__(kBlock) k_scene_box(int32_t T, const int32_t* __restrict__ faces,
+                                                      const float* __restrict__ verts, uint32_t* box_u) {
+    Box acc = empty_box();
+    for (int32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
+        acc = box_merge(acc, tri_box(faces, verts, t));
+    // wave reduction
+    float v[6] = {acc.mnx, acc.mny, acc.mnz, acc.mxx, acc.mxy, acc.mxz};
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 6; ++k) {
+            float o = __shfl_down(v[k], off);
+            v[k] = k < 3 ? fminf(v[k], o) : fmaxf(v[k], o);
+        }
+    __shared__ float red[kBlock / 64][6];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0)
+        for (int k = 0; k < 6; ++k) red[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float r = red[0][k];
+        for (int i = 1; i < kBlock / 64; ++i) r = k < 3 ? fminf(r, red[i][k]) : fmaxf(r, red[i][k]);
+        if (k < 3) atomicMin(box_u + k, f2o(r));
+        else atomicMax(box_u + k, f2o(r));
+    }
+}
+
+__global__ void k_scene_box_final(const uint32_t* box_u, float* box) {
+    if (threadIdx.x < 6) box[threadIdx.x] = o2f(box_u[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_codes(int32_t T, const int32_t* __restrict__ faces,
+                                                  const float* __restrict__ verts, const float* __restrict__ scene,
+                                                  uint32_t* __restrict__ codes, int32_t* __restrict__ vals) {
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    codes[t] = box_code(tri_box(faces, verts, t), scene);
+    vals[t] = t;
+}
+
+void launch_scene_box(DevState& d, int32_t T, hipStream_t s) {
+    hipLaunchKernelGGL(k_scene_box_init, dim3(1), dim3(64), 0, s, d.scene_box_u);
+    unsigned g = grid_for(T);
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(k_scene_box, dim3(g), dim3(kBlock), 0, s, T, d.faces, d.verts, d.scene_box_u);
+    hipLaunchKernelGGL(k_scene_box_final, dim3(1), dim3(64), 0, s, d.scene_box_u, d.scene_box);
+}
+
+void launch_codes(DevState& d, int32_t T, uint32_t* codes, int32_t* vals, hipStream_t s) {
+    hipLaunchKernelGGL(k_codes, dim3(grid_for(T)), dim3(kBlock), 0, s, T, d.faces, d.verts, d.scene_box, codes, vals);
+}
+
+// ======================================================================= exclusive scan (u32)
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;  // 4096
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ inline uint32_t block_exscan(uint32_t v, uint32_t& excl) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t wpre = 0, total = 0;
+    for (int i = 0; i < kBlock / 64; ++i) {
+        uint32_t s = wsum[i];
+        if (i < w) wpre += s;
+        total += s;
+    }
+    __syncthreads();
+    excl = wpre + x - v;
+    return total;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_sums(const uint32_t* __restrict__ in, int64_t n, uint32_t* sums) {
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        int64_t j = base + (int64_t)i * kBlock + threadIdx.x;
+        if (j < n) s += in[j];
+    }
+    uint32_t ex;
+    uint32_t tot = block_exscan(s, ex);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Single-block exclusive scan of m block sums (in place); optional total.
+__global__ void __launch_bounds__(kBlock) k_scan_single(uint32_t* sums, int64_t m, uint32_t* total_out) {
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < m; base += kBlock) {
+        int64_t j = base + threadIdx.x;
+        uint32_t v = j < m ? sums[j] : 0u;
+        uint32_t ex;
+        uint32_t tot = block_exscan(v, ex);
+        if (j < m) sums[j] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+// `in` may equal `out` (in-place scan): every thread reads its items before the barrier in
+// block_exscan and writes them after it.
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const uint32_t* in, int64_t n, const uint32_t* __restrict__ sums,
+                                                       uint32_t* out) {
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        int64_t j = base + i;
+        v[i] = j < n ? in[j] : 0u;
+        s += v[i];
+    }
+    uint32_t ex;
+    block_exscan(s, ex);
+    uint32_t run = sums[blockIdx.x] + ex;
+    for (int i = 0; i < kScanItems; ++i) {
+        int64_t j = base + i;
+        if (j < n) out[j] = run;
+        run += v[i];
+    }
+}
+
+// out[0..n-1] = exclusive scan of in[0..n-1]; if total != nullptr, *total = sum (device).
+void launch_exscan(DevState& d, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t blocks = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, n, d.scan_tmp);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, s, d.scan_tmp, blocks, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, n, d.scan_tmp, out);
+}
+
+// ======================================================================= radix sort
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems;  // 4096 keys per tile
+constexpr int kRadix = 256;
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_sort_hist(const K* __restrict__ keys, int64_t n, int shift,
+                                                      uint32_t* __restrict__ hist, int64_t tiles) {
+    __shared__ uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    for (int i = 0; i < kSortItems; ++i) {
+        int64_t j = base + (int64_t)i * kBlock + threadIdx.x;
+        if (j < n) atomicAdd(&h[(uint32_t)(keys[j] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one 4096-key tile. Items are ranked in index order (round-major, then
+// wave, then lane) with a wave64 multisplit: 8 ballots give the peers with an equal digit.
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_sort_scatter(const K* __restrict__ keys, const int32_t* __restrict__ vals,
+                                                         int64_t n, int shift, const uint32_t* __restrict__ hist,
+                                                         int64_t tiles, K* __restrict__ okeys, int32_t* __restrict__ ovals) {
+    __shared__ uint32_t tile_start[kRadix];   // exclusive scan of the tile's digit counts
+    __shared__ uint32_t gofs[kRadix];         // global offset of the tile's first key per digit
+    __shared__ uint32_t run[kRadix];          // per-digit running count over rounds
+    __shared__ uint32_t whist[kBlock / 64][kRadix];
+    __shared__ K skeys[kSortTile];
+    __shared__ int32_t svals[kSortTile];
+    __shared__ uint8_t sdig[kSortTile];
+
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const int valid = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
+
+    // tile digit counts
+    run[tid] = 0;
+    __syncthreads();
+    for (int i = 0; i < kSortItems; ++i) {
+        int j = i * kBlock + tid;
+        if (j < valid) atomicAdd(&run[(uint32_t)(keys[base + j] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    {
+        uint32_t c = run[tid], ex;
+        block_exscan(c, ex);
+        tile_start[tid] = ex;
+        gofs[tid] = hist[(int64_t)tid * tiles + blockIdx.x];
+        run[tid] = 0;
+    }
+    __syncthreads();
+
+    for (int r = 0; r < kSortItems; ++r) {
+        const int j = r * kBlock + tid;
+        const bool ok = j < valid;
+        K key = ok ? keys[base + j] : (K)0;
+        const uint32_t dig = ok ? (uint32_t)(key >> shift) & 0xFFu : 0xFFFFFFFFu;
+        // peers with the same digit among valid lanes of this wave
+        uint64_t peers = __ballot(ok);
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot(ok && ((dig >> b) & 1u));
+            peers &= ((dig >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+        // first lane of each peer group publishes the group size
+        for (int k = 0; k < kRadix; k += 64) whist[w][k + lane] = 0;
+        __syncthreads();
+        if (ok && rank == 0) whist[w][dig] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {   // per digit: prefix over waves, then advance the running count
+            uint32_t acc = run[tid];
+            for (int ww = 0; ww < kBlock / 64; ++ww) {
+                uint32_t c = whist[ww][tid];
+                whist[ww][tid] = acc;
+                acc += c;
+            }
+            run[tid] = acc;
+        }
+        __syncthreads();
+        if (ok) {
+            const uint32_t pos = tile_start[dig] + whist[w][dig] + rank;
+            skeys[pos] = key;
+            svals[pos] = vals[base + j];
+            sdig[pos] = (uint8_t)dig;
+        }
+        __syncthreads();
+    }
+    // coalesced write-out in tile-sorted order
+    for (int i = 0; i < kSortItems; ++i) {
+        const int j = i * kBlock + tid;
+        if (j < valid) {
+            const uint32_t dg = sdig[j];
+            const int64_t o = (int64_t)gofs[dg] + (j - (int64_t)tile_start[dg]);
+            okeys[o] = skeys[j];
+            ovals[o] = svals[j];
+        }
+    }
+}
+
+// Stable LSD sort of (keys, vals) over bits [0, nbits). Ping-pongs between the given buffers;
+// the result ends in (keys, vals) (an even number of passes is arranged by copying if needed).
+template <class K>
+void radix_sort_impl(DevState& d, K* keys, int32_t* vals, K* tkeys, int32_t* tvals, int64_t n, int nbits,
+                     hipStream_t s) {
+    if (n <= 1) return;
+    const int64_t tiles = (n + kSortTile - 1) / kSortTile;
+    const int passes = (nbits + 7) / 8;
+    K* ik = keys; int32_t* iv = vals;
+    K* ok = tkeys; int32_t* ov = tvals;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_sort_hist<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ik, n, shift, d.sort_hist, tiles);
+        launch_exscan(d, d.sort_hist, d.sort_hist, (int64_t)kRadix * tiles, nullptr, s);
+        hipLaunchKernelGGL(k_sort_scatter<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ik, iv, n, shift, d.sort_hist,
+                           tiles, ok, ov);
+        std::swap(ik, ok);
+        std::swap(iv, ov);
+    }
+    if (ik != keys) {
+        hipMemcpyAsync(keys, ik, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(vals, iv, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
+    }
+}
+
+void radix_sort_u32(DevState& d, uint32_t* keys, int32_t* vals, uint32_t* tkeys, int32_t* tvals, int64_t n, int nbits,
+                    hipStream_t s) {
+    radix_sort_impl<uint32_t>(d, keys, vals, tkeys, tvals, n, nbits, s);
+}
+void radix_sort_u64(DevState& d, uint64_t* keys, int32_t* vals, uint64_t* tkeys, int32_t* tvals, int64_t n, int nbits,
+                    hipStream_t s) {
+    radix_sort_impl<uint64_t>(d, keys, vals, tkeys, tvals, n, nbits, s);
+}
+
+// ======================================================================= emission
+__device__ inline int delta_at(const uint32_t* __restrict__ c, int32_t i, int32_t n) {
+    if (i >= n - 1) return 33;
+    const uint32_t x = c[i] ^ c[i + 1];
+    return x ? 32 - __clz(x) : 0;
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+    if (n <= 0) return;
+    unsigned g = grid_for(n);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(kBlock), 0, s, p, n, v);
+}
+void fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+    if (n <= 0) return;
+    unsigned g = grid_for(n);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, n, v);
+}
+
+// Split node k of the sorted run [lo, hi] (leaves lo..hi; splits lo..hi-1). Boundaries of the
+// run act as delta = +inf, so a run is built exactly as a standalone tree over its leaves.
+// Range [a, b]: a-1 = nearest split left of k with delta >= d (a = lo if none); b = nearest
+// split right of k with delta > d (b = hi if none).
+__device__ inline void split_range(const uint32_t* __restrict__ c, int32_t k, int32_t lo, int32_t hi, int d,
+                                   int32_t& a, int32_t& b) {
+    // a: smallest index in [lo, k] whose code agrees with c[k] on bits >= d-1 (d >= 1).
+    if (d == 0) {
+        a = k;
+    } else {
+        const uint32_t ck = c[k];
+        const int sh = d - 1;
+        int32_t L = lo, R = k;  // answer in [L, R]
+        while (L < R) {
+            const int32_t m = L + ((R - L) >> 1);
+            if (((c[m] ^ ck) >> sh) == 0u) R = m;
+            else L = m + 1;
+        }
+        a = L;
+    }
+    // b: largest index in [k+1, hi] whose code agrees with c[k+1] on bits >= d.
+    {
+        const uint32_t ck1 = c[k + 1];
+        int32_t L = k + 1, R = hi;
+        if (d >= 32) {
+            L = hi;
+        } else {
+            while (L < R) {
+                const int32_t m = L + ((R - L + 1) >> 1);
+                if (((c[m] ^ ck1) >> d) == 0u) L = m;
+                else R = m - 1;
+            }
+        }
+        b = L;
+    }
+}
+
+// One thread per split k of the sorted codes; segments: runs [seg_lo, seg_hi] given per split
+// through seg_of (nullptr = one run [0, n-1]).
+__global__ void __launch_bounds__(kBlock) k_emit_internal(int32_t n, const uint32_t* __restrict__ c,
+                                                          const int32_t* __restrict__ run_lo,
+                                                          const int32_t* __restrict__ run_hi,
+                                                          int32_t* __restrict__ s_a, int32_t* __restrict__ s_b,
+                                                          int32_t* __restrict__ s_par, int32_t* __restrict__ s_lc,
+                                                          int32_t* __restrict__ s_rc, int32_t* __restrict__ s_top,
+                                                          uint32_t* __restrict__ depths) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n - 1) return;
+    const int32_t lo = run_lo ? run_lo[k] : 0;
+    const int32_t hi = run_hi ? run_hi[k] : n - 1;
+    if (lo < 0) return;  // split not inside a run being (re)built
+    const int d = delta_at(c, k, n);
+    int32_t a, b;
+    split_range(c, k, lo, hi, d, a, b);
+    s_a[k] = a;
+    s_b[k] = b;
+    atomicAdd(&depths[a], 1u);
+    if (a == lo && b == hi) {  // run root
+        s_par[k] = -1;
+        s_top[a] = k;
+        return;
+    }
+    const int dl = a > lo ? delta_at(c, a - 1, n) : 1 << 20;
+    const int dr = b < hi ? delta_at(c, b, n) : 1 << 20;
+    if (a > lo && dl < dr) {  // right child of split a-1
+        s_par[k] = (a - 1) | (1 << 30);
+        s_rc[a - 1] = k << 1;
+        s_top[a] = k;
+    } else {                  // left child of split b
+        s_par[k] = b;
+        s_lc[b] = k << 1;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_emit_leaves(int32_t n, const uint32_t* __restrict__ c,
+                                                        const int32_t* __restrict__ leaf_lo,
+                                                        const int32_t* __restrict__ leaf_hi,
+                                                        int32_t* __restrict__ s_lpar, int32_t* __restrict__ s_lc,
+                                                        int32_t* __restrict__ s_rc) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t lo = leaf_lo ? leaf_lo[i] : 0;
+    const int32_t hi = leaf_hi ? leaf_hi[i] : n - 1;
+    if (lo < 0) return;
+    const int dl = i > lo ? delta_at(c, i - 1, n) : 1 << 20;
+    const int dr = i < hi ? delta_at(c, i, n) : 1 << 20;
+    if (i > lo && dl < dr) {
+        s_lpar[i] = (i - 1) | (1 << 30);
+        s_rc[i - 1] = (i << 1) | 1;
+    } else {
+        s_lpar[i] = i;
+        s_lc[i] = (i << 1) | 1;
+    }
+}
+
+// Preorder ids: the internal nodes whose leftmost leaf is x form a left spine starting at
+// s_top[x]; they get ids base + offsets[x] .. in top-down order.
+__global__ void __launch_bounds__(kBlock) k_assign_preorder(int32_t n, const int32_t* __restrict__ s_top,
+                                                            const uint32_t* __restrict__ depths,
+                                                            const uint32_t* __restrict__ offsets,
+                                                            const int32_t* __restrict__ s_lc,
+                                                            const int32_t* __restrict__ leaf_base,
+                                                            int32_t* __restrict__ tkmap) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    int32_t node = s_top[x];
+    if (node < 0) return;
+    int32_t id = (int32_t)offsets[x] + (leaf_base ? leaf_base[x] : 0);
+    for (uint32_t dd = depths[x]; dd > 0; --dd) {
+        tkmap[node] = id++;
+        node = s_lc[node] >> 1;
+    }
+}
+
+__device__ inline int32_t enc_lca(int32_t x, int32_t n, const int32_t* __restrict__ s_top,
+                                  const int32_t* __restrict__ tkmap) {
+    if (x >= n) return -1;
+    const int32_t t = s_top[x];
+    return t >= 0 ? tkmap[t] << 1 : (x << 1) | 1;
+}
+
+// Writes preorder SoA topology and the w words of the node records.
+__global__ void __launch_bounds__(kBlock) k_write_internal(int32_t n, const int32_t* __restrict__ run_lo,
+                                                           const int32_t* __restrict__ s_a, const int32_t* __restrict__ s_b,
+                                                           const int32_t* __restrict__ s_par,
+                                                           const int32_t* __restrict__ s_lc, const int32_t* __restrict__ s_rc,
+                                                           const int32_t* __restrict__ s_top, const int32_t* __restrict__ tkmap,
+                                                           const int32_t* __restrict__ keep_par,
+                                                           const uint32_t* __restrict__ keep_mark, DevState d) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n - 1) return;
+    if (run_lo && run_lo[k] < 0) return;
+    const int32_t id = tkmap[k];
+    const int32_t lce = s_lc[k], rce = s_rc[k];
+    const bool lleaf = lce & 1, rleaf = rce & 1;
+    const int32_t lc = lleaf ? (lce >> 1) : tkmap[lce >> 1];
+    const int32_t rc = rleaf ? (rce >> 1) : tkmap[rce >> 1];
+    const int32_t sp = s_par[k];
+    uint32_t mark = (lleaf ? 1u : 0u) | (rleaf ? 2u : 0u);
+    int32_t par;
+    if (sp == -1) {  // run root: keeps the parent link and right-child bit it had
+        par = keep_par ? keep_par[id] : -1;
+        mark |= keep_mark ? (keep_mark[id] & 4u) : 0u;
+    } else {
+        par = tkmap[sp & ~(1 << 30)];
+        if (sp & (1 << 30)) mark |= 4u;
+    }
+    d.int_lc[id] = lc;
+    d.int_rc[id] = rc;
+    d.int_par[id] = par;
+    d.int_mark[id] = mark;
+    d.int_rx[id] = s_a[k];
+    d.int_ry[id] = s_b[k];
+    d.inode[id].lo.w = i2f(s_a[k]);
+    d.inode[id].hi.w = i2f(enc_lca(s_b[k] + 1, n, s_top, tkmap));
+}
+
+__global__ void __launch_bounds__(kBlock) k_write_leaves(int32_t n, const int32_t* __restrict__ leaf_lo,
+                                                         const int32_t* __restrict__ s_lpar,
+                                                         const int32_t* __restrict__ s_top,
+                                                         const int32_t* __restrict__ tkmap, DevState d) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    if (leaf_lo && leaf_lo[x] < 0) return;
+    const int32_t sp = s_lpar[x];
+    const int32_t par = tkmap[sp & ~(1 << 30)];
+    d.leaf_par[x] = par;
+    d.leaf_mark[x] = (sp & (1 << 30)) ? 7u : 3u;
+    // the run's first leaf keeps its lca (its topmost spine node may lie above the run)
+    if (!leaf_lo || leaf_lo[x] != x) d.leaf_lca[x] = enc_lca(x, n, s_top, tkmap);
+    d.leaf[x].lo.w = i2f(par);
+    if (!leaf_lo) {
+        d.leaf[x].hi.w = i2f(enc_lca(x + 1, n, s_top, tkmap));
+        if (x == n - 1) d.leaf_lca[n] = -1;
+    }
+}
+
+// escape words of leaves whose successor's lca changed (restructure runs)
+__global__ void k_fix_leaf_escape(int32_t n, const int32_t* __restrict__ leaf_lca, NodeRec* leaf) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    leaf[x].hi.w = i2f(leaf_lca[x + 1]);
+}
+__global__ void k_fix_int_escape(int32_t I, const int32_t* __restrict__ int_ry, const int32_t* __restrict__ leaf_lca,
+                                 NodeRec* inode) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= I) return;
+    inode[k].hi.w = i2f(leaf_lca[int_ry[k] + 1]);
+}
+
+// Full-tree emission from sorted codes d.codes[0..n-1] (n >= 2).
+void launch_emit_full(DevState& d, int32_t n, hipStream_t s) {
+    fill_u32(d.depths, n, 0u, s);
+    fill_i32(d.s_top, n, -1, s);
+    hipLaunchKernelGGL(k_emit_internal, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d.codes, nullptr, nullptr, d.s_a,
+                       d.s_b, d.s_par, d.s_lc, d.s_rc, d.s_top, d.depths);
+    hipLaunchKernelGGL(k_emit_leaves, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.codes, nullptr, nullptr, d.s_lpar,
+                       d.s_lc, d.s_rc);
+    launch_exscan(d, d.depths, d.offsets, n, nullptr, s);
+    hipLaunchKernelGGL(k_assign_preorder, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.s_top, d.depths, d.offsets,
+                       d.s_lc, nullptr, d.tkmap);
+    hipLaunchKernelGGL(k_write_internal, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, nullptr, d.s_a, d.s_b, d.s_par,
+                       d.s_lc, d.s_rc, d.s_top, d.tkmap, nullptr, nullptr, d);
+    hipLaunchKernelGGL(k_write_leaves, dim3(grid_for(n)), dim3(kBlock), 0, s, n, nullptr, d.s_lpar, d.s_top, d.tkmap,
+                       d);
+}
+
+// ======================================================================= restructure emission
+// Runs: leaves of each restructured subtree [a, b]; run_lo/run_hi per leaf (and per split,
+// with -1 outside runs). Preorder ids of run r start at its root id R: base[x] = R - offsets[a].
+__global__ void k_run_bounds(int32_t n, const int32_t* __restrict__ rt_of_leaf, const int32_t* __restrict__ int_rx,
+                             const int32_t* __restrict__ int_ry, int32_t* __restrict__ leaf_lo,
+                             int32_t* __restrict__ leaf_hi, int32_t* __restrict__ split_lo,
+                             int32_t* __restrict__ split_hi) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int32_t r = rt_of_leaf[x];
+    if (r < 0) {
+        leaf_lo[x] = -1; leaf_hi[x] = -1;
+        if (x < n - 1) { split_lo[x] = -1; split_hi[x] = -1; }
+        return;
+    }
+    const int32_t a = int_rx[r], b = int_ry[r];
+    leaf_lo[x] = a; leaf_hi[x] = b;
+    if (x < n - 1) {
+        if (x < b) { split_lo[x] = a; split_hi[x] = b; }
+        else { split_lo[x] = -1; split_hi[x] = -1; }
+    }
+}
+
+__global__ void k_run_base(int32_t n, const int32_t* __restrict__ rt_of_leaf, const int32_t* __restrict__ int_rx,
+                           const uint32_t* __restrict__ offsets, int32_t* __restrict__ base) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int32_t r = rt_of_leaf[x];
+    base[x] = r < 0 ? 0 : r - (int32_t)offsets[int_rx[r]];
+}
+
+// Rebuild the topology inside the runs (codes in d.codes already re-sorted per run).
+// Scratch: s_a/s_b/... as in the full build; run arrays passed in.
+void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t* leaf_lo, int32_t* leaf_hi,
+                      int32_t* split_lo, int32_t* split_hi, int32_t* base, int32_t* keep_par, uint32_t* keep_mark,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_run_bounds, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rt_of_leaf, d.int_rx, d.int_ry, leaf_lo,
+                       leaf_hi, split_lo, split_hi);
+    fill_u32(d.depths, n, 0u, s);
+    fill_i32(d.s_top, n, -1, s);
+    hipLaunchKernelGGL(k_emit_internal, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d.codes, split_lo, split_hi,
+                       d.s_a, d.s_b, d.s_par, d.s_lc, d.s_rc, d.s_top, d.depths);
+    hipLaunchKernelGGL(k_emit_leaves, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.codes, leaf_lo, leaf_hi, d.s_lpar,
+                       d.s_lc, d.s_rc);
+    launch_exscan(d, d.depths, d.offsets, n, nullptr, s);
+    hipLaunchKernelGGL(k_run_base, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rt_of_leaf, d.int_rx, d.offsets, base);
+    // keep_par / keep_mark: copies of int_par / int_mark before rewriting
+    hipMemcpyAsync(keep_par, d.int_par, sizeof(int32_t) * (n - 1), hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(keep_mark, d.int_mark, sizeof(uint32_t) * (n - 1), hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL(k_assign_preorder, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.s_top, d.depths, d.offsets,
+                       d.s_lc, base, d.tkmap);
+    hipLaunchKernelGGL(k_write_internal, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, split_lo, d.s_a, d.s_b, d.s_par,
+                       d.s_lc, d.s_rc, d.s_top, d.tkmap, keep_par, keep_mark, d);
+    hipLaunchKernelGGL(k_write_leaves, dim3(grid_for(n)), dim3(kBlock), 0, s, n, leaf_lo, d.s_lpar, d.s_top, d.tkmap,
+                       d);
+    hipLaunchKernelGGL(k_fix_leaf_escape, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.leaf_lca, d.leaf);
+    hipLaunchKernelGGL(k_fix_int_escape, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n - 1, d.int_ry, d.leaf_lca,
+                       d.inode);
+}
+
+// ======================================================================= refit
+// Leaf boxes from the triangles, then a bottom-up climb: the second thread to reach a node
+// (agent-scope acq_rel counter) merges both children and continues upwards.
+__global__ void __launch_bounds__(kBlock) k_refit(int32_t n, const int32_t* __restrict__ faces,
+                                                  const float* __restrict__ verts, const int32_t* __restrict__ prim_idx,
+                                                  NodeRec* __restrict__ leaf, NodeRec* __restrict__ inode,
+                                                  const int32_t* __restrict__ leaf_par,
+                                                  const int32_t* __restrict__ int_par,
+                                                  const int32_t* __restrict__ int_lc, const int32_t* __restrict__ int_rc,
+                                                  const uint32_t* __restrict__ int_mark, uint32_t* __restrict__ flag) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    Box b = tri_box(faces, verts, prim_idx[x]);
+    float4* lp = reinterpret_cast<float4*>(&leaf[x]);
+    lp[0].x = b.mnx; lp[0].y = b.mny; lp[0].z = b.mnz;
+    lp[1].x = b.mxx; lp[1].y = b.mxy; lp[1].z = b.mxz;
+    if (n < 2) return;
+    int32_t p = leaf_par[x];
+    int32_t self = x;
+    bool self_leaf = true;
+    for (int32_t guard = 0; p >= 0 && guard < n; ++guard) {
+        const uint32_t old = __hip_atomic_fetch_add(&flag[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u) break;
+        const uint32_t m = int_mark[p];
+        const int32_t l = int_lc[p], r = int_rc[p];
+        // sibling of `self`
+        const bool self_is_left = ((m & 1u) != 0) == self_leaf && l == self;
+        const int32_t sib = self_is_left ? r : l;
+        const bool sib_leaf = self_is_left ? (m & 2u) : (m & 1u);
+        const NodeRec* sr = sib_leaf ? &leaf[sib] : &inode[sib];
+        const float4 slo = sr->lo, shi = sr->hi;
+        b = box_merge(b, rec_box(slo, shi));
+        float4* ip = reinterpret_cast<float4*>(&inode[p]);
+        ip[0].x = b.mnx; ip[0].y = b.mny; ip[0].z = b.mnz;
+        ip[1].x = b.mxx; ip[1].y = b.mxy; ip[1].z = b.mxz;
+        self = p;
+        self_leaf = false;
+        p = int_par[p];
+    }
+}
+
+void launch_refit(DevState& d, int32_t n, hipStream_t s) {
+    if (n >= 2) fill_u32(d.int_flag, n - 1, 0u, s);
+    hipLaunchKernelGGL(k_refit, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.faces, d.verts, d.prim_idx, d.leaf, d.inode,
+                       d.leaf_par, d.int_par, d.int_lc, d.int_rc, d.int_mark, d.int_flag);
+}
+
+// ======================================================================= prim map / misc
+__global__ void k_prim_map(int32_t n, const int32_t* __restrict__ prim_idx, int32_t* __restrict__ prim_map) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    prim_map[prim_idx[s]] = s;
+}
+void launch_prim_map(DevState& d, int32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_prim_map, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.prim_idx, d.prim_map);
+}
+
+}  // namespace bvhcd

@@ -1,0 +1,26 @@
+// Learned restructure scheduler (MLP 16 -> 64 -> 64 -> 1, bf16 weights, fp32 accumulation).
+#pragma once
+#include <string>
+#include "bvhcd_internal.hpp"
+
+namespace bvhcd {
+struct LearnedState;
+LearnedState* learned_create(hipStream_t s);
+void learned_destroy(LearnedState* L);
+int learned_set_weights(LearnedState* L, const float* p, hipStream_t s, std::string* err);
+int learned_get_weights(LearnedState* L, float* p, hipStream_t s, std::string* err);
+int learned_features(LearnedState* L, DevState& d, int32_t n, int64_t frame, float* feats, int32_t* roots, int64_t cap,
+                     int64_t* rows, hipStream_t s, std::string* err);
+int learned_forward(LearnedState* L, const float* feats, int64_t rows, float* out, int where, hipStream_t s,
+                    std::string* err);
+int learned_train_step(LearnedState* L, const float* feats, const float* targets, int64_t rows, float lr, float* loss,
+                       int where, hipStream_t s, std::string* err);
+int learned_grad(LearnedState* L, const float* feats, const float* targets, int64_t rows, float* grad, float* loss,
+                 int where, hipStream_t s, std::string* err);
+// True once weights are loaded.
+bool learned_ready(const LearnedState* L);
+// Selects the subtrees to restructure with the MLP (difference marks in d.r_leaf_lo, as the
+// heuristic selection leaves them); returns subtree / internal-node / leaf counts.
+int learned_select(LearnedState* L, DevState& d, int32_t n, int64_t frame, hipStream_t s, int64_t* nsub,
+                   int64_t* nint, int64_t* nleaf, std::string* err);
+}  // namespace bvhcd

@@ -1,0 +1,229 @@
+/*
+ * bvhcd.h — C ABI of the MI355X (gfx950) BVH / BVTT-front collision-detection pipeline.
+ *
+ * Scene in (faces + per-frame vertices) -> LBVH -> BVTT front -> colliding triangle pairs out.
+ * Plain pointers and sizes only; no C++ or HIP types cross this boundary.
+ *
+ * Reference interfaces each group replaces (reference paths are relative to
+ * Source/Project/Mine/ of littlemine/BVH-based-Collision-Detection-Scheme):
+ *   bvhcd_create / bvhcd_destroy
+ *       LBvhFixedDeformable::LBvhFixedDeformable(LBvhBuildConfig&&)      collision/lbvh/LBvhFixedDeformable.h:30
+ *       BvttFrontLooseIntra::BvttFrontLooseIntra(BvttFrontIntraBuildConfig) collision/bvtt_front/BvttFrontLooseIntra.h:31
+ *       (capacities: setting/BvhSettings.h:49-54, setting/BvttFrontSettings.h:108-119)
+ *   bvhcd_set_topology / bvhcd_update_vertices
+ *       LBvhFixedDeformable::updatePrimData(const SceneData&)            collision/lbvh/LBvhFixedDeformable.cu:115-119
+ *       (SceneData{pos, fids}: world/Scene.h:16-20)
+ *   bvhcd_step(bvh_op, front_op)
+ *       LBvhFixedDeformable::maintain(LBvhFixedDeformableMaintenance, SceneData)  collision/lbvh/LBvhFixedDeformable.cu:68-87
+ *       BvttFrontLooseIntra::maintain(BvttFrontLooseIntraMaintenance)            collision/bvtt_front/BvttFrontLooseIntra.cu:69-126
+ *   bvhcd_step_scheduled
+ *       BenchmarkLogic::maintainScheme()                                  system/Logic/BenchmarkLogic.hpp:90-156
+ *   bvhcd_get_pairs
+ *       BvttFrontLooseIntra::applyCpResults(uint*, uint2*)                collision/bvtt_front/BvttFrontLooseIntra.cu:63-67
+ *   bvhcd_get_stats
+ *       BvttFrontLooseIntra::inspectResults()                             collision/bvtt_front/BvttFrontLooseIntra.cu:50-61
+ *   bvhcd_export_bvh / bvhcd_export_front
+ *       accessors LBvhFixedDeformable::{clvs,ctks,getPrimMap} and the front MultiArray
+ *       (collision/lbvh/LBvhFixedDeformable.h:34-60, base/MultiArray.h:20-161) — exported so that
+ *       callers and tests can check topology and front contents.
+ *   bvhcd_scene_*
+ *       world/Scene.cpp:14-49 (mesh ingest); here seeded synthetic deforming meshes.
+ *
+ * Every call returns 0 on success or a negative BVHCD_E* code; bvhcd_last_error() gives text.
+ * Nothing exits or prints. Capacity overflow is detected on the device before any
+ * out-of-bounds write; the step then fails with BVHCD_E_CAPACITY and bvhcd_get_stats()
+ * reports the required sizes.
+ */
+#ifndef BVHCD_H_
+#define BVHCD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define BVHCD_OK            0
+#define BVHCD_E_INVALID    -1   /* bad argument or call order */
+#define BVHCD_E_OOM        -2   /* device allocation failed */
+#define BVHCD_E_CAPACITY   -3   /* front or pair capacity exceeded (see stats.required_*) */
+#define BVHCD_E_DEVICE     -4   /* HIP runtime error */
+#define BVHCD_E_NOT_BUILT  -5   /* step needs a BVH (or front) that does not exist yet */
+
+/* ---- memory location of caller buffers ---- */
+#define BVHCD_MEM_HOST   0
+#define BVHCD_MEM_DEVICE 1
+
+/* ---- BVH maintenance op (LBvhFixedDeformableMaintenance, LBvhFixedDeformable.h:18-24) ---- */
+#define BVHCD_BVH_BUILD       0
+#define BVHCD_BVH_REFIT       1
+#define BVHCD_BVH_RESTRUCTURE 2   /* restructure the subtrees flagged by the last quality check */
+#define BVHCD_BVH_NONE        3   /* leave the BVH untouched (vertices must be unchanged) */
+
+/* ---- front maintenance op (BvttFrontLooseIntraMaintenance, BvttFrontLooseIntra.h:19-26) ---- */
+#define BVHCD_FRONT_PURE_BVH_CD 0   /* stackless self-CD traversal, no front kept */
+#define BVHCD_FRONT_GENERATE    1
+#define BVHCD_FRONT_UPDATE      2   /* prune + sprout, then quality check and ordering */
+#define BVHCD_FRONT_REORDER     3   /* "balance": keep + re-bucket int front */
+#define BVHCD_FRONT_KEEP        4
+
+/* ---- scheduler (bvhcd_step_scheduled) ---- */
+#define BVHCD_SCHED_PLAIN   0   /* reference STATIC_MANDATORY schedule + restructure heuristic */
+#define BVHCD_SCHED_LEARNED 1   /* MLP per-subtree restructure choice, heuristic fallback */
+#define BVHCD_SCHED_GENERATE 2  /* reference GENERATE scheme: BUILD + PURE_BVH_CD each frame */
+
+typedef struct bvhcd_config {
+    int64_t max_tris;        /* capacity in triangles (>= 1) */
+    int64_t max_verts;       /* capacity in vertices */
+    int64_t int_front_cap;   /* internal-front entries; 0 = derive from free HBM */
+    int64_t ext_front_cap;   /* external-front entries; 0 = derive from free HBM */
+    int64_t pair_cap;        /* broad-phase pair capacity; 0 = derive from free HBM */
+    int32_t narrow_phase;    /* 1 = run the triangle-triangle SAT (CDBenchmarkSettings includeNarrowPhase) */
+    int32_t scheduler;       /* BVHCD_SCHED_* used by bvhcd_step_scheduled */
+    int32_t enable_restructure; /* 1 = quality check + restructure (CDBenchmarkSettings enableRestr) */
+    int32_t bvh_cycle;       /* mandatory rebuild cycle, 0 = 12 (BvhSettings.h:45) */
+    int32_t front_cycle;     /* front cycle, 0 = 6 (BvttFrontSettings.h:105) */
+    int32_t device;          /* HIP device ordinal */
+    void*   stream;          /* hipStream_t to run on; NULL = the library creates one */
+} bvhcd_config;
+
+typedef struct bvhcd_stats {
+    int64_t frame;             /* frames stepped so far */
+    int32_t last_bvh_op;       /* op actually executed on the last step */
+    int32_t last_front_op;
+    int64_t int_front;         /* current internal-front length */
+    int64_t ext_front;         /* current external-front length */
+    int64_t bp_pairs;          /* broad-phase pairs of the last step (cpNum) */
+    int64_t np_pairs;          /* narrow-phase hits of the last step (actualCpNum) */
+    int64_t required_int_front;/* sizes the last step needed (meaningful after BVHCD_E_CAPACITY) */
+    int64_t required_ext_front;
+    int64_t required_pairs;
+    int64_t node_visits;       /* node tests made by traversal / front kernels in the last step */
+    int64_t restructured_subtrees; /* subtrees rebuilt by the last step (restructure) */
+    int64_t restructured_leaves;
+    double  stage_ms[8];       /* last step: 0 build, 1 refit, 2 restructure, 3 front/broad,
+                                  4 ordering + quality, 5 narrow, 6 scheduler/MLP, 7 total */
+    double  stage_bytes[8];    /* algorithmic HBM bytes of those stages in the last step */
+} bvhcd_stats;
+
+typedef struct bvhcd_ctx bvhcd_ctx;
+
+int  bvhcd_create(const bvhcd_config* cfg, bvhcd_ctx** out);
+int  bvhcd_destroy(bvhcd_ctx* ctx);
+const char* bvhcd_last_error(const bvhcd_ctx* ctx);
+
+/* faces: T x int32[3] vertex ids. Topology is fixed until the next call (it resets BVH and front). */
+int  bvhcd_set_topology(bvhcd_ctx* ctx, const int32_t* faces, int64_t num_tris, int64_t num_verts, int where);
+/* xyz: V x float[3]. Copied into the context (host: H2D, device: D2D) on the context stream. */
+int  bvhcd_update_vertices(bvhcd_ctx* ctx, const float* xyz, int64_t num_verts, int where);
+
+/* One frame with explicit ops (LBvh::maintain then BvttFront::maintain). Stream-ordered. */
+int  bvhcd_step(bvhcd_ctx* ctx, int bvh_op, int front_op);
+/* One frame with ops chosen by the configured scheduler. */
+int  bvhcd_step_scheduled(bvhcd_ctx* ctx);
+
+/* Pairs of the last step as (tri_a, tri_b) int32 pairs with tri_a < tri_b, unordered list.
+   filtered=1: narrow-phase hits; 0: broad-phase (AABB) pairs. Synchronises the stream. */
+int  bvhcd_get_pairs(bvhcd_ctx* ctx, int32_t* out, int64_t cap_pairs, int64_t* num_pairs, int where, int filtered);
+/* Order-independent 64-bit digest of the pair set of the last step (sum of a 64-bit mix of
+   each pair, with the count), computed on the device. */
+int  bvhcd_pair_digest(bvhcd_ctx* ctx, int filtered, uint64_t* digest, int64_t* num_pairs);
+int  bvhcd_get_stats(bvhcd_ctx* ctx, bvhcd_stats* out);
+/* Per-stage hipEvent timing: frames > 0 records up to that many frames (events plus a
+   device-side counter log, no host syncs); 0 switches it off. stats.stage_ms then holds the
+   last step's stage times, and bvhcd_timing_frames() returns every recorded frame. */
+int  bvhcd_set_timing(bvhcd_ctx* ctx, int frames);
+/* Recorded frames since bvhcd_set_timing: per frame 8 stage times (ms, layout of
+   bvhcd_stats.stage_ms), 8 algorithmic byte counts (layout of stage_bytes) and the executed
+   (bvh_op, front_op). Synchronises the stream. */
+int  bvhcd_timing_frames(bvhcd_ctx* ctx, int64_t cap, double* ms, double* bytes, int32_t* ops, int64_t* n);
+
+/* Export the current BVH to host arrays (any pointer may be NULL). n = leaves = num_tris.
+   codes/prim_idx/prim_map: n; int_*: n-1 (preorder); leaf_par/leaf_mark: n; leaf_lca: n+1;
+   int_box/leaf_box: 6 floats per node (minx, miny, minz, maxx, maxy, maxz). */
+int  bvhcd_export_bvh(bvhcd_ctx* ctx, uint32_t* codes, int32_t* prim_idx, int32_t* prim_map,
+                      int32_t* int_lc, int32_t* int_rc, int32_t* int_par, uint32_t* int_mark,
+                      int32_t* int_rangex, int32_t* int_rangey,
+                      int32_t* leaf_par, uint32_t* leaf_mark, int32_t* leaf_lca,
+                      float* int_box, float* leaf_box);
+/* Export front entries (query leaf, node) as int32 pairs; which = 0 internal, 1 external. */
+int  bvhcd_export_front(bvhcd_ctx* ctx, int which, int32_t* out, int64_t cap, int64_t* n);
+/* Scene box of the last BUILD (6 floats). */
+int  bvhcd_export_scene_box(bvhcd_ctx* ctx, float* box6);
+
+/* ---- learned restructure scheduler (north-star pattern: MLP 16->64->64->1) ---- */
+#define BVHCD_MLP_IN  16
+#define BVHCD_MLP_H   64
+#define BVHCD_MLP_NPARAM (16*64 + 64 + 64*64 + 64 + 64 + 1)
+/* Load fp32 master weights (packed W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3). */
+int  bvhcd_mlp_set_weights(bvhcd_ctx* ctx, const float* params, int64_t n);
+int  bvhcd_mlp_get_weights(bvhcd_ctx* ctx, float* params, int64_t n);
+/* Features of the candidate subtrees of the current BVH/front: rows x 16 fp32, plus the
+   preorder root of each row. */
+int  bvhcd_mlp_features(bvhcd_ctx* ctx, float* feats, int32_t* roots, int64_t cap, int64_t* rows);
+/* Forward pass on the device over host/device feature rows (bf16 weights, fp32 accumulate). */
+int  bvhcd_mlp_forward(bvhcd_ctx* ctx, const float* feats, int64_t rows, float* out, int where);
+/* One training step (fp32 master weights, bf16 forward, MSE loss, Adam) on device; returns loss. */
+int  bvhcd_mlp_train_step(bvhcd_ctx* ctx, const float* feats, const float* targets, int64_t rows,
+                          float lr, float* loss, int where);
+/* Gradients of the MSE loss for one batch (fp32), for checking against a CPU reference. */
+int  bvhcd_mlp_grad(bvhcd_ctx* ctx, const float* feats, const float* targets, int64_t rows,
+                    float* grad, float* loss, int where);
+
+/* ---- batched scenes (many scenes per launch; one hipGraph per frame) ---- */
+typedef struct bvhcd_batch bvhcd_batch;
+int  bvhcd_batch_create(const bvhcd_config* cfg, int32_t num_scenes, const int64_t* tris_per_scene,
+                        const int64_t* verts_per_scene, bvhcd_batch** out);
+int  bvhcd_batch_destroy(bvhcd_batch* b);
+/* faces: concatenated per-scene faces with scene-local vertex ids. */
+int  bvhcd_batch_set_topology(bvhcd_batch* b, const int32_t* faces, int where);
+/* xyz: concatenated per-scene vertices. */
+int  bvhcd_batch_update_vertices(bvhcd_batch* b, const float* xyz, int where);
+/* One frame for every scene: build or refit + pure traversal + narrow phase, captured in a
+   hipGraph after the first frame (use_graph = 1). */
+int  bvhcd_batch_step(bvhcd_batch* b, int bvh_op, int use_graph);
+/* Pairs of scene s (scene-local triangle ids), narrow-phase filtered. */
+int  bvhcd_batch_get_pairs(bvhcd_batch* b, int32_t scene, int32_t* out, int64_t cap, int64_t* n);
+int  bvhcd_batch_pair_counts(bvhcd_batch* b, int64_t* counts /* num_scenes */);
+const char* bvhcd_batch_last_error(const bvhcd_batch* b);
+
+/* ---- autotuner ---- */
+/* Time candidate block sizes for every tuned kernel on the context's current data and keep
+   the fastest (cached per kernel and size class for the process). */
+int  bvhcd_autotune(bvhcd_ctx* ctx);
+int  bvhcd_autotune_report(char* buf, int64_t cap);
+
+/* ---- seeded synthetic scenes (host) ---- */
+#define BVHCD_SCENE_TWO_SPHERES  0   /* two jittered icospheres approaching each other */
+#define BVHCD_SCENE_CLOTH        1   /* n x n folding sheet */
+#define BVHCD_SCENE_LAYERS       2   /* stacked folding sheets */
+#define BVHCD_SCENE_SHEET_SPHERE 3   /* folding sheet + rigid icosphere on a straight path */
+
+typedef struct bvhcd_scene_params {
+    uint64_t seed;
+    int32_t  n;          /* grid vertices per side (cloth kinds) */
+    int32_t  frames;     /* sequence length F */
+    int32_t  layers;     /* stacked sheets (LAYERS) */
+    int32_t  subdiv;     /* icosphere subdivision level */
+    float    amplitude;  /* max fold amplitude */
+    float    compress;   /* lateral compression of x at the last frame */
+    float    sigma;      /* per-vertex jitter std (TWO_SPHERES) */
+    float    radius;     /* icosphere radius */
+    float    spacing;    /* layer spacing (LAYERS) or sphere centre gap (TWO_SPHERES) */
+    float    speed;      /* approach speed per frame (TWO_SPHERES) */
+} bvhcd_scene_params;
+
+typedef struct bvhcd_scene bvhcd_scene;
+int     bvhcd_scene_create(int kind, const bvhcd_scene_params* p, bvhcd_scene** out);
+void    bvhcd_scene_destroy(bvhcd_scene* s);
+int64_t bvhcd_scene_num_tris(const bvhcd_scene* s);
+int64_t bvhcd_scene_num_verts(const bvhcd_scene* s);
+int     bvhcd_scene_faces(const bvhcd_scene* s, int32_t* faces);
+/* Vertices of frame f (0 <= f < frames) into xyz (V x 3 floats); threads <= 0 = all cores. */
+int     bvhcd_scene_frame(const bvhcd_scene* s, int32_t frame, float* xyz, int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BVHCD_H_ */

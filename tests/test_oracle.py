@@ -1,0 +1,346 @@
+"""CPU tests that pin the oracle (oracle/bvhcd_oracle.cpp) before anything is checked against it.
+
+Each check uses something other than the oracle's own formula: closed-form known answers,
+an independent bit-by-bit Morton interleave, numpy's stable argsort, a naive recursive
+"leftmost maximal delta" tree builder, brute-force all-pairs overlap, subtree min/max boxes,
+and a double-precision separating-axis test.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_package
+
+
+# ---------------------------------------------------------------- Morton
+def test_expand_bits_known_answers(oracle):
+    assert oracle.expand_bits(0x3FF) == 0x09249249
+    assert oracle.expand_bits(0) == 0
+    assert oracle.expand_bits(1) == 1
+    assert oracle.expand_bits(2) == 0b1000
+    for v in range(1024):
+        ref = 0
+        for b in range(10):
+            ref |= ((v >> b) & 1) << (3 * b)
+        assert oracle.expand_bits(v) == ref
+
+
+def test_morton3d_known_answers(oracle):
+    assert oracle.morton3d(0.0, 0.0, 0.0) == 0
+    assert oracle.morton3d(1.0, 1.0, 1.0) == 0x3FFFFFFF      # clamp to 1023
+    assert oracle.morton3d(2.0, -1.0, 0.0) == 0x24924924     # x clamps high, y clamps low
+    # x owns the highest bit of each triple
+    assert oracle.morton3d(0.5, 0.0, 0.0) == 1 << 29
+    assert oracle.morton3d(0.0, 0.5, 0.0) == 1 << 28
+    assert oracle.morton3d(0.0, 0.0, 0.5) == 1 << 27
+    assert oracle.morton3d(float("nan"), 0.0, 0.0) == 0      # 0/0 extent -> 0
+
+
+def _interleave(ix, iy, iz):
+    out = np.zeros_like(ix, dtype=np.uint64)
+    for b in range(10):
+        out |= ((ix >> b) & 1).astype(np.uint64) << np.uint64(3 * b + 2)
+        out |= ((iy >> b) & 1).astype(np.uint64) << np.uint64(3 * b + 1)
+        out |= ((iz >> b) & 1).astype(np.uint64) << np.uint64(3 * b)
+    return out.astype(np.uint32)
+
+
+def _codes_numpy(faces, verts, box):
+    v = verts[faces]                      # T x 3 x 3
+    mn = v.min(axis=1).astype(np.float32)
+    mx = v.max(axis=1).astype(np.float32)
+    c = ((mn + mx).astype(np.float32) * np.float32(0.5)).astype(np.float32)
+    off = (c - box[:3]).astype(np.float32)
+    ext = (box[3:] - box[:3]).astype(np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        q = (off / ext).astype(np.float32)
+    q = np.fmin(np.fmax(q * np.float32(1024.0), np.float32(0.0)), np.float32(1023.0))
+    iq = q.astype(np.uint32)
+    return _interleave(iq[:, 0], iq[:, 1], iq[:, 2])
+
+
+@pytest.mark.parametrize("name,kw,frame", [
+    ("spheres", dict(subdiv=3), 17),
+    ("cloth256", dict(n=48), 0),        # flat sheet: zero z extent
+    ("cloth256", dict(n=48), 299),
+])
+def test_morton_codes_independent(oracle, name, kw, frame):
+    pkg = load_package()
+    sc = pkg.Scene.preset(name, **kw)
+    f, v = sc.faces(), sc.frame(frame)
+    box = oracle.scene_box(f, v)
+    np.testing.assert_array_equal(box[:3], v[np.unique(f)].min(axis=0))
+    np.testing.assert_array_equal(box[3:], v[np.unique(f)].max(axis=0))
+    np.testing.assert_array_equal(oracle.morton_codes(f, v, box), _codes_numpy(f, v, box))
+
+
+def test_sort_is_stable_argsort(oracle):
+    rng = np.random.default_rng(7)
+    codes = rng.integers(0, 64, size=5000, dtype=np.uint32)  # many duplicates
+    s, idx, pm = oracle.sort_codes(codes)
+    ref = np.argsort(codes, kind="stable")
+    np.testing.assert_array_equal(idx, ref)
+    np.testing.assert_array_equal(s, codes[ref])
+    np.testing.assert_array_equal(pm[idx], np.arange(codes.size))
+
+
+# ---------------------------------------------------------------- topology
+def naive_topology(c):
+    """Recursive builder: split of [a, b] = leftmost k in [a, b-1] with maximal delta(k);
+    internal nodes numbered in preorder; encodings as in the reference (A.5)."""
+    n = len(c)
+    I = n - 1
+
+    def delta(k):
+        x = int(c[k]) ^ int(c[k + 1])
+        return x.bit_length()
+
+    lc, rc, par, mark, rx, ry = ([0] * I for _ in range(6))
+    lpar, lmark = [0] * n, [0] * n
+    lca = [None] * (n + 1)
+    nid = [0]
+    stack = [(0, n - 1, -1, False)]  # range, parent id, is right child
+    while stack:
+        a, b, p, right = stack.pop()
+        if a == b:
+            lpar[a] = p
+            lmark[a] = 7 if right else 3
+            continue
+        me = nid[0]
+        nid[0] += 1
+        best, k = -1, a
+        for j in range(a, b):
+            d = delta(j)
+            if d > best:
+                best, k = d, j
+        rx[me], ry[me], par[me] = a, b, p
+        mark[me] = 4 if right else 0
+        if lca[a] is None:
+            lca[a] = me
+        # children: left [a, k], right [k+1, b]; left subtree is numbered first
+        if k == a:
+            lc[me] = a
+            mark[me] |= 1
+        else:
+            lc[me] = me + 1
+        if k + 1 == b:
+            rc[me] = b
+            mark[me] |= 2
+        else:
+            rc[me] = me + 1 + (k - a)  # internal nodes in the left subtree: k - a
+        stack.append((k + 1, b, me, True))
+        stack.append((a, k, me, False))
+    enc = [(lca[x] << 1) if lca[x] is not None else (x << 1 | 1) for x in range(n)] + [-1]
+    return dict(int_lc=lc, int_rc=rc, int_par=par, int_mark=mark, int_rx=rx, int_ry=ry,
+                leaf_par=lpar, leaf_mark=lmark, leaf_lca=enc)
+
+
+@pytest.mark.parametrize("seed,n,hi", [(1, 2, 4), (2, 3, 2), (3, 17, 8), (4, 64, 4), (5, 300, 1 << 30),
+                                       (6, 300, 16), (7, 257, 1), (8, 1000, 200)])
+def test_topology_matches_naive_builder(oracle, seed, n, hi):
+    rng = np.random.default_rng(seed)
+    c = np.sort(rng.integers(0, hi, size=n, dtype=np.uint32))
+    got = oracle.build_topology(c)
+    ref = naive_topology(c)
+    for k, v in ref.items():
+        np.testing.assert_array_equal(got[k], np.asarray(v), err_msg=k)
+
+
+def check_invariants(t, n):
+    """Intent of checkBVHIntegrity (LBvhBuildKernels.cu:235-293)."""
+    I = n - 1
+    lc, rc, par, mark, rx, ry = (t[k] for k in ("int_lc", "int_rc", "int_par", "int_mark", "int_rx", "int_ry"))
+    assert t["leaf_lca"][n] == -1
+    assert par[0] == -1 and rx[0] == 0 and ry[0] == n - 1
+    for i in range(I):
+        if not (mark[i] & 1):
+            assert lc[i] == i + 1                      # left internal child follows in preorder
+            assert rx[lc[i]] == rx[i] and par[lc[i]] == i and not (mark[lc[i]] & 4)
+        else:
+            assert lc[i] == rx[i] and t["leaf_par"][lc[i]] == i and t["leaf_mark"][lc[i]] == 3
+        if not (mark[i] & 2):
+            assert ry[rc[i]] == ry[i] and par[rc[i]] == i and (mark[rc[i]] & 4)
+            assert rc[i] == i + (ry[i] - rx[i]) - (ry[rc[i]] - rx[rc[i]])
+        else:
+            assert rc[i] == ry[i] and t["leaf_par"][rc[i]] == i and t["leaf_mark"][rc[i]] == 7
+    for x in range(n):
+        e = t["leaf_lca"][x]
+        if e & 1:
+            assert e >> 1 == x and t["leaf_mark"][x] == 7
+        else:
+            top = e >> 1
+            assert rx[top] == x and (top == 0 or (mark[top] & 4))
+
+
+def test_topology_invariants_random(oracle):
+    rng = np.random.default_rng(11)
+    for n in (2, 5, 100, 2000):
+        c = np.sort(rng.integers(0, 1 << 12, size=n, dtype=np.uint32))
+        check_invariants(oracle.build_topology(c), n)
+
+
+def test_refit_is_subtree_min_max(oracle):
+    pkg = load_package()
+    sc = pkg.Scene.preset("spheres", subdiv=2)
+    f, v = sc.faces(), sc.frame(3)
+    ctx = oracle.OracleContext(f, sc.num_verts)
+    ctx.update_vertices(v)
+    ctx.step(0, 0)
+    b = ctx.export_bvh()
+    lb = b["leaf_box"]
+    np.testing.assert_array_equal(lb, oracle.tri_boxes(f, v)[b["prim_idx"]])
+    for i in range(len(b["int_rx"])):
+        seg = lb[b["int_rx"][i]: b["int_ry"][i] + 1]
+        np.testing.assert_array_equal(b["int_box"][i, :3], seg[:, :3].min(axis=0))
+        np.testing.assert_array_equal(b["int_box"][i, 3:], seg[:, 3:].max(axis=0))
+    check_invariants(b, sc.num_tris)
+
+
+# ---------------------------------------------------------------- narrow phase
+def tri(*p):
+    return np.array(p, np.float32).reshape(3, 3)
+
+
+def test_sat_special_cases(oracle):
+    A = tri(0, 0, 0, 1, 0, 0, 0, 1, 0)
+    assert oracle.sat(A, A)                                          # identical
+    assert oracle.sat(A, tri(0, 0, 0, -1, 0, 0, 0, -1, 0.3))         # shares one vertex (touching)
+    assert oracle.sat(A, tri(1, 0, 0, 0, 1, 0, 1, 1, 0))             # shares an edge
+    assert not oracle.sat(A, A + np.float32(5.0))                    # far apart
+    assert oracle.sat(A, tri(0.2, 0.2, 0, 0.9, 0.1, 0, 0.1, 0.9, 0))  # coplanar overlapping
+    assert not oracle.sat(A, tri(2, 2, 0, 3, 2, 0, 2, 3, 0))         # coplanar disjoint
+    assert oracle.sat(A, tri(0.25, 0.25, -1, 0.25, 0.25, 1, 0.3, 0.2, 1))  # piercing edge
+    assert not oracle.sat(A, tri(0.25, 0.25, 0.1, 0.25, 0.3, 1, 0.3, 0.2, 1))  # above the plane
+    # edge-edge crossing of two perpendicular triangles
+    assert oracle.sat(tri(-1, 0, 0, 1, 0, 0, 0, 0, 1), tri(0, -1, 0.5, 0, 1, 0.5, 0, 0, -1))
+
+
+def _sat_double(P, Q):
+    """Separating-axis test in float64 with the same 17 axes; returns (hit, min margin)."""
+    P = P.astype(np.float64)
+    Q = Q.astype(np.float64)
+    e = [P[1] - P[0], P[2] - P[1], P[0] - P[2]]
+    f = [Q[1] - Q[0], Q[2] - Q[1], Q[0] - Q[2]]
+    n, m = np.cross(e[0], e[1]), np.cross(f[0], f[1])
+    axes = [n, m] + [np.cross(a, b) for a in e for b in f] + [np.cross(a, n) for a in e] + [np.cross(b, m) for b in f]
+    margin = np.inf
+    for ax in axes:
+        p, q = P @ ax, Q @ ax
+        sep = max(p.min() - q.max(), q.min() - p.max())
+        scale = np.abs(ax).sum() * (np.abs(P).max() + np.abs(Q).max() + 1e-30)
+        if sep > 0:
+            return False, sep / scale
+        margin = min(margin, -sep / scale if scale > 0 else np.inf)
+    return True, margin
+
+
+def test_sat_vs_double_precision(oracle):
+    rng = np.random.default_rng(3)
+    disagree = 0
+    for _ in range(4000):
+        P = rng.normal(size=(3, 3)).astype(np.float32)
+        Q = (rng.normal(size=(3, 3)) * 0.7 + rng.normal(size=3) * 0.8).astype(np.float32)
+        hit = oracle.sat(P, Q)
+        ref, margin = _sat_double(P, Q)
+        if hit != ref:
+            disagree += 1
+            assert margin < 1e-5, (P, Q, margin)   # only decision-boundary cases may differ
+    assert disagree <= 4
+
+
+# ---------------------------------------------------------------- brute force
+def _bruteforce_numpy(f, v, narrow, oracle):
+    b = oracle.tri_boxes(f, v)
+    mn, mx = b[:, :3], b[:, 3:]
+    ov = np.all((mn[:, None, :] <= mx[None, :, :]) & (mx[:, None, :] >= mn[None, :, :]), axis=2)
+    i, j = np.nonzero(np.triu(ov, 1))
+    if narrow:
+        keep = [oracle.sat(v[f[a]], v[f[c]]) for a, c in zip(i, j)]
+        i, j = i[keep], j[keep]
+    return np.stack([i, j], 1)
+
+
+def test_bruteforce_matches_numpy(oracle):
+    pkg = load_package()
+    sc = pkg.Scene.preset("spheres", subdiv=2, spacing=1.9)
+    f, v = sc.faces(), sc.frame(0)
+    for narrow in (False, True):
+        got = oracle.pair_set(oracle.bruteforce_pairs(f, v, narrow=narrow))
+        ref = oracle.pair_set(_bruteforce_numpy(f, v, narrow, oracle))
+        np.testing.assert_array_equal(got, ref)
+
+
+# ---------------------------------------------------------------- pipeline vs brute force
+def front_partitions(fi, fe, b, n):
+    """Front cut completeness: per query i, the front subtrees cover leaves (i, n-1] once."""
+    cover = np.zeros(n + 1, np.int64)
+    rx, ry = b["int_rx"], b["int_ry"]
+    q = np.concatenate([fi[:, 0], fe[:, 0]])
+    lo = np.concatenate([rx[fi[:, 1]], fe[:, 1]])
+    hi = np.concatenate([ry[fi[:, 1]], fe[:, 1]])
+    assert np.all(lo > q)
+    # sum over queries of covered lengths must equal sum of (n-1-i)
+    assert int((hi - lo + 1).sum()) == int(((n - 1) - np.arange(n - 1)).sum())
+    # and no overlap within a query: sort by (q, lo) and check contiguity
+    order = np.lexsort((lo, q))
+    q, lo, hi = q[order], lo[order], hi[order]
+    starts = np.r_[True, q[1:] != q[:-1]]
+    assert np.all(lo[starts] == q[starts] + 1)
+    nxt = ~starts[1:]
+    assert np.all(lo[1:][nxt] == hi[:-1][nxt] + 1)
+    ends = np.r_[q[1:] != q[:-1], True]
+    assert np.all(hi[ends] == n - 1)
+
+
+@pytest.mark.parametrize("name,kw,frames", [
+    ("spheres", dict(subdiv=3, spacing=2.05, speed=0.004), 30),
+    ("cloth256", dict(n=40, frames=30), 30),
+])
+def test_oracle_pipeline_equals_bruteforce(oracle, name, kw, frames):
+    pkg = load_package()
+    sc = pkg.Scene.preset(name, **kw)
+    f = sc.faces()
+    ctx = oracle.OracleContext(f, sc.num_verts, bvh_cycle=12, front_cycle=6)
+    seen_ops = set()
+    for fr in range(frames):
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        ctx.step_scheduled()
+        st = ctx.stats()
+        seen_ops.add((st["last_bvh_op"], st["last_front_op"]))
+        bf_bp = oracle.pair_set(oracle.bruteforce_pairs(f, v, narrow=False))
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), bf_bp, err_msg=f"frame {fr} bp")
+        bf_np = oracle.pair_set(oracle.bruteforce_pairs(f, v, narrow=True))
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), bf_np, err_msg=f"frame {fr} np")
+        if st["last_front_op"] != 0:
+            front_partitions(ctx.front(0), ctx.front(1), ctx.export_bvh(), sc.num_tris)
+    fronts = {fo for _, fo in seen_ops}
+    assert {1, 2, 3, 4} <= fronts, seen_ops
+
+
+def test_oracle_explicit_ops_and_restructure(oracle):
+    """Every front op after every BVH op gives the brute-force pairs; forced restructure
+    keeps the tree valid."""
+    pkg = load_package()
+    sc = pkg.Scene.preset("cloth256", n=32, frames=20)
+    f = sc.faces()
+    ctx = oracle.OracleContext(f, sc.num_verts)
+    ctx.update_vertices(sc.frame(0))
+    ctx.step(0, 1)
+    for fr in range(1, 20):
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        fo = [2, 4, 3, 2, 0][fr % 5]
+        bo = 2 if ctx.stats()["restr_pending"] else 1
+        ctx.step(bo, fo)
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)),
+                                      oracle.pair_set(oracle.bruteforce_pairs(f, v, True)), err_msg=str(fr))
+        check_invariants(ctx.export_bvh(), sc.num_tris)
+
+
+def test_plain_schedule_table(oracle):
+    pkg = load_package()
+    got = [oracle.plain_schedule(f) for f in range(26)]
+    assert got == [pkg.plain_schedule(f) for f in range(26)]
+    # frame 0 build+generate; 1-4 refit+keep; 5 refit+reorder; 6 refit+update; 11 keep; 12 build+generate
+    assert got[0] == (0, 1) and got[1] == (1, 4) and got[5] == (1, 3) and got[6] == (1, 2)
+    assert got[11] == (1, 4) and got[12] == (0, 1) and got[17] == (1, 3) and got[18] == (1, 2)
