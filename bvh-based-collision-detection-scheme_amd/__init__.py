@@ -99,6 +99,7 @@ def lib():
         "bvhcd_pair_digest": (C.c_int, [P, C.c_int, C.POINTER(U64), C.POINTER(I64)]),
         "bvhcd_get_stats": (C.c_int, [P, C.POINTER(Stats)]),
         "bvhcd_set_timing": (C.c_int, [P, C.c_int]),
+        "bvhcd_timing_frames": (C.c_int, [P, I64, P, P, P, C.POINTER(I64)]),
         "bvhcd_export_bvh": (C.c_int, [P] + [P] * 14),
         "bvhcd_export_front": (C.c_int, [P, C.c_int, P, I64, C.POINTER(I64)]),
         "bvhcd_export_scene_box": (C.c_int, [P, P]),
@@ -275,8 +276,21 @@ class CollisionContext:
     def step_scheduled(self):
         self._chk(lib().bvhcd_step_scheduled(self._h), "step_scheduled")
 
-    def set_timing(self, on=True):
-        self._chk(lib().bvhcd_set_timing(self._h, 1 if on else 0), "set_timing")
+    def set_timing(self, frames):
+        """Record per-stage event timing for up to `frames` frames (0 = off)."""
+        self._chk(lib().bvhcd_set_timing(self._h, int(frames)), "set_timing")
+
+    def timing_frames(self):
+        """(ms[n, 8], bytes[n, 8], ops[n, 2]) of the frames recorded since set_timing."""
+        n = C.c_int64()
+        self._chk(lib().bvhcd_timing_frames(self._h, 0, None, None, None, C.byref(n)), "timing_frames")
+        ms = np.zeros((n.value, 8), np.float64)
+        by = np.zeros((n.value, 8), np.float64)
+        ops = np.zeros((n.value, 2), np.int32)
+        if n.value:
+            self._chk(lib().bvhcd_timing_frames(self._h, n.value, _ptr(ms), _ptr(by), _ptr(ops), C.byref(n)),
+                      "timing_frames")
+        return ms, by, ops
 
     def pairs(self, filtered=True) -> np.ndarray:
         n = C.c_int64()

@@ -1,0 +1,113 @@
+"""CPU-only tests of the product library: it loads, exports every entry point declared in
+include/*.h, generates deterministic scenes, and fails loudly (no fallback) without a GPU.
+Also pins the oracle against the committed golden fixtures."""
+import glob
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_package
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(bvhcd_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    pkg = load_package()
+    L = pkg.lib()
+    names = declared_symbols()
+    assert len(names) > 30
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_scene_sizes():
+    pkg = load_package()
+    expect = {"spheres": (40960, 20484), "cloth256": (130050, 65536), "cloth1024": (2093058, 1048576),
+              "layers4": (8372232, 4194304)}
+    for name, (T, V) in expect.items():
+        sc = pkg.Scene.preset(name)
+        assert (sc.num_tris, sc.num_verts) == (T, V), name
+
+
+def test_scene_faces_valid_and_deterministic():
+    pkg = load_package()
+    sc = pkg.Scene.preset("spheres", subdiv=3)
+    f = sc.faces()
+    assert f.min() == 0 and f.max() == sc.num_verts - 1
+    a = sc.frame(7)
+    b = pkg.Scene.preset("spheres", subdiv=3).frame(7, threads=1)
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(sc.frame(7), sc.frame(8))  # fresh jitter per frame
+    cl = pkg.Scene.preset("cloth256", n=32)
+    f0 = cl.frame(0)
+    assert np.all(f0[:, 2] == 0)                            # flat at frame 0
+    assert np.allclose(cl.frame(299)[:, 0].max(), 0.5, atol=1e-6)  # x compressed by 0.5
+
+
+def test_scene_frames_match_golden_hashes():
+    pkg = load_package()
+    meta = json.load(open(os.path.join(GOLDEN, "golden.json")))
+    for key, h in meta["frame_sha256"].items():
+        name, fr = key.split("/")
+        got = hashlib.sha256(pkg.Scene.preset(name).frame(int(fr)).tobytes()).hexdigest()
+        assert got == h, key
+
+
+def test_oracle_matches_golden_build(oracle):
+    g = np.load(os.path.join(GOLDEN, "golden_build1000.npz"), allow_pickle=False)
+    box = oracle.scene_box(g["faces"], g["verts"])
+    np.testing.assert_array_equal(box, g["box"])
+    codes = oracle.morton_codes(g["faces"], g["verts"], box)
+    np.testing.assert_array_equal(codes, g["codes"])
+    s, idx, pm = oracle.sort_codes(codes)
+    np.testing.assert_array_equal(idx, g["prim_idx"])
+    topo = oracle.build_topology(s)
+    for k, v in topo.items():
+        np.testing.assert_array_equal(v, g[k], err_msg=k)
+
+
+@pytest.mark.parametrize("seq", ["spheres", "cloth64"])
+def test_oracle_matches_golden_digests(oracle, seq):
+    pkg = load_package()
+    meta = json.load(open(os.path.join(GOLDEN, "golden.json")))[seq]
+    name, kw = ("spheres", {}) if seq == "spheres" else ("cloth256", {"n": 64})
+    sc = pkg.Scene.preset(name, **kw)
+    ctx = oracle.OracleContext(sc.faces(), sc.num_verts)
+    last = max(int(k) for k in meta)
+    for fr in range(last + 1):
+        ctx.update_vertices(sc.frame(fr))
+        ctx.step_scheduled()
+        if str(fr) in meta:
+            d, n = oracle.pair_digest(ctx.pairs(True))
+            assert (n, str(d)) == (meta[str(fr)]["pairs"], meta[str(fr)]["digest"]), fr
+
+
+def test_context_without_gpu_fails_loudly():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    pkg = load_package()
+    with pytest.raises(pkg.BvhcdError):
+        pkg.CollisionContext(100, 100)
+
+
+def test_bench_rejects_multi_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stdout
