@@ -98,6 +98,11 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.int_rx, T);
     ok &= dalloc(c, d.int_ry, T);
     ok &= dalloc(c, d.int_flag, T);
+    ok &= dalloc(c, d.itopo, T);
+    ok &= dalloc(c, d.tri_finite, T);
+    ok &= dalloc(c, d.gpart, 12 * T);
+    ok &= dalloc(c, d.lroot, T);
+    ok &= dalloc(c, d.lbase, T);
     ok &= dalloc(c, d.leaf_par, T);
     ok &= dalloc(c, d.leaf_mark, T);
     ok &= dalloc(c, d.leaf_lca, T + 1);
@@ -111,13 +116,6 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.depths, T + 1);
     ok &= dalloc(c, d.offsets, T + 1);
     ok &= dalloc(c, d.tkmap, T);
-    // scan block sums: enough for the largest scanned array (digit table of the sort)
-    const int64_t tiles = (T + 4095) / 4096;
-    d.sort_hist_n = (size_t)(256 * tiles);
-    const int64_t max_scan = std::max<int64_t>((int64_t)d.sort_hist_n, T + 2);
-    d.scan_tmp_n = (size_t)((max_scan + 4095) / 4096 + 1);
-    ok &= dalloc(c, d.scan_tmp, d.scan_tmp_n);
-    ok &= dalloc(c, d.sort_hist, d.sort_hist_n);
     ok &= dalloc(c, d.scene_box, 8);
     ok &= dalloc(c, d.scene_box_u, 8);
     ok &= dalloc(c, d.fsize, 4);
@@ -164,10 +162,17 @@ int allocate(bvhcd_ctx* c) {
     d.front_cap[0] = std::max<int64_t>(ci, 1);
     d.front_cap[1] = std::max<int64_t>(ce, 1);
     d.pair_cap = std::max<int64_t>(cp, 1);
-    ok &= dalloc(c, d.front[0][0], d.front_cap[0]);
-    ok &= dalloc(c, d.front[1][0], d.front_cap[0]);
-    ok &= dalloc(c, d.front[0][1], d.front_cap[1]);
-    ok &= dalloc(c, d.front[1][1], d.front_cap[1]);
+    for (int k = 0; k < 2; ++k)
+        for (int b = 0; b < 2; ++b) {
+            ok &= dalloc(c, d.fnode[k][b], d.front_cap[k]);
+            ok &= dalloc(c, d.fquery[k][b], d.front_cap[k]);
+        }
+    // digit table of the largest sort (front ordering) and the scan block sums
+    const int64_t max_sort = std::max<int64_t>(T, std::max(d.front_cap[0], d.front_cap[1]));
+    d.sort_hist_n = (size_t)(256 * ((max_sort + 4095) / 4096));
+    d.scan_tmp_n = (size_t)((std::max<int64_t>((int64_t)d.sort_hist_n, T + 2) + 4095) / 4096 + 1);
+    ok &= dalloc(c, d.sort_hist, d.sort_hist_n);
+    ok &= dalloc(c, d.scan_tmp, d.scan_tmp_n);
     ok &= dalloc(c, d.bp, d.pair_cap);
     ok &= dalloc(c, d.np, d.pair_cap);
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
@@ -187,19 +192,20 @@ void mark(bvhcd_ctx* c, int stage, bool end) {
 void begin(bvhcd_ctx* c, int stage) { mark(c, stage, false); }
 void end(bvhcd_ctx* c, int stage) { mark(c, stage, true); }
 
-__global__ void k_log_frame(DevState d, int cur, uint64_t* out) {
+__global__ void k_log_frame(DevState d, int cur_int, int cur_ext, uint64_t* out) {
     if (threadIdx.x != 0) return;
     out[0] = ctr64(d.ctr)[C64_VISITS];
     out[1] = d.ctr[C_BP];
     out[2] = d.ctr[C_NP];
-    out[3] = d.fsize[2 * cur];
-    out[4] = d.fsize[2 * cur + 1];
+    out[3] = d.fsize[cur_int];
+    out[4] = d.fsize[2 + cur_ext];
     out[5] = d.ctr[C_OVF];
 }
 
 void log_frame(bvhcd_ctx* c, int bvh_op, int front_op) {
     if (!timing_on(c)) return;
-    hipLaunchKernelGGL(k_log_frame, dim3(1), dim3(64), 0, c->stream, c->d, c->d.cur, c->dlog + kLog * c->tn);
+    hipLaunchKernelGGL(k_log_frame, dim3(1), dim3(64), 0, c->stream, c->d, c->d.fcur[0], c->d.fcur[1],
+                       c->dlog + kLog * c->tn);
     c->tops.push_back(bvh_op);
     c->tops.push_back(front_op);
     c->trestr.push_back(c->restr_leaf);
@@ -244,6 +250,7 @@ int do_build(bvhcd_ctx* c) {
     launch_prim_map(d, n, s);
     if (n >= 2) {
         launch_emit_full(d, n, s);
+        launch_refit_prep(d, n, s);
     } else {
         // single leaf: no internal nodes; lca = {leaf 0, end}
         const int32_t lca[2] = {1, -1};
@@ -512,7 +519,7 @@ int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V,
     c->V = V;
     c->built = c->front_valid = c->restr_pending = false;
     c->frame = 0;
-    c->d.cur = 0;
+    c->d.fcur[0] = c->d.fcur[1] = 0;
     HIPCHK(c, hipMemsetAsync(c->d.fsize, 0, 4 * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
     return BVHCD_OK;
@@ -630,8 +637,8 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     out->frame = c->frame;
     out->last_bvh_op = c->last_bvh;
     out->last_front_op = c->last_front;
-    out->int_front = sz[2 * c->d.cur];
-    out->ext_front = sz[2 * c->d.cur + 1];
+    out->int_front = sz[c->d.fcur[0]];
+    out->ext_front = sz[2 + c->d.fcur[1]];
     out->bp_pairs = ctr[C_BP];
     out->np_pairs = ctr[C_NP];
     out->required_int_front = out->int_front;
@@ -716,14 +723,24 @@ int bvhcd_export_front(bvhcd_ctx* c, int which, int32_t* out, int64_t cap, int64
     uint32_t sz[4];
     HIPCHK(c, hipMemcpyAsync(sz, c->d.fsize, sizeof(sz), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int64_t n = sz[2 * c->d.cur + which];
+    const int b = c->d.fcur[which];
+    const int64_t n = sz[2 * which + b];
     *num = n;
     if (out && cap > 0) {
         const int64_t k = std::min<int64_t>(std::min<int64_t>(cap, n), c->d.front_cap[which]);
-        if (k > 0)
-            HIPCHK(c, hipMemcpyAsync(out, c->d.front[c->d.cur][which], sizeof(int2) * k, hipMemcpyDeviceToHost,
+        if (k > 0) {
+            std::vector<uint32_t> node((size_t)k);
+            std::vector<int32_t> query((size_t)k);
+            HIPCHK(c, hipMemcpyAsync(node.data(), c->d.fnode[which][b], sizeof(uint32_t) * k, hipMemcpyDeviceToHost,
                                      c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipMemcpyAsync(query.data(), c->d.fquery[which][b], sizeof(int32_t) * k,
+                                     hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (int64_t i = 0; i < k; ++i) {
+                out[2 * i] = query[i];
+                out[2 * i + 1] = (int32_t)node[i];
+            }
+        }
     }
     return check_overflow(c);
 }

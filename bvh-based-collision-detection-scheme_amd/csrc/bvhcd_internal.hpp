@@ -124,6 +124,12 @@ struct DevState {
     int32_t* int_rx = nullptr;
     int32_t* int_ry = nullptr;
     uint32_t* int_flag = nullptr;   // T-1 refit arrival counters
+    int4* itopo = nullptr;          // T-1 packed (lc, rc, par, mark) for the refit climb
+    uint8_t* tri_finite = nullptr;  // T (by triangle id): all 9 |coordinates| <= 1e8 (set by refit)
+    float* gpart = nullptr;         // 12 floats per internal node: child boxes handed over
+                                    // between workgroups at chunk-spanning nodes (left, right)
+    int32_t* lroot = nullptr;       // T maximal chunk-local subtree root of each leaf (-1 none)
+    int32_t* lbase = nullptr;       // T LDS slot base of that subtree
     int32_t* leaf_par = nullptr;    // T
     uint32_t* leaf_mark = nullptr;  // T
     int32_t* leaf_lca = nullptr;    // T+1
@@ -145,11 +151,12 @@ struct DevState {
     size_t sort_hist_n = 0;
     float* scene_box = nullptr;     // 6 floats (+ pad)
     uint32_t* scene_box_u = nullptr;// ordered-int reduction scratch, 6 words
-    // fronts (double buffered) + logs
-    int2* front[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buf][int/ext]
-    int64_t front_cap[2] = {0, 0};  // int, ext
-    int cur = 0;                    // current buffer index
-    uint32_t* fsize = nullptr;      // [buf*2 + which] sizes (4 words)
+    // fronts: SoA (node id, query leaf), double buffered per kind (0 internal, 1 external)
+    uint32_t* fnode[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [kind][buf]
+    int32_t* fquery[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int64_t front_cap[2] = {0, 0};  // entries per buffer, per kind
+    int fcur[2] = {0, 0};           // current buffer per kind
+    uint32_t* fsize = nullptr;      // [kind*2 + buf] sizes (4 words)
     uint32_t* intcnt = nullptr;     // T
     uint32_t* extcnt = nullptr;     // T+1
     uint32_t* intbeg = nullptr;     // T+1

@@ -13,42 +13,140 @@
 //   narrow phase         collision/narrow_phase/narrow_phase.cu:8-24 (17-axis SAT written here from the
 //                        separating-axis theorem; canonical argument order: lower triangle id first)
 //
-// Every node visit is one 32-byte record (two 16-byte loads): box + rangex/escape (internal) or
-// box + parent/escape (leaf). Appends to fronts and pair lists are wave-aggregated (one atomic
-// per wave64) and bounds-checked against capacity, with an overflow bit instead of a write.
-// Front kernels fetch work dynamically per wave so that uneven traversals balance.
+// Layout and scheduling on gfx950:
+//  * Every node visit is one 32-byte record (two 16-byte loads): box + rangex/escape
+//    (internal) or box + parent/escape (leaf).
+//  * Fronts are SoA (node id, query leaf) so that ordering is a radix sort on the node id
+//    with the query as payload: no per-node counters, no atomics, and the bucket starts
+//    (the reference's FlOrderLog offsets) come from binary searches on the sorted ids.
+//  * Work is distributed grid-stride (consecutive entries per wave64, coalesced SoA loads).
+//  * Appends go through a per-wave LDS staging buffer: lanes take slots with a ballot
+//    prefix, and one global atomic reserves space for a whole buffer when it is flushed.
+//    Capacity is checked before every global write (overflow bit instead of a write).
 #include "bvhcd_internal.hpp"
 #include "kernels.hpp"
 
 namespace bvhcd {
 
-struct FrontIO {
-    const int2* fi;       // current internal front
-    const int2* fe;       // current external front
-    const uint32_t* fsz;  // current sizes [int, ext]
-    int2* oi;             // next internal front
-    int2* oe;             // next external front
-    uint32_t* osz;        // next sizes [int, ext]
-    uint32_t cap_i, cap_e;
+constexpr int kWaves = kBlock / 64;
+constexpr uint32_t kStage = 256;  // staged entries per wave per output list
+
+__device__ inline uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+
+// ---------------------------------------------------------------- output sinks
+struct FrontSink {  // SoA front buffer
+    uint32_t* count;
+    uint32_t* node;
+    int32_t* query;
+    uint32_t cap;
+    uint32_t ovf_bit;
+    __device__ void write(uint32_t o, int2 v) const {
+        node[o] = (uint32_t)v.y;
+        query[o] = v.x;
+    }
+};
+struct PairSink {  // int2 pair list
+    uint32_t* count;
+    int2* out;
+    uint32_t cap;
+    uint32_t ovf_bit;
+    __device__ void write(uint32_t o, int2 v) const { out[o] = v; }
 };
 
-__device__ inline void push_pair(const DevState& d, int32_t qi, int32_t j) {
-    const int32_t a = d.prim_idx[qi], b = d.prim_idx[j];
-    const uint32_t slot = wave_append(&d.ctr[C_BP], true);
-    if (slot < (uint32_t)d.pair_cap) d.bp[slot] = a < b ? make_int2(a, b) : make_int2(b, a);
-    else atomicOr(&d.ctr[C_OVF], 4u);
+template <class Sink>
+__device__ void stage_flush(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, uint32_t f, uint64_t active) {
+    const int lane = lane_id();
+    const int leader = __ffsll((long long)active) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(sk.count, f);
+    base = __shfl(base, leader);
+    const uint32_t rank = (uint32_t)__popcll(active & lanes_below());
+    const uint32_t nact = (uint32_t)__popcll(active);
+    bool over = false;
+    for (uint32_t i = rank; i < f; i += nact) {
+        const uint32_t o = base + i;
+        const int2 v = make_int2(sbuf[i].x, sbuf[i].y);
+        if (o < sk.cap) sk.write(o, v);
+        else over = true;
+    }
+    if (over) atomicOr(ovf, sk.ovf_bit);
 }
-__device__ inline void push_int(const DevState& d, const FrontIO& io, int32_t qi, int32_t node) {
-    const uint32_t slot = wave_append(&io.osz[0], true);
-    if (slot < io.cap_i) io.oi[slot] = make_int2(qi, node);
-    else atomicOr(&d.ctr[C_OVF], 1u);
-    atomicAdd(&d.intcnt[node], 1u);
+
+// Called by the currently active lanes of a wave; `pred` selects the lanes that emit `v`.
+template <class Sink>
+__device__ inline void stage_push(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
+                                  bool pred, int2 v) {
+    const uint64_t active = __ballot(1);
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    uint32_t f = *sfill;
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    if (f + cnt > kStage) {
+        stage_flush(sk, ovf, sbuf, f, active);
+        f = 0;
+    }
+    if (pred) {
+        const uint32_t slot = f + (uint32_t)__popcll(m & lanes_below());
+        sbuf[slot].x = v.x;
+        sbuf[slot].y = v.y;
+    }
+    if (lane_id() == __ffsll((long long)active) - 1) *sfill = f + cnt;
 }
-__device__ inline void push_ext(const DevState& d, const FrontIO& io, int32_t qi, int32_t leaf) {
-    const uint32_t slot = wave_append(&io.osz[1], true);
-    if (slot < io.cap_e) io.oe[slot] = make_int2(qi, leaf);
-    else atomicOr(&d.ctr[C_OVF], 2u);
-    atomicAdd(&d.extcnt[leaf], 1u);
+
+template <class Sink>
+__device__ inline void stage_final(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill) {
+    const uint32_t f = *sfill;
+    if (f) stage_flush(sk, ovf, sbuf, f, __ballot(1));
+    if (lane_id() == 0) *sfill = 0;
+}
+
+// Per-block staging storage for up to three lists.
+struct Stages {
+    int2 buf[3][kWaves][kStage];
+    uint32_t fill[3][kWaves];
+};
+__device__ inline void stages_init(Stages& st) {
+    if (lane_id() < 3) st.fill[lane_id()][threadIdx.x >> 6] = 0;
+}
+
+// Output context of a traversal thread.
+struct Out {
+    FrontSink fi, fe;
+    PairSink bp;
+    uint32_t* ovf;
+    volatile int2* sb[3];
+    volatile uint32_t* sf[3];
+    const int32_t* prim_idx;
+    __device__ void pair(int32_t qi, int32_t j) const {
+        const int32_t a = prim_idx[qi], b = prim_idx[j];
+        stage_push(bp, ovf, sb[2], sf[2], true, a < b ? make_int2(a, b) : make_int2(b, a));
+    }
+    __device__ void int_entry(int32_t qi, int32_t node) const {
+        stage_push(fi, ovf, sb[0], sf[0], true, make_int2(qi, node));
+    }
+    __device__ void ext_entry(int32_t qi, int32_t leaf) const {
+        stage_push(fe, ovf, sb[1], sf[1], true, make_int2(qi, leaf));
+    }
+    __device__ void final_flush() const {
+        stage_final(fi, ovf, sb[0], sf[0]);
+        stage_final(fe, ovf, sb[1], sf[1]);
+        stage_final(bp, ovf, sb[2], sf[2]);
+    }
+};
+
+__device__ inline Out make_out(Stages& st, const FrontSink& fi, const FrontSink& fe, const DevState& d) {
+    Out o;
+    o.fi = fi;
+    o.fe = fe;
+    o.bp = PairSink{&d.ctr[C_BP], d.bp, (uint32_t)min((int64_t)d.pair_cap, (int64_t)0xFFFFFFFF), 4u};
+    o.ovf = &d.ctr[C_OVF];
+    const int w = threadIdx.x >> 6;
+    for (int k = 0; k < 3; ++k) {
+        o.sb[k] = st.buf[k][w];
+        o.sf[k] = &st.fill[k][w];
+    }
+    o.prim_idx = d.prim_idx;
+    return o;
 }
 
 __device__ inline bool node_overlap(const DevState& d, int32_t nd, const Box& q) {
@@ -59,8 +157,7 @@ __device__ inline bool node_overlap(const DevState& d, int32_t nd, const Box& q)
 // Stackless DFS from encoded node `st` until `stop` (lca-escape encoding). With FRONT, the
 // non-overlapping internal nodes and all reached leaves are appended to the next front.
 template <bool FRONT>
-__device__ uint32_t stackless(const DevState& d, const FrontIO& io, int32_t qi, const Box& q, int32_t st,
-                              int32_t stop) {
+__device__ uint32_t stackless(const DevState& d, const Out& out, int32_t qi, const Box& q, int32_t st, int32_t stop) {
     uint32_t visits = 0;
     // A correct traversal visits every node at most once; the bound only guards against a
     // corrupted tree (flag 16 reported as a device error instead of a hang).
@@ -75,19 +172,22 @@ __device__ uint32_t stackless(const DevState& d, const FrontIO& io, int32_t qi, 
         if (!(st & 1)) {
             int32_t nd = st >> 1;
             float4 lo = d.inode[nd].lo, hi = d.inode[nd].hi;
-            j = f2i(lo.w);                  // leftmost leaf of the whole left spine
+            j = f2i(lo.w);                    // leftmost leaf of the whole left spine
             const int32_t t = d.leaf_par[j];  // bottom of the spine
             bool reached = false;
             for (;;) {
                 ++visits;
                 if (!box_overlap(q, lo, hi)) break;
-                if (nd == t) { reached = true; break; }
+                if (nd == t) {
+                    reached = true;
+                    break;
+                }
                 ++nd;
                 lo = d.inode[nd].lo;
                 hi = d.inode[nd].hi;
             }
             if (!reached) {
-                if (FRONT) push_int(d, io, qi, nd);
+                if (FRONT) out.int_entry(qi, nd);
                 st = f2i(hi.w);
                 continue;
             }
@@ -96,45 +196,52 @@ __device__ uint32_t stackless(const DevState& d, const FrontIO& io, int32_t qi, 
         }
         const float4 llo = d.leaf[j].lo, lhi = d.leaf[j].hi;
         ++visits;
-        if (box_overlap(q, llo, lhi)) push_pair(d, qi, j);
-        if (FRONT) push_ext(d, io, qi, j);
+        if (box_overlap(q, llo, lhi)) out.pair(qi, j);
+        if (FRONT) out.ext_entry(qi, j);
         st = f2i(lhi.w);
     }
     return visits;
 }
 
+// Block-reduced visit count, one 64-bit atomic per block.
 __device__ inline void add_visits(const DevState& d, uint32_t v) {
-    // wave reduction, one 64-bit atomic per wave
+    __shared__ uint64_t red[kWaves];
     uint64_t x = v;
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-    if (lane_id() == 0 && x) atomicAdd((unsigned long long*)&ctr64(d.ctr)[C64_VISITS], (unsigned long long)x);
-}
-
-// ---------------------------------------------------------------- pure traversal / generate
-template <bool FRONT>
-__global__ void __launch_bounds__(kBlock) k_traverse(int32_t n, DevState d, FrontIO io) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t v = 0;
-    if (i < n - 1) {
-        const float4 lo = d.leaf[i].lo, hi = d.leaf[i].hi;
-        v = stackless<FRONT>(d, io, i, rec_box(lo, hi), f2i(hi.w), -1);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int i = 0; i < kWaves; ++i) t += red[i];
+        if (t) atomicAdd((unsigned long long*)&ctr64(d.ctr)[C64_VISITS], (unsigned long long)t);
     }
-    add_visits(d, v);
 }
 
-static FrontIO make_io(DevState& d) {
-    FrontIO io;
-    const int c = d.cur, nx = d.cur ^ 1;
-    io.fi = d.front[c][0];
-    io.fe = d.front[c][1];
-    io.fsz = d.fsize + 2 * c;
-    io.oi = d.front[nx][0];
-    io.oe = d.front[nx][1];
-    io.osz = d.fsize + 2 * nx;
-    io.cap_i = (uint32_t)std::min<int64_t>(d.front_cap[0], 0xFFFFFFFFll);
-    io.cap_e = (uint32_t)std::min<int64_t>(d.front_cap[1], 0xFFFFFFFFll);
-    return io;
+// ---------------------------------------------------------------- host-side front views
+struct FrontIn {
+    const uint32_t* node[2];
+    const int32_t* query[2];
+    const uint32_t* size[2];
+    uint32_t cap[2];
+};
+
+static FrontSink next_sink(DevState& d, int kind) {
+    const int nb = d.fcur[kind] ^ 1;
+    return FrontSink{d.fsize + 2 * kind + nb, d.fnode[kind][nb], d.fquery[kind][nb],
+                     (uint32_t)std::min<int64_t>(d.front_cap[kind], 0xFFFFFFFFll), kind ? 2u : 1u};
 }
+static FrontIn cur_front(DevState& d) {
+    FrontIn f;
+    for (int k = 0; k < 2; ++k) {
+        const int b = d.fcur[k];
+        f.node[k] = d.fnode[k][b];
+        f.query[k] = d.fquery[k][b];
+        f.size[k] = d.fsize + 2 * k + b;
+        f.cap[k] = (uint32_t)std::min<int64_t>(d.front_cap[k], 0xFFFFFFFFll);
+    }
+    return f;
+}
+__host__ __device__ inline FrontSink null_sink() { return FrontSink{nullptr, nullptr, nullptr, 0u, 0u}; }
 
 static void reset_step_counters(DevState& d, hipStream_t s) {
     // pair counts, overflow, visits
@@ -142,128 +249,154 @@ static void reset_step_counters(DevState& d, hipStream_t s) {
     hipMemsetAsync(ctr64(d.ctr) + C64_VISITS, 0, sizeof(uint64_t), s);
 }
 
+static unsigned stream_grid() { return 2048; }
+
+// ---------------------------------------------------------------- pure traversal / generate
+template <bool FRONT>
+__global__ void __launch_bounds__(kBlock) k_traverse(int32_t n, DevState d, FrontSink fi, FrontSink fe) {
+    __shared__ Stages st;
+    stages_init(st);
+    const Out out = make_out(st, fi, fe, d);
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t v = 0;
+    if (i < n - 1) {
+        const float4 lo = d.leaf[i].lo, hi = d.leaf[i].hi;
+        v = stackless<FRONT>(d, out, i, rec_box(lo, hi), f2i(hi.w), -1);
+    }
+    if (FRONT) {
+        out.final_flush();
+    } else {
+        stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
+    }
+    add_visits(d, v);
+}
+
 void launch_pure_cd(DevState& d, int32_t n, hipStream_t s) {
     reset_step_counters(d, s);
-    FrontIO io = make_io(d);
-    if (n >= 2) hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, io);
+    if (n >= 2)
+        hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, null_sink(),
+                           null_sink());
 }
 
-// ---------------------------------------------------------------- ordering (counting sort by node)
-__global__ void __launch_bounds__(kBlock) k_order_scatter(DevState d, FrontIO io) {
-    const uint32_t ni = io.fsz[0], ne = io.fsz[1];
-    const uint32_t ci = min(ni, io.cap_i), ce = min(ne, io.cap_e);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ci + ce; k += gridDim.x * blockDim.x) {
-        if (k < ci) {
-            const int2 e = io.fi[k];
-            const uint32_t pos = d.intbeg[e.y] + atomicAdd(&d.intcnt[e.y], 1u);
-            if (pos < io.cap_i) io.oi[pos] = e;
-        } else {
-            const int2 e = io.fe[k - ci];
-            const uint32_t pos = d.extbeg[e.y] + atomicAdd(&d.extcnt[e.y], 1u);
-            if (pos < io.cap_e) io.oe[pos] = e;
+// ---------------------------------------------------------------- ordering
+// beg[b] = first position of bucket b in the sorted keys (b in [0, nb]); key count from *dn.
+__global__ void __launch_bounds__(kBlock) k_bucket_begin(const uint32_t* __restrict__ keys, const uint32_t* dn,
+                                                         uint32_t cap, int32_t nb, uint32_t* __restrict__ beg) {
+    const uint32_t m = min(*dn, cap);
+    for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nb; b += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = m;  // first index with keys[idx] >= b
+        while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (keys[mid] < (uint32_t)b) lo = mid + 1;
+            else hi = mid;
         }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        io.osz[0] = ni;
-        io.osz[1] = ne;
+        beg[b] = lo;
     }
 }
 
-static unsigned persistent_grid() { return 2048; }
+static int bits_for(int64_t v) {
+    int b = 1;
+    while (((int64_t)1 << b) <= v) ++b;
+    return b;
+}
 
-// prepare (exclusive scans of the per-node counts), clear, scatter into the other buffer, flip.
+// Sort one front kind by node id (query as payload) and flip to the sorted buffer.
+static void sort_front(DevState& d, int kind, int64_t max_id, hipStream_t s) {
+    const int c = d.fcur[kind], o = c ^ 1;
+    const bool in_tmp = radix_sort_u32_dev(d, d.fnode[kind][c], d.fquery[kind][c], d.fnode[kind][o], d.fquery[kind][o],
+                                           d.front_cap[kind], d.fsize + 2 * kind + c, bits_for(max_id), s);
+    if (in_tmp) {
+        hipMemcpyAsync(d.fsize + 2 * kind + o, d.fsize + 2 * kind + c, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        d.fcur[kind] = o;
+    }
+}
+
+// The reference's FlOrderLog::prepare + counting-sort scatter: here a radix sort per kind,
+// then intbeg[0..n-1] (internal ids) and extbeg[0..n] (leaves) from the sorted node ids.
 void launch_order_fronts(DevState& d, int32_t n, hipStream_t s) {
-    launch_exscan(d, d.intcnt, d.intbeg, n, nullptr, s);
-    launch_exscan(d, d.extcnt, d.extbeg, (int64_t)n + 1, nullptr, s);
-    hipMemsetAsync(d.intcnt, 0, sizeof(uint32_t) * n, s);
-    hipMemsetAsync(d.extcnt, 0, sizeof(uint32_t) * ((size_t)n + 1), s);
-    FrontIO io = make_io(d);
-    hipLaunchKernelGGL(k_order_scatter, dim3(persistent_grid()), dim3(kBlock), 0, s, d, io);
-    d.cur ^= 1;
+    sort_front(d, 0, n - 1, s);
+    sort_front(d, 1, n, s);
+    hipLaunchKernelGGL(k_bucket_begin, dim3(grid_for(n)), dim3(kBlock), 0, s, d.fnode[0][d.fcur[0]],
+                       d.fsize + d.fcur[0], (uint32_t)std::min<int64_t>(d.front_cap[0], 0xFFFFFFFFll), n - 1, d.intbeg);
+    hipLaunchKernelGGL(k_bucket_begin, dim3(grid_for(n + 1)), dim3(kBlock), 0, s, d.fnode[1][d.fcur[1]],
+                       d.fsize + 2 + d.fcur[1], (uint32_t)std::min<int64_t>(d.front_cap[1], 0xFFFFFFFFll), n, d.extbeg);
 }
 
 void launch_generate(DevState& d, int32_t n, hipStream_t s) {
     reset_step_counters(d, s);
-    hipMemsetAsync(d.intcnt, 0, sizeof(uint32_t) * n, s);
-    hipMemsetAsync(d.extcnt, 0, sizeof(uint32_t) * ((size_t)n + 1), s);
-    hipMemsetAsync(d.fsize + 2 * (d.cur ^ 1), 0, sizeof(uint32_t) * 2, s);
-    FrontIO io = make_io(d);
-    if (n >= 2) hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, io);
-    d.cur ^= 1;
-    launch_order_fronts(d, n, s);
-}
-
-// ---------------------------------------------------------------- dynamic work fetch
-// Each wave takes 64 consecutive items at a time from a device cursor.
-__device__ inline uint32_t wave_fetch(uint32_t* cursor) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(cursor, 64u);
-    return __shfl(base, 0);
+    FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
+    hipMemsetAsync(fi.count, 0, sizeof(uint32_t), s);
+    hipMemsetAsync(fe.count, 0, sizeof(uint32_t), s);
+    if (n >= 2) hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, fi, fe);
+    d.fcur[0] ^= 1;
+    d.fcur[1] ^= 1;
+    if (n >= 2) launch_order_fronts(d, n, s);
 }
 
 // ---------------------------------------------------------------- UPDATE (prune / sprout)
-__global__ void __launch_bounds__(kBlock) k_update(DevState d, FrontIO io) {
-    const uint32_t ni = min(io.fsz[0], io.cap_i), ne = min(io.fsz[1], io.cap_e), total = ni + ne;
+__global__ void __launch_bounds__(kBlock) k_update(DevState d, FrontIn in, FrontSink fi, FrontSink fe) {
+    __shared__ Stages st;
+    stages_init(st);
+    const Out out = make_out(st, fi, fe, d);
+    const uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]), total = ni + ne;
     uint32_t visits = 0;
-    for (;;) {
-        const uint32_t base = wave_fetch(&d.ctr[C_WORK]);
-        if (base >= total) break;
-        const uint32_t k = base + lane_id();
-        if (k >= total) continue;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
         if (k < ni) {
-            const int2 e = io.fi[k];
-            int32_t st = e.y;
-            const float4 qlo = d.leaf[e.x].lo, qhi = d.leaf[e.x].hi;
-            const Box q = rec_box(qlo, qhi);
-            const float4 lo = d.inode[st].lo, hi = d.inode[st].hi;
+            const int32_t q = in.query[0][k];
+            int32_t node = (int32_t)in.node[0][k];
+            const float4 qlo = d.leaf[q].lo, qhi = d.leaf[q].hi;
+            const Box qb = rec_box(qlo, qhi);
+            const float4 lo = d.inode[node].lo, hi = d.inode[node].hi;
             ++visits;
-            if (!box_overlap(q, lo, hi)) {  // prune: climb the left spine while not overlapping
+            if (!box_overlap(qb, lo, hi)) {  // prune: climb the left spine while not overlapping
                 const int32_t t = max(d.leaf_lca[f2i(lo.w)] >> 1, 0);
-                for (st--; st >= t; st--) {
+                for (node--; node >= t; node--) {
                     ++visits;
-                    if (node_overlap(d, st, q)) break;
+                    if (node_overlap(d, node, qb)) break;
                 }
-                if (st < t && st + 1 > 0) {
+                bool keep = true;
+                if (node < t && node + 1 > 0) {
                     ++visits;
-                    if (!node_overlap(d, d.int_par[st + 1], q)) continue;  // covered by a left sibling
+                    keep = node_overlap(d, d.int_par[node + 1], qb);  // else a left sibling's entry covers it
                 }
-                push_int(d, io, e.x, st + 1);
-                continue;
+                if (keep) out.int_entry(q, node + 1);
+            } else {
+                const int32_t start = (d.int_lc[node] << 1) | (int32_t)(d.int_mark[node] & 1u);
+                visits += stackless<true>(d, out, q, qb, start, f2i(hi.w));  // sprout
             }
-            const int32_t start = (d.int_lc[st] << 1) | (int32_t)(d.int_mark[st] & 1u);
-            visits += stackless<true>(d, io, e.x, q, start, f2i(hi.w));  // sprout
         } else {
-            const int2 e = io.fe[k - ni];
-            const int32_t j = e.y;
-            const float4 qlo = d.leaf[e.x].lo, qhi = d.leaf[e.x].hi;
-            const Box q = rec_box(qlo, qhi);
+            const int32_t q = in.query[1][k - ni];
+            const int32_t j = (int32_t)in.node[1][k - ni];
+            const float4 qlo = d.leaf[q].lo, qhi = d.leaf[q].hi;
+            const Box qb = rec_box(qlo, qhi);
             const float4 llo = d.leaf[j].lo, lhi = d.leaf[j].hi;
             ++visits;
-            if (box_overlap(q, llo, lhi)) {
-                push_ext(d, io, e.x, j);
-                push_pair(d, e.x, j);
-                continue;
-            }
-            const int32_t gfa = f2i(llo.w);
-            ++visits;
-            if (node_overlap(d, gfa, q)) {
-                push_ext(d, io, e.x, j);
-                continue;
-            }
-            if (d.leaf_mark[j] & 4u) continue;  // right child: the left sibling's entry climbs
-            const int32_t top = max(d.leaf_lca[j] >> 1, 0);
-            int32_t st;
-            for (st = gfa - 1; st >= top; st--) {
+            if (box_overlap(qb, llo, lhi)) {
+                out.ext_entry(q, j);
+                out.pair(q, j);
+            } else {
+                const int32_t gfa = f2i(llo.w);
                 ++visits;
-                if (node_overlap(d, st, q)) break;
+                if (node_overlap(d, gfa, qb)) {
+                    out.ext_entry(q, j);
+                } else if (!(d.leaf_mark[j] & 4u)) {  // right child: the left sibling's entry climbs
+                    const int32_t top = max(d.leaf_lca[j] >> 1, 0);
+                    int32_t node;
+                    for (node = gfa - 1; node >= top; node--) {
+                        ++visits;
+                        if (node_overlap(d, node, qb)) break;
+                    }
+                    bool keep = true;
+                    if (node < top && node + 1 > 0) {
+                        ++visits;
+                        keep = node_overlap(d, d.int_par[node + 1], qb);
+                    }
+                    if (keep) out.int_entry(q, node + 1);
+                }
             }
-            if (st < top && st + 1 > 0) {
-                ++visits;
-                if (!node_overlap(d, d.int_par[st + 1], q)) continue;
-            }
-            push_int(d, io, e.x, st + 1);
         }
     }
+    out.final_flush();
     add_visits(d, visits);
 }
 
@@ -275,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) k_quality(int32_t n, DevState d) {
     const int32_t x = d.int_rx[i], y = d.int_ry[i];
     const int intCnt = (int)(d.intbeg[i + y - x] - d.intbeg[i]);
     const int extCnt = (int)(d.extbeg[y + 1] - d.extbeg[x]);
-    const float snap1 = d.snapshot[i] + 1.0f;
+    const float snap1 = d.snapshot[i] + 1.0f;  // a float sum, as in the reference expression
     if (1.0 * intCnt / extCnt > (double)snap1 - 0.7 * intCnt / (double)d.intbeg[I]) {
         atomicAdd(&d.intrange[i], 1);
         atomicAdd(&d.intrange[i + y - x], -1);
@@ -298,90 +431,92 @@ void launch_snapshot(DevState& d, int32_t n, hipStream_t s) {
 
 void launch_update(DevState& d, int32_t n, int quality, hipStream_t s) {
     reset_step_counters(d, s);
-    hipMemsetAsync(d.intcnt, 0, sizeof(uint32_t) * n, s);
-    hipMemsetAsync(d.extcnt, 0, sizeof(uint32_t) * ((size_t)n + 1), s);
-    hipMemsetAsync(d.fsize + 2 * (d.cur ^ 1), 0, sizeof(uint32_t) * 2, s);
-    hipMemsetAsync(d.ctr + C_WORK, 0, sizeof(uint32_t), s);
-    FrontIO io = make_io(d);
-    hipLaunchKernelGGL(k_update, dim3(persistent_grid()), dim3(kBlock), 0, s, d, io);
-    d.cur ^= 1;
+    const FrontIn in = cur_front(d);
+    FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
+    hipMemsetAsync(fi.count, 0, sizeof(uint32_t), s);
+    hipMemsetAsync(fe.count, 0, sizeof(uint32_t), s);
+    hipLaunchKernelGGL(k_update, dim3(stream_grid()), dim3(kBlock), 0, s, d, in, fi, fe);
+    d.fcur[0] ^= 1;
+    d.fcur[1] ^= 1;
     if (quality && n >= 2) {
         hipMemsetAsync(d.intrange, 0, sizeof(int32_t) * n, s);
         fill_i32(d.rtroot, n, INT32_MAX, s);
         hipLaunchKernelGGL(k_quality, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d);
     }
-    launch_order_fronts(d, n, s);
+    if (n >= 2) launch_order_fronts(d, n, s);
 }
 
 // ---------------------------------------------------------------- KEEP
-__global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIO io) {
-    const uint32_t ni = min(io.fsz[0], io.cap_i), ne = min(io.fsz[1], io.cap_e), total = ni + ne;
+__global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in) {
+    __shared__ Stages st;
+    stages_init(st);
+    const Out out = make_out(st, null_sink(), null_sink(), d);
+    const uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]), total = ni + ne;
     uint32_t visits = 0;
-    for (;;) {
-        const uint32_t base = wave_fetch(&d.ctr[C_WORK]);
-        if (base >= total) break;
-        const uint32_t k = base + lane_id();
-        if (k >= total) continue;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
         if (k < ni) {
-            const int2 e = io.fi[k];
-            const float4 qlo = d.leaf[e.x].lo, qhi = d.leaf[e.x].hi;
-            const Box q = rec_box(qlo, qhi);
-            const float4 hi = d.inode[e.y].hi, lo = d.inode[e.y].lo;
+            const int32_t q = in.query[0][k];
+            const int32_t node = (int32_t)in.node[0][k];
+            const float4 qlo = d.leaf[q].lo, qhi = d.leaf[q].hi;
+            const Box qb = rec_box(qlo, qhi);
+            const float4 lo = d.inode[node].lo, hi = d.inode[node].hi;
             ++visits;
-            if (!box_overlap(q, lo, hi)) continue;
-            const int32_t start = (d.int_lc[e.y] << 1) | (int32_t)(d.int_mark[e.y] & 1u);
-            visits += stackless<false>(d, io, e.x, q, start, f2i(hi.w));
+            if (box_overlap(qb, lo, hi)) {
+                const int32_t start = (d.int_lc[node] << 1) | (int32_t)(d.int_mark[node] & 1u);
+                visits += stackless<false>(d, out, q, qb, start, f2i(hi.w));
+            }
         } else {
-            const int2 e = io.fe[k - ni];
-            const float4 alo = d.leaf[e.x].lo, ahi = d.leaf[e.x].hi;
-            const float4 blo = d.leaf[e.y].lo, bhi = d.leaf[e.y].hi;
+            const int32_t q = in.query[1][k - ni];
+            const int32_t j = (int32_t)in.node[1][k - ni];
+            const float4 alo = d.leaf[q].lo, ahi = d.leaf[q].hi;
+            const float4 blo = d.leaf[j].lo, bhi = d.leaf[j].hi;
             ++visits;
-            if (box_overlap(rec_box(alo, ahi), blo, bhi)) push_pair(d, e.x, e.y);
+            if (box_overlap(rec_box(alo, ahi), blo, bhi)) out.pair(q, j);
         }
     }
+    stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
     add_visits(d, visits);
 }
 
 void launch_keep(DevState& d, int32_t n, hipStream_t s) {
     reset_step_counters(d, s);
-    hipMemsetAsync(d.ctr + C_WORK, 0, sizeof(uint32_t), s);
-    FrontIO io = make_io(d);
-    hipLaunchKernelGGL(k_keep, dim3(persistent_grid()), dim3(kBlock), 0, s, d, io);
+    hipLaunchKernelGGL(k_keep, dim3(stream_grid()), dim3(kBlock), 0, s, d, cur_front(d));
 }
 
 // ---------------------------------------------------------------- REORDER (balance)
-// Internal entries re-bucketed by node: non-overlapping ones from the bucket start, overlapping
-// ones from the bucket end; the external front is copied. Pairs as in KEEP.
-__global__ void __launch_bounds__(kBlock) k_reorder_int(DevState d, FrontIO io) {
-    const uint32_t ni = min(io.fsz[0], io.cap_i);
+// Pairs as in KEEP from the unchanged cut; then the internal entries of each node bucket are
+// re-ordered with the non-overlapping ones first (sort key node * 2 + overlap).
+__global__ void __launch_bounds__(kBlock) k_reorder_keys(DevState d, FrontIn in, uint32_t* __restrict__ keys) {
+    const uint32_t ni = min(*in.size[0], in.cap[0]);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ni; k += gridDim.x * blockDim.x) {
-        const int2 e = io.fi[k];
-        const float4 qlo = d.leaf[e.x].lo, qhi = d.leaf[e.x].hi;
-        const bool ov = node_overlap(d, e.y, rec_box(qlo, qhi));
-        uint32_t pos;
-        if (!ov) pos = d.intbeg[e.y] + atomicAdd(&d.cursor[e.y], 1u);
-        else pos = d.intbeg[e.y + 1] - 1u - atomicAdd(&d.intback[e.y], 1u);
-        if (pos < io.cap_i) io.oi[pos] = e;
+        const uint32_t node = in.node[0][k];
+        const int32_t q = in.query[0][k];
+        const float4 qlo = d.leaf[q].lo, qhi = d.leaf[q].hi;
+        const bool ov = node_overlap(d, (int32_t)node, rec_box(qlo, qhi));
+        keys[k] = node * 2u + (ov ? 1u : 0u);
     }
 }
-__global__ void __launch_bounds__(kBlock) k_copy_ext(FrontIO io) {
-    const uint32_t ne = min(io.fsz[1], io.cap_e);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ne; k += gridDim.x * blockDim.x) io.oe[k] = io.fe[k];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        io.osz[0] = io.fsz[0];
-        io.osz[1] = io.fsz[1];
-    }
+__global__ void __launch_bounds__(kBlock) k_halve(uint32_t* __restrict__ keys, const uint32_t* dn, uint32_t cap) {
+    const uint32_t m = min(*dn, cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) keys[k] >>= 1;
 }
 
 void launch_reorder(DevState& d, int32_t n, hipStream_t s) {
-    launch_keep(d, n, s);  // pairs from the unchanged cut
-    launch_exscan(d, d.intcnt, d.intbeg, n, nullptr, s);
-    hipMemsetAsync(d.cursor, 0, sizeof(uint32_t) * n, s);
-    hipMemsetAsync(d.intback, 0, sizeof(uint32_t) * n, s);
-    FrontIO io = make_io(d);
-    hipLaunchKernelGGL(k_reorder_int, dim3(persistent_grid()), dim3(kBlock), 0, s, d, io);
-    hipLaunchKernelGGL(k_copy_ext, dim3(persistent_grid()), dim3(kBlock), 0, s, io);
-    d.cur ^= 1;
+    launch_keep(d, n, s);
+    if (n < 2) return;
+    const int c = d.fcur[0], o = c ^ 1;
+    const FrontIn in = cur_front(d);
+    // keys (node * 2 + overlap) into the spare node array; the query ids stay as payload
+    hipLaunchKernelGGL(k_reorder_keys, dim3(stream_grid()), dim3(kBlock), 0, s, d, in, d.fnode[0][o]);
+    // sort (fnode[o], fquery[c]) with temporaries (fnode[c], fquery[o])
+    const bool in_tmp = radix_sort_u32_dev(d, d.fnode[0][o], d.fquery[0][c], d.fnode[0][c], d.fquery[0][o],
+                                           d.front_cap[0], d.fsize + c, bits_for(2 * (int64_t)n), s);
+    const uint32_t cap = (uint32_t)std::min<int64_t>(d.front_cap[0], 0xFFFFFFFFll);
+    hipLaunchKernelGGL(k_halve, dim3(stream_grid()), dim3(kBlock), 0, s, in_tmp ? d.fnode[0][c] : d.fnode[0][o],
+                       d.fsize + c, cap);
+    // re-pair the arrays so that buffer c holds the sorted (node, query) again
+    if (in_tmp) std::swap(d.fquery[0][c], d.fquery[0][o]);  // result: (fnode[c], fquery[o])
+    else std::swap(d.fnode[0][c], d.fnode[0][o]);            // result: (fnode[o], fquery[c])
 }
 
 // ---------------------------------------------------------------- restructure decision
@@ -459,24 +594,37 @@ __device__ bool sat_tri_tri(float3 P1, float3 P2, float3 P3, float3 Q1, float3 Q
 }
 
 __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
-    const uint32_t np_ = min(d.ctr[C_BP], (uint32_t)d.pair_cap);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < np_; k += gridDim.x * blockDim.x) {
+    __shared__ int2 sbuf[kWaves][kStage];
+    __shared__ uint32_t sfill[kWaves];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) sfill[w] = 0;
+    const PairSink sink{&d.ctr[C_NP], d.np, (uint32_t)min((int64_t)d.pair_cap, (int64_t)0xFFFFFFFF), 8u};
+    const uint32_t npairs = min(d.ctr[C_BP], (uint32_t)d.pair_cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += gridDim.x * blockDim.x) {
         const int2 pr = d.bp[k];  // pr.x < pr.y
         const int32_t* fa = d.faces + 3 * (int64_t)pr.x;
         const int32_t* fb = d.faces + 3 * (int64_t)pr.y;
-        const bool hit = sat_tri_tri(ld3(d.verts, fa[0]), ld3(d.verts, fa[1]), ld3(d.verts, fa[2]),
-                                     ld3(d.verts, fb[0]), ld3(d.verts, fb[1]), ld3(d.verts, fb[2]));
-        if (hit) {
-            const uint32_t slot = wave_append(&d.ctr[C_NP], true);
-            if (slot < (uint32_t)d.pair_cap) d.np[slot] = pr;
-            else atomicOr(&d.ctr[C_OVF], 8u);
+        const int32_t a0 = fa[0], a1 = fa[1], a2 = fa[2], b0 = fb[0], b1 = fb[1], b2 = fb[2];
+        // A shared vertex has bitwise-equal translated coordinates on both sides, so while
+        // every product stays finite (all |coordinates| <= 1e8) its projection lies in both
+        // intervals on every axis: the SAT always reports a hit, and the test is skipped.
+        const bool share = a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 ||
+                           a2 == b1 || a2 == b2;
+        bool hit;
+        if (share && d.tri_finite[pr.x] && d.tri_finite[pr.y]) {
+            hit = true;
+        } else {
+            hit = sat_tri_tri(ld3(d.verts, a0), ld3(d.verts, a1), ld3(d.verts, a2), ld3(d.verts, b0),
+                              ld3(d.verts, b1), ld3(d.verts, b2));
         }
+        stage_push(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w], hit, pr);
     }
+    stage_final(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w]);
 }
 
 void launch_narrow(DevState& d, hipStream_t s) {
     hipMemsetAsync(d.ctr + C_NP, 0, sizeof(uint32_t), s);
-    hipLaunchKernelGGL(k_narrow, dim3(persistent_grid() * 2), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_narrow, dim3(stream_grid() * 2), dim3(kBlock), 0, s, d);
 }
 
 // ---------------------------------------------------------------- pair-set digest

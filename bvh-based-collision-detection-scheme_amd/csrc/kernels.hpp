@@ -12,6 +12,10 @@ void launch_codes(DevState& d, int32_t T, uint32_t* codes, int32_t* vals, hipStr
 void launch_exscan(DevState& d, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, hipStream_t s);
 void radix_sort_u32(DevState& d, uint32_t* keys, int32_t* vals, uint32_t* tkeys, int32_t* tvals, int64_t n, int nbits,
                     hipStream_t s);
+// n_max bounds the count; the actual count is read on the device from *dn. Returns true when
+// the result is in (tkeys, tvals).
+bool radix_sort_u32_dev(DevState& d, uint32_t* keys, int32_t* vals, uint32_t* tkeys, int32_t* tvals, int64_t n_max,
+                        const uint32_t* dn, int nbits, hipStream_t s);
 void radix_sort_u64(DevState& d, uint64_t* keys, int32_t* vals, uint64_t* tkeys, int32_t* tvals, int64_t n, int nbits,
                     hipStream_t s);
 void launch_emit_full(DevState& d, int32_t n, hipStream_t s);
@@ -19,15 +23,16 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
                       int32_t* split_lo, int32_t* split_hi, int32_t* base, int32_t* keep_par, uint32_t* keep_mark,
                       hipStream_t s);
 void launch_refit(DevState& d, int32_t n, hipStream_t s);
+void launch_refit_prep(DevState& d, int32_t n, hipStream_t s);  // after every topology change
 void launch_prim_map(DevState& d, int32_t n, hipStream_t s);
 
 // front.hip
 void launch_pure_cd(DevState& d, int32_t n, hipStream_t s);
-void launch_generate(DevState& d, int32_t n, hipStream_t s);          // writes buffer d.cur ^ 1, then flips
+void launch_generate(DevState& d, int32_t n, hipStream_t s);          // traversal + ordering
 void launch_update(DevState& d, int32_t n, int quality, hipStream_t s);// prune/sprout, quality, ordering
 void launch_keep(DevState& d, int32_t n, hipStream_t s);
 void launch_reorder(DevState& d, int32_t n, hipStream_t s);
-void launch_order_fronts(DevState& d, int32_t n, hipStream_t s);      // counting sort by node
+void launch_order_fronts(DevState& d, int32_t n, hipStream_t s);      // radix sort by node + bucket starts
 void launch_snapshot(DevState& d, int32_t n, hipStream_t s);
 void launch_narrow(DevState& d, hipStream_t s);
 void launch_digest(DevState& d, int filtered, hipStream_t s);

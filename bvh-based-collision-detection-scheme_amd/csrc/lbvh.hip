@@ -18,7 +18,8 @@
 //  * Sorting is an LSD radix sort with 8-bit digits over an LDS-staged tile of 4096 keys:
 //    per-tile digit counts, a device-wide exclusive scan of the digit-major count table,
 //    then a stable wave64 multisplit ranking in LDS and coalesced scatter.
-//  * Refit climbs from the leaves with one agent-scope acquire/release counter per node.
+//  * Refit is two-level: chunk-local subtrees climb through LDS inside one workgroup; only
+//    the few nodes spanning chunk boundaries use agent-scope acquire/release counters.
 #include "bvhcd_internal.hpp"
 #include "kernels.hpp"
 
@@ -182,9 +183,10 @@ constexpr int kSortTile = kBlock * kSortItems;  // 4096 keys per tile
 constexpr int kRadix = 256;
 
 template <class K>
-__global__ void __launch_bounds__(kBlock) k_sort_hist(const K* __restrict__ keys, int64_t n, int shift,
-                                                      uint32_t* __restrict__ hist, int64_t tiles) {
+__global__ void __launch_bounds__(kBlock) k_sort_hist(const K* __restrict__ keys, int64_t n, const uint32_t* dn,
+                                                      int shift, uint32_t* __restrict__ hist, int64_t tiles) {
     __shared__ uint32_t h[kRadix];
+    if (dn) n = min(n, (int64_t)*dn);
     h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
@@ -200,8 +202,9 @@ __global__ void __launch_bounds__(kBlock) k_sort_hist(const K* __restrict__ keys
 // wave, then lane) with a wave64 multisplit: 8 ballots give the peers with an equal digit.
 template <class K>
 __global__ void __launch_bounds__(kBlock) k_sort_scatter(const K* __restrict__ keys, const int32_t* __restrict__ vals,
-                                                         int64_t n, int shift, const uint32_t* __restrict__ hist,
-                                                         int64_t tiles, K* __restrict__ okeys, int32_t* __restrict__ ovals) {
+                                                         int64_t n, const uint32_t* dn, int shift,
+                                                         const uint32_t* __restrict__ hist, int64_t tiles,
+                                                         K* __restrict__ okeys, int32_t* __restrict__ ovals) {
     __shared__ uint32_t tile_start[kRadix];   // exclusive scan of the tile's digit counts
     __shared__ uint32_t gofs[kRadix];         // global offset of the tile's first key per digit
     __shared__ uint32_t run[kRadix];          // per-digit running count over rounds
@@ -211,7 +214,9 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(const K* __restrict__ k
     __shared__ uint8_t sdig[kSortTile];
 
     const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    if (dn) n = min(n, (int64_t)*dn);
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    if (base >= n) return;  // whole block uniform: before any barrier
     const int valid = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
 
     // tile digit counts
@@ -278,29 +283,45 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(const K* __restrict__ k
     }
 }
 
-// Stable LSD sort of (keys, vals) over bits [0, nbits). Ping-pongs between the given buffers;
-// the result ends in (keys, vals) (an even number of passes is arranged by copying if needed).
+// Stable LSD sort of (keys, vals) over bits [0, nbits), ping-ponging between the given
+// buffers. n is the element count, or (dn != nullptr) its upper bound with the actual count
+// read on the device. Returns true when the result sits in (tkeys, tvals) (odd pass count).
 template <class K>
-void radix_sort_impl(DevState& d, K* keys, int32_t* vals, K* tkeys, int32_t* tvals, int64_t n, int nbits,
-                     hipStream_t s) {
-    if (n <= 1) return;
+bool radix_sort_passes(DevState& d, K* keys, int32_t* vals, K* tkeys, int32_t* tvals, int64_t n, const uint32_t* dn,
+                       int nbits, hipStream_t s) {
+    if (n <= 1 && !dn) return false;
+    if (n < 1) return false;
     const int64_t tiles = (n + kSortTile - 1) / kSortTile;
     const int passes = (nbits + 7) / 8;
     K* ik = keys; int32_t* iv = vals;
     K* ok = tkeys; int32_t* ov = tvals;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_sort_hist<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ik, n, shift, d.sort_hist, tiles);
+        hipLaunchKernelGGL(k_sort_hist<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ik, n, dn, shift, d.sort_hist,
+                           tiles);
         launch_exscan(d, d.sort_hist, d.sort_hist, (int64_t)kRadix * tiles, nullptr, s);
-        hipLaunchKernelGGL(k_sort_scatter<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ik, iv, n, shift, d.sort_hist,
-                           tiles, ok, ov);
+        hipLaunchKernelGGL(k_sort_scatter<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ik, iv, n, dn, shift,
+                           d.sort_hist, tiles, ok, ov);
         std::swap(ik, ok);
         std::swap(iv, ov);
     }
-    if (ik != keys) {
-        hipMemcpyAsync(keys, ik, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
-        hipMemcpyAsync(vals, iv, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
+    return passes & 1;
+}
+
+// Result always ends in (keys, vals).
+template <class K>
+void radix_sort_impl(DevState& d, K* keys, int32_t* vals, K* tkeys, int32_t* tvals, int64_t n, int nbits,
+                     hipStream_t s) {
+    if (n <= 1) return;
+    if (radix_sort_passes<K>(d, keys, vals, tkeys, tvals, n, nullptr, nbits, s)) {
+        hipMemcpyAsync(keys, tkeys, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(vals, tvals, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
     }
+}
+
+bool radix_sort_u32_dev(DevState& d, uint32_t* keys, int32_t* vals, uint32_t* tkeys, int32_t* tvals, int64_t n_max,
+                        const uint32_t* dn, int nbits, hipStream_t s) {
+    return radix_sort_passes<uint32_t>(d, keys, vals, tkeys, tvals, n_max, dn, nbits, s);
 }
 
 void radix_sort_u32(DevState& d, uint32_t* keys, int32_t* vals, uint32_t* tkeys, int32_t* tvals, int64_t n, int nbits,
@@ -490,6 +511,7 @@ __global__ void __launch_bounds__(kBlock) k_write_internal(int32_t n, const int3
     d.int_rc[id] = rc;
     d.int_par[id] = par;
     d.int_mark[id] = mark;
+    d.itopo[id] = make_int4(lc, rc, par, (int32_t)mark);
     d.int_rx[id] = s_a[k];
     d.int_ry[id] = s_b[k];
     d.inode[id].lo.w = i2f(s_a[k]);
@@ -607,50 +629,155 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
 }
 
 // ======================================================================= refit
-// Leaf boxes from the triangles, then a bottom-up climb: the second thread to reach a node
-// (agent-scope acq_rel counter) merges both children and continues upwards.
-__global__ void __launch_bounds__(kBlock) k_refit(int32_t n, const int32_t* __restrict__ faces,
-                                                  const float* __restrict__ verts, const int32_t* __restrict__ prim_idx,
-                                                  NodeRec* __restrict__ leaf, NodeRec* __restrict__ inode,
-                                                  const int32_t* __restrict__ leaf_par,
-                                                  const int32_t* __restrict__ int_par,
-                                                  const int32_t* __restrict__ int_lc, const int32_t* __restrict__ int_rc,
-                                                  const uint32_t* __restrict__ int_mark, uint32_t* __restrict__ flag) {
+// Two-level bottom-up refit. Leaves are cut into chunks of kRefitChunk consecutive slots; a
+// node whose leaf range lies inside one chunk is "local" and is refit inside that chunk's
+// workgroup through LDS (boxes, arrival counters with workgroup-scope ordering). A local
+// node's LDS slot is (id - r) + (rx(r) - chunk start), r = its maximal local subtree root:
+// preorder makes every local subtree a contiguous id range next to a contiguous leaf range,
+// so the slots of one chunk never collide. Only nodes spanning two chunks (a few per chunk
+// boundary) climb through global memory with agent-scope acquire/release counters.
+constexpr int kRefitChunk = 1024;
+
+// Per leaf: maximal local root (internal id, or -1 when the parent already spans two chunks)
+// and the LDS slot base of that local subtree. Recomputed after every topology change.
+__global__ void __launch_bounds__(kBlock) k_refit_prep(int32_t n, const int32_t* __restrict__ leaf_par,
+                                                       const int32_t* __restrict__ int_par,
+                                                       const int32_t* __restrict__ int_rx,
+                                                       const int32_t* __restrict__ int_ry, int32_t* __restrict__ lroot,
+                                                       int32_t* __restrict__ lbase) {
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
-    Box b = tri_box(faces, verts, prim_idx[x]);
-    float4* lp = reinterpret_cast<float4*>(&leaf[x]);
-    lp[0].x = b.mnx; lp[0].y = b.mny; lp[0].z = b.mnz;
-    lp[1].x = b.mxx; lp[1].y = b.mxy; lp[1].z = b.mxz;
-    if (n < 2) return;
+    const int32_t s = (x / kRefitChunk) * kRefitChunk;
+    const int32_t e = min(s + kRefitChunk, n);
+    int32_t r = -1;
     int32_t p = leaf_par[x];
-    int32_t self = x;
-    bool self_leaf = true;
+    for (int32_t guard = 0; p >= 0 && guard < kRefitChunk; ++guard) {
+        if (int_rx[p] < s || int_ry[p] >= e) break;
+        r = p;
+        p = int_par[p];
+    }
+    lroot[x] = r;
+    lbase[x] = r >= 0 ? int_rx[r] - s - r : 0;
+}
+
+void launch_refit_prep(DevState& d, int32_t n, hipStream_t s) {
+    if (n >= 2)
+        hipLaunchKernelGGL(k_refit_prep, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.leaf_par, d.int_par, d.int_rx,
+                           d.int_ry, d.lroot, d.lbase);
+}
+
+__device__ inline void store_box(NodeRec* rec, const Box& b) {
+    float4* p = reinterpret_cast<float4*>(rec);
+    p[0].x = b.mnx; p[0].y = b.mny; p[0].z = b.mnz;
+    p[1].x = b.mxx; p[1].y = b.mxy; p[1].z = b.mxz;
+}
+
+// Agent-scope (device-coherent) relaxed store / load of one float: used only to hand a
+// child box to the workgroup that finishes a chunk-spanning node. Ordering comes from
+// waiting for the stores before the arrival counter moves (s_waitcnt) and from issuing the
+// loads only after the counter's return value is known, so no L2 write-back or
+// invalidate (an agent-scope release/acquire fence) is needed on this path.
+__device__ inline void st_agent(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __builtin_bit_cast(uint32_t, v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float ld_agent(const float* p) {
+    return __builtin_bit_cast(float, __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Continue the climb above the chunk from `self` (leaf or internal) holding box `b`.
+__device__ void refit_global_climb(int32_t n, int32_t self, bool self_leaf, Box b, const DevState& d) {
+    int32_t p = self_leaf ? d.leaf_par[self] : d.int_par[self];
     for (int32_t guard = 0; p >= 0 && guard < n; ++guard) {
-        const uint32_t old = __hip_atomic_fetch_add(&flag[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) break;
-        const uint32_t m = int_mark[p];
-        const int32_t l = int_lc[p], r = int_rc[p];
-        // sibling of `self`
-        const bool self_is_left = ((m & 1u) != 0) == self_leaf && l == self;
-        const int32_t sib = self_is_left ? r : l;
-        const bool sib_leaf = self_is_left ? (m & 2u) : (m & 1u);
-        const NodeRec* sr = sib_leaf ? &leaf[sib] : &inode[sib];
-        const float4 slo = sr->lo, shi = sr->hi;
-        b = box_merge(b, rec_box(slo, shi));
-        float4* ip = reinterpret_cast<float4*>(&inode[p]);
-        ip[0].x = b.mnx; ip[0].y = b.mny; ip[0].z = b.mnz;
-        ip[1].x = b.mxx; ip[1].y = b.mxy; ip[1].z = b.mxz;
+        const int4 tp = d.itopo[p];  // lc, rc, par, mark
+        const uint32_t m = (uint32_t)tp.w;
+        const bool self_is_left = (((m & 1u) != 0) == self_leaf) && tp.x == self;
+        float* mine = d.gpart + 12 * (int64_t)p + (self_is_left ? 0 : 6);
+        st_agent(mine + 0, b.mnx); st_agent(mine + 1, b.mny); st_agent(mine + 2, b.mnz);
+        st_agent(mine + 3, b.mxx); st_agent(mine + 4, b.mxy); st_agent(mine + 5, b.mxz);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);  // the six stores are complete before the counter moves
+        const uint32_t old = __hip_atomic_fetch_add(&d.int_flag[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (old == 0u) return;  // the sibling's thread finishes p
+        const float* other = d.gpart + 12 * (int64_t)p + (self_is_left ? 6 : 0);
+        const Box o{ld_agent(other + 0), ld_agent(other + 1), ld_agent(other + 2),
+                    ld_agent(other + 3), ld_agent(other + 4), ld_agent(other + 5)};
+        b = box_merge(b, o);
+        store_box(&d.inode[p], b);  // read by later kernels only
         self = p;
         self_leaf = false;
-        p = int_par[p];
+        p = tp.z;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_refit(int32_t n, DevState d) {
+    __shared__ Box leafbox[kRefitChunk];
+    __shared__ Box nodebox[kRefitChunk];
+    __shared__ uint32_t flag[kRefitChunk];
+    const int32_t s = blockIdx.x * kRefitChunk;
+    const int32_t e = min(s + kRefitChunk, n);
+    for (int i = threadIdx.x; i < kRefitChunk; i += kBlock) flag[i] = 0;
+    // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase
+    for (int32_t x = s + threadIdx.x; x < e; x += kBlock) {
+        const int32_t t = d.prim_idx[x];
+        const int32_t* f = d.faces + 3 * (int64_t)t;
+        bool fin = true;
+        Box b = empty_box();
+        for (int k = 0; k < 3; ++k) {
+            const float* v = d.verts + 3 * (int64_t)f[k];
+            const float vx = v[0], vy = v[1], vz = v[2];
+            // |x| <= 1e8 keeps every SAT product finite (dots of edge x normal crosses stay
+            // below 1536 * 1e32): no NaN can drop the shared vertex's projection.
+            fin = fin && fabsf(vx) <= 1e8f && fabsf(vy) <= 1e8f && fabsf(vz) <= 1e8f;
+            box_add(b, vx, vy, vz);
+        }
+        d.tri_finite[t] = fin ? 1 : 0;
+        leafbox[x - s] = b;
+        store_box(&d.leaf[x], b);
+    }
+    __syncthreads();
+    if (n < 2) return;
+    for (int32_t x = s + threadIdx.x; x < e; x += kBlock) {
+        Box b = leafbox[x - s];
+        const int32_t root = d.lroot[x];
+        if (root < 0) {  // parent spans two chunks
+            refit_global_climb(n, x, true, b, d);
+            continue;
+        }
+        const int32_t base = d.lbase[x];
+        int32_t self = x;
+        bool self_leaf = true;
+        int32_t p = d.leaf_par[x];
+        for (int32_t guard = 0; guard < kRefitChunk; ++guard) {
+            const int4 tp = d.itopo[p];  // issued before the arrival counter: off the critical path
+            const int32_t slot = p + base;
+            const uint32_t old =
+                __hip_atomic_fetch_add(&flag[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) break;  // sibling not done yet: it continues
+            const uint32_t m = (uint32_t)tp.w;
+            const bool self_is_left = (((m & 1u) != 0) == self_leaf) && tp.x == self;
+            const int32_t sib = self_is_left ? tp.y : tp.x;
+            const bool sib_leaf = self_is_left ? (m & 2u) : (m & 1u);
+            b = box_merge(b, sib_leaf ? leafbox[sib - s] : nodebox[sib + base]);
+            nodebox[slot] = b;
+            store_box(&d.inode[p], b);
+            if (p == root) {
+                refit_global_climb(n, p, false, b, d);
+                break;
+            }
+            self = p;
+            self_leaf = false;
+            p = tp.z;
+        }
     }
 }
 
 void launch_refit(DevState& d, int32_t n, hipStream_t s) {
     if (n >= 2) fill_u32(d.int_flag, n - 1, 0u, s);
-    hipLaunchKernelGGL(k_refit, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.faces, d.verts, d.prim_idx, d.leaf, d.inode,
-                       d.leaf_par, d.int_par, d.int_lc, d.int_rc, d.int_mark, d.int_flag);
+    const unsigned chunks = (unsigned)((n + kRefitChunk - 1) / kRefitChunk);
+    hipLaunchKernelGGL(k_refit, dim3(chunks), dim3(kBlock), 0, s, n, d);
 }
 
 // ======================================================================= prim map / misc

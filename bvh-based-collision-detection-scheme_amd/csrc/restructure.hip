@@ -60,6 +60,7 @@ int restructure_subtrees(DevState& d, int32_t n, hipStream_t s) {
     launch_prim_map(d, n, s);
     launch_emit_runs(d, n, d.rt_of_leaf, d.r_leaf_lo, d.r_leaf_hi, d.r_split_lo, d.r_split_hi, d.r_base, d.r_keep_par,
                      d.r_keep_mark, s);
+    launch_refit_prep(d, n, s);
     launch_refit(d, n, s);
     return BVHCD_OK;
 }

@@ -156,6 +156,28 @@ def test_degenerate_inputs(pkg, oracle):
     ctx.close()
 
 
+def test_huge_and_shared_vertices(pkg, oracle):
+    """Vertex-sharing pairs at extreme coordinates: the narrow phase's shared-vertex shortcut
+    only applies below |x| = 1e8; above it the full SAT decides, as in the oracle."""
+    print("[gpu] huge coordinates", flush=True)
+    rng = np.random.default_rng(9)
+    T = 300
+    v = (rng.normal(size=(T + 2, 3)) * 0.01).astype(np.float32)
+    v[::7] *= np.float32(1e30)   # some vertices far beyond 1e8
+    v[3::11] = np.inf
+    f = np.stack([np.arange(T), np.arange(T) + 1, np.arange(T) + 2], 1).astype(np.int32)  # strip
+    ctx = pkg.CollisionContext(T, T + 2)
+    ctx.set_topology(f, T + 2)
+    ctx.update_vertices(v)
+    ctx.step(0, 1)
+    orc = oracle.OracleContext(f, T + 2)
+    orc.update_vertices(v)
+    orc.step(0, 1)
+    np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), oracle.pair_set(orc.pairs(False)))
+    np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), oracle.pair_set(orc.pairs(True)))
+    ctx.close()
+
+
 def test_capacity_overflow_reported(pkg):
     print("[gpu] overflow", flush=True)
     sc = pkg.Scene.preset("spheres", subdiv=3, spacing=2.0)

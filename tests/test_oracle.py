@@ -214,6 +214,21 @@ def test_sat_special_cases(oracle):
     assert oracle.sat(tri(-1, 0, 0, 1, 0, 0, 0, 0, 1), tri(0, -1, 0.5, 0, 1, 0.5, 0, 0, -1))
 
 
+def test_sat_shared_vertex_always_hits(oracle):
+    """Property behind the narrow-phase shortcut: with every |coordinate| <= 1e8, two triangles
+    that share a vertex are reported as intersecting, whatever their shape or scale."""
+    rng = np.random.default_rng(5)
+    for scale in (1e-30, 1e-6, 1.0, 1e3, 1e8):
+        for _ in range(400):
+            P = (rng.normal(size=(3, 3)) * scale).astype(np.float32)
+            P = np.clip(P, -1e8, 1e8).astype(np.float32)
+            Q = (rng.normal(size=(3, 3)) * scale).astype(np.float32)
+            Q = np.clip(Q, -1e8, 1e8).astype(np.float32)
+            i, j = rng.integers(0, 3, size=2)
+            Q[j] = P[i]
+            assert oracle.sat(P, Q) and oracle.sat(Q, P), (P, Q)
+
+
 def _sat_double(P, Q):
     """Separating-axis test in float64 with the same 17 axes; returns (hit, min margin)."""
     P = P.astype(np.float64)
