@@ -109,6 +109,8 @@ def lib():
         "bvhcd_mlp_forward": (C.c_int, [P, P, I64, P, C.c_int]),
         "bvhcd_mlp_train_step": (C.c_int, [P, P, P, I64, C.c_float, C.POINTER(C.c_float), C.c_int]),
         "bvhcd_mlp_grad": (C.c_int, [P, P, P, I64, P, C.POINTER(C.c_float), C.c_int]),
+        "bvhcd_mlp_set_threshold": (C.c_int, [P, C.c_float]),
+        "bvhcd_restructure": (C.c_int, [P, P, I64]),
         "bvhcd_batch_create": (C.c_int, [C.POINTER(Config), I32, P, P, C.POINTER(P)]),
         "bvhcd_batch_destroy": (C.c_int, [P]),
         "bvhcd_batch_set_topology": (C.c_int, [P, P, C.c_int]),
@@ -382,6 +384,15 @@ class CollisionContext:
         self._chk(lib().bvhcd_mlp_grad(self._h, _ptr(x), _ptr(y), x.shape[0], _ptr(g), C.byref(loss), MEM_HOST),
                   "mlp_grad")
         return g, float(loss.value)
+
+    def mlp_set_threshold(self, t):
+        self._chk(lib().bvhcd_mlp_set_threshold(self._h, float(t)), "mlp_set_threshold")
+
+    def restructure(self, roots):
+        """Restructure the subtrees under the given preorder internal-node ids now; the next
+        step regenerates the front."""
+        r = np.ascontiguousarray(roots, dtype=np.int32).ravel()
+        self._chk(lib().bvhcd_restructure(self._h, _ptr(r) if r.size else None, r.size), "restructure")
 
 
 def plain_schedule(frame, bvh_cycle=12, front_cycle=6):

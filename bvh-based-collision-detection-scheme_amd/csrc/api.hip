@@ -33,6 +33,7 @@ struct bvhcd_ctx {
     std::string err;
     int64_t T = 0, V = 0;
     bool built = false, front_valid = false, restr_pending = false;
+    bool learned_frame = false;  // set by the learned scheduler for its decision frames
     int64_t frame = 0;
     int last_bvh = -1, last_front = -1;
     int64_t restr_sub = 0, restr_leaf = 0;
@@ -100,6 +101,9 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.int_flag, T);
     ok &= dalloc(c, d.itopo, T);
     ok &= dalloc(c, d.tri_finite, T);
+    ok &= dalloc(c, d.sa_ref, T);
+    ok &= dalloc(c, d.sa_ref_leaf, T);
+    ok &= dalloc(c, d.built_frame, T);
     ok &= dalloc(c, d.gpart, 12 * T);
     ok &= dalloc(c, d.lroot, T);
     ok &= dalloc(c, d.lbase, T);
@@ -270,18 +274,30 @@ int build_and_refit(bvhcd_ctx* c) {
     begin(c, S_REFIT);
     launch_refit(c->d, (int32_t)c->T, c->stream);
     end(c, S_REFIT);
+    if (c->T >= 2) update_sa_ref(c->d, (int32_t)c->T, 1, (int32_t)c->frame, c->stream);
+    return BVHCD_OK;
+}
+
+int apply_restructure(bvhcd_ctx* c, int64_t nsub, int64_t nleaf) {
+    begin(c, S_RESTR);
+    int st = restructure_subtrees(c->d, (int32_t)c->T, c->stream);
+    if (st) return fail(c, st, "restructure failed");
+    update_sa_ref(c->d, (int32_t)c->T, 0, (int32_t)c->frame, c->stream);
+    end(c, S_RESTR);
+    c->restr_sub = nsub;
+    c->restr_leaf = nleaf;
     return BVHCD_OK;
 }
 
 // Restructure frame: select subtrees (reference heuristic, or the learned scheduler), then
 // rebuild (> I/2 flagged internal nodes), refit (< I/8) or restructure the selection.
-int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
+int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed, bool use_learned) {
     DevState& d = c->d;
     const int32_t n = (int32_t)c->T;
     const int64_t I = n - 1;
     begin(c, S_SCHED);
     int64_t nsub = 0, nint = 0, nleaf = 0;
-    bool learned = c->cfg.scheduler == BVHCD_SCHED_LEARNED && learned_ready(c->learned);
+    const bool learned = use_learned && learned_ready(c->learned);
     if (learned) {
         int st = learned_select(c->learned, d, n, c->frame, c->stream, &nsub, &nint, &nleaf, &c->err);
         if (st < 0) return st;
@@ -304,12 +320,8 @@ int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
         end(c, S_REFIT);
         bvh_op = BVHCD_BVH_REFIT;
     } else {
-        begin(c, S_RESTR);
-        int st = restructure_subtrees(d, n, c->stream);
-        if (st) return fail(c, st, "restructure failed");
-        end(c, S_RESTR);
-        c->restr_sub = nsub;
-        c->restr_leaf = nleaf;
+        int st = apply_restructure(c, nsub, nleaf);
+        if (st) return st;
         topo_changed = true;
     }
     return BVHCD_OK;
@@ -317,6 +329,13 @@ int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
 
 int run_bvh(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
     topo_changed = false;
+    const bool learned_frame = c->learned_frame;
+    c->learned_frame = false;
+    if (learned_frame && c->built && c->T >= 2) {
+        bvh_op = BVHCD_BVH_RESTRUCTURE;
+        c->restr_sub = c->restr_leaf = 0;
+        return run_restructure(c, bvh_op, topo_changed, true);
+    }
     if (bvh_op == BVHCD_BVH_RESTRUCTURE && (!c->built || !c->restr_pending)) bvh_op = BVHCD_BVH_REFIT;
     if (bvh_op != BVHCD_BVH_BUILD && bvh_op != BVHCD_BVH_NONE && !c->built)
         return fail(c, BVHCD_E_NOT_BUILT, "no BVH yet: first step must BUILD");
@@ -336,7 +355,7 @@ int run_bvh(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
             end(c, S_REFIT);
             break;
         case BVHCD_BVH_RESTRUCTURE: {
-            int st = run_restructure(c, bvh_op, topo_changed);
+            int st = run_restructure(c, bvh_op, topo_changed, c->cfg.scheduler == BVHCD_SCHED_LEARNED);
             if (st) return st;
             break;
         }
@@ -517,7 +536,7 @@ int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V,
     if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
     c->T = T;
     c->V = V;
-    c->built = c->front_valid = c->restr_pending = false;
+    c->built = c->front_valid = c->restr_pending = c->learned_frame = false;
     c->frame = 0;
     c->d.fcur[0] = c->d.fcur[1] = 0;
     HIPCHK(c, hipMemsetAsync(c->d.fsize, 0, 4 * sizeof(uint32_t), c->stream));
@@ -544,6 +563,19 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
     int bvh_op, front_op;
     if (c->cfg.scheduler == BVHCD_SCHED_GENERATE) return do_step(c, BVHCD_BVH_BUILD, BVHCD_FRONT_PURE_BVH_CD);
     plain_schedule(c->frame, c->cfg.bvh_cycle, c->cfg.front_cycle, bvh_op, front_op);
+    if (c->cfg.scheduler == BVHCD_SCHED_LEARNED && c->built && learned_ready(c->learned)) {
+        // Learned schedule: the mandatory rebuild and the heuristic's restructure frames become
+        // MLP decisions per subtree (restructure the chosen ones, refit the rest); the front
+        // keeps the plain cycle (a rebuild frame's forced GENERATE becomes that frame's UPDATE).
+        if (bvh_op == BVHCD_BVH_BUILD) {
+            front_op = BVHCD_FRONT_UPDATE;
+            c->learned_frame = true;
+        } else if (c->cfg.enable_restructure && c->restr_pending) {
+            c->learned_frame = true;
+        }
+        bvh_op = BVHCD_BVH_REFIT;
+        return do_step(c, bvh_op, front_op);
+    }
     if (bvh_op == BVHCD_BVH_REFIT && c->cfg.enable_restructure && c->restr_pending) bvh_op = BVHCD_BVH_RESTRUCTURE;
     return do_step(c, bvh_op, front_op);
 }
@@ -780,6 +812,29 @@ int bvhcd_mlp_grad(bvhcd_ctx* c, const float* feats, const float* targets, int64
     if (!c || !feats || !targets || !grad || rows <= 0) return BVHCD_E_INVALID;
     if (!c->learned) return fail(c, BVHCD_E_INVALID, "no weights");
     return learned_grad(c->learned, feats, targets, rows, grad, loss, where, c->stream, &c->err);
+}
+int bvhcd_mlp_set_threshold(bvhcd_ctx* c, float t) {
+    if (!c || !(t == t)) return BVHCD_E_INVALID;
+    if (!c->learned) c->learned = learned_create(c->stream);
+    if (!c->learned) return fail(c, BVHCD_E_OOM, "learned state");
+    learned_set_threshold(c->learned, t);
+    return BVHCD_OK;
+}
+int bvhcd_restructure(bvhcd_ctx* c, const int32_t* roots, int64_t k) {
+    if (!c || (k > 0 && !roots) || k < 0) return BVHCD_E_INVALID;
+    if (!c->built) return fail(c, BVHCD_E_NOT_BUILT, "restructure needs a BVH");
+    if (c->T < 2 || k == 0) return BVHCD_OK;
+    int64_t nsub = 0, nint = 0, nleaf = 0;
+    int st = mark_given_roots(c->d, (int32_t)c->T, roots, k, c->stream, &nsub, &nint, &nleaf, &c->err);
+    if (st) return st;
+    const int64_t tcap = c->tcap;  // outside a frame: keep it out of the stage timing
+    c->tcap = 0;
+    st = apply_restructure(c, nsub, nleaf);
+    c->tcap = tcap;
+    if (st) return st;
+    c->front_valid = false;
+    c->restr_pending = false;
+    return check_overflow(c);
 }
 
 }  // extern "C"

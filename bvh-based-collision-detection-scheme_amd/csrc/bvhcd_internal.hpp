@@ -125,6 +125,9 @@ struct DevState {
     int32_t* int_ry = nullptr;
     uint32_t* int_flag = nullptr;   // T-1 refit arrival counters
     int4* itopo = nullptr;          // T-1 packed (lc, rc, par, mark) for the refit climb
+    float* sa_ref = nullptr;        // T-1 internal-node surface area at its last rebuild
+    float* sa_ref_leaf = nullptr;   // T leaf-box surface area at its last rebuild
+    int32_t* built_frame = nullptr; // T frame at which each leaf's subtree was last rebuilt
     uint8_t* tri_finite = nullptr;  // T (by triangle id): all 9 |coordinates| <= 1e8 (set by refit)
     float* gpart = nullptr;         // 12 floats per internal node: child boxes handed over
                                     // between workgroups at chunk-spanning nodes (left, right)

@@ -23,4 +23,11 @@ bool learned_ready(const LearnedState* L);
 // heuristic selection leaves them); returns subtree / internal-node / leaf counts.
 int learned_select(LearnedState* L, DevState& d, int32_t n, int64_t frame, hipStream_t s, int64_t* nsub,
                    int64_t* nint, int64_t* nleaf, std::string* err);
+// Surface-area references and rebuild frame (all nodes after a build; all = 0: only the
+// subtrees just restructured, from d.rt_of_leaf).
+void update_sa_ref(DevState& d, int32_t n, int all, int32_t frame, hipStream_t s);
+// Marks host-given subtree roots for restructure_subtrees (validated: disjoint ranges).
+int mark_given_roots(DevState& d, int32_t n, const int32_t* roots, int64_t k, hipStream_t s, int64_t* nsub,
+                     int64_t* nint, int64_t* nleaf, std::string* err);
+void learned_set_threshold(LearnedState* L, float t);
 }  // namespace bvhcd

@@ -152,7 +152,7 @@ int  bvhcd_export_front(bvhcd_ctx* ctx, int which, int32_t* out, int64_t cap, in
 /* Scene box of the last BUILD (6 floats). */
 int  bvhcd_export_scene_box(bvhcd_ctx* ctx, float* box6);
 
-/* ---- learned restructure scheduler (north-star pattern: MLP 16->64->64->1) ---- */
+/* ---- learned restructure scheduler (per-subtree MLP 16->64->64->1) ---- */
 #define BVHCD_MLP_IN  16
 #define BVHCD_MLP_H   64
 #define BVHCD_MLP_NPARAM (16*64 + 64 + 64*64 + 64 + 64 + 1)
@@ -170,6 +170,15 @@ int  bvhcd_mlp_train_step(bvhcd_ctx* ctx, const float* feats, const float* targe
 /* Gradients of the MSE loss for one batch (fp32), for checking against a CPU reference. */
 int  bvhcd_mlp_grad(bvhcd_ctx* ctx, const float* feats, const float* targets, int64_t rows,
                     float* grad, float* loss, int where);
+/* Score threshold of the learned selection: a candidate subtree is restructured when the
+   MLP's predicted gain exceeds it (default 0). */
+int  bvhcd_mlp_set_threshold(bvhcd_ctx* ctx, float threshold);
+/* Restructure the given subtrees (preorder internal-node ids with disjoint leaf ranges) of the
+   current BVH now, refit, and invalidate the front (the next step must GENERATE). The
+   explicit-root form of LBvhFixedDeformable::restructure (collision/lbvh/LBvhFixedDeformable.cu:210 onward,
+   whose roots come from the front's quality check); it labels training data for the learned
+   scheduler. */
+int  bvhcd_restructure(bvhcd_ctx* ctx, const int32_t* roots, int64_t k);
 
 /* ---- batched scenes (many scenes per launch; one hipGraph per frame) ---- */
 typedef struct bvhcd_batch bvhcd_batch;

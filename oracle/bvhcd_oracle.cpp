@@ -1044,4 +1044,27 @@ void orc_export_restr_log(void* p, int32_t* intrange, int32_t* rtroot) {
     if (rtroot) std::memcpy(rtroot, c->rtroot.data(), 4 * c->rtroot.size());
 }
 
+// Explicit-root restructure (same rebuild as the heuristic path, roots given by the caller):
+// validated disjoint preorder roots, one shared scene box, then a full refit; the front must be
+// regenerated afterwards. Returns 0, or -1 for a bad root / overlapping ranges.
+int orc_restructure_roots(void* p, const int32_t* roots, int64_t k, int threads) {
+    Ctx* c = static_cast<Ctx*>(p);
+    if (!c->built || c->T < 2) return -5;
+    std::vector<std::pair<int32_t, int32_t>> rg;
+    for (int64_t i = 0; i < k; ++i) {
+        if (roots[i] < 0 || roots[i] >= c->T - 1) return -1;
+        rg.push_back({c->bvh.rx[roots[i]], c->bvh.ry[roots[i]]});
+    }
+    std::sort(rg.begin(), rg.end());
+    for (size_t i = 1; i < rg.size(); ++i)
+        if (rg[i].first <= rg[i - 1].second) return -1;
+    if (k == 0) return 0;
+    Box scene = scene_box(c->T, c->faces.data(), c->verts.data());
+    for (int64_t i = 0; i < k; ++i) restructure_subtree(c->bvh, roots[i], scene, c->faces.data(), c->verts.data());
+    refit_bvh(c->bvh, c->faces.data(), c->verts.data(), threads);
+    c->front_valid = false;
+    c->restr_pending = false;
+    return 0;
+}
+
 }  // extern "C"

@@ -53,6 +53,7 @@ def lib():
         "orc_export_bvh": (None, [P] + [P] * 14),
         "orc_stats": (None, [P, P]),
         "orc_export_restr_log": (None, [P, P, P]),
+        "orc_restructure_roots": (I32, [P, P, I64, I32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -203,6 +204,12 @@ class OracleContext:
                  "leaf_par", "leaf_mark", "leaf_lca", "int_box", "leaf_box"]
         lib().orc_export_bvh(self._h, *[_p(a[k]) for k in order])
         return a
+
+    def restructure(self, roots, threads=0):
+        r = np.ascontiguousarray(roots, dtype=np.int32).ravel()
+        st = lib().orc_restructure_roots(self._h, _p(r), r.size, threads)
+        if st:
+            raise ValueError(f"orc_restructure_roots: {st}")
 
     def stats(self):
         s = np.zeros(11, np.int64)

@@ -1,0 +1,221 @@
+"""GPU tests of the learned restructure scheduler through the C ABI.
+
+* MFMA inference (bf16 weights, fp32 accumulation) against oracle/mlp_ref.py;
+* the hand-written backward pass against the reference gradient, and the Adam step;
+* per-subtree features against values recomputed on the host from the exported BVH and front;
+* explicit subtree restructure bit-exact against the oracle's restructure;
+* the learned schedule keeps every frame's pair set equal to the oracle's.
+"""
+import numpy as np
+import pytest
+
+import mlp_ref
+from conftest import load_package
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    return load_package()
+
+
+def _ctx(pkg, sc, **kw):
+    ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts, **kw)
+    ctx.set_topology(sc.faces(), sc.num_verts)
+    return ctx
+
+
+def _feature_rows(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, 16)).astype(np.float32)
+    x[:, 6:] = np.abs(x[:, 6:]) / 4
+    return x
+
+
+def test_mlp_forward_matches_reference(pkg):
+    print("[gpu] mlp forward", flush=True)
+    ctx = pkg.CollisionContext(16, 48)
+    p = mlp_ref.init_params(11)
+    ctx.mlp_set_weights(p)
+    x = _feature_rows(1000, 0)   # 1000 rows: the last wave is partly filled
+    y = ctx.mlp_forward(x).astype(np.float64)
+    yq = mlp_ref.forward(p, x, quant=True)
+    y32 = mlp_ref.forward(p, x, quant=False)
+    err = np.abs(y - yq)
+    # fp32 accumulation order differs from the float64 reference; an activation that lands on a
+    # bf16 rounding tie can round one ulp apart (2^-8 relative), hence the loose bound per row
+    assert np.max(err / (1 + np.abs(yq))) < 2e-2, np.max(err)
+    assert np.median(err) < 2e-4, np.median(err)
+    # the bf16 rounding is really applied: much closer to the quantised network
+    assert np.median(err) < 0.2 * np.median(np.abs(y - y32))
+    np.testing.assert_array_equal(ctx.mlp_get_weights(), p)
+    ctx.close()
+
+
+def test_mlp_gradient_matches_reference(pkg):
+    print("[gpu] mlp gradient", flush=True)
+    ctx = pkg.CollisionContext(16, 48)
+    p = mlp_ref.init_params(12)
+    ctx.mlp_set_weights(p)
+    x = _feature_rows(777, 1)
+    t = np.random.default_rng(2).normal(size=777).astype(np.float32)
+    g, loss = ctx.mlp_grad(x, t)
+    lref, gref = mlp_ref.loss_and_grad(p, x, t, quant=True)
+    assert loss == pytest.approx(lref, rel=1e-3)
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    assert rel < 5e-3, rel
+    # per parameter block
+    P = mlp_ref.unpack(g)
+    R = mlp_ref.unpack(gref)
+    for k in R:
+        assert np.linalg.norm(P[k] - R[k]) <= 1e-2 * np.linalg.norm(R[k]) + 1e-6, k
+    ctx.close()
+
+
+def test_mlp_training_step_and_convergence(pkg):
+    print("[gpu] mlp training", flush=True)
+    ctx = pkg.CollisionContext(16, 48)
+    p0 = mlp_ref.init_params(13)
+    ctx.mlp_set_weights(p0)
+    x = _feature_rows(2048, 3)
+    t = (np.tanh(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2] - x[:, 3] ** 2 * 0.3).astype(np.float32)
+    _, gref = mlp_ref.loss_and_grad(p0, x, t, quant=True)
+    lr = 1e-3
+    l0 = ctx.mlp_train_step(x, t, lr)
+    p1 = ctx.mlp_get_weights().astype(np.float64)
+    # first Adam step: m/(1-b1) = g, v/(1-b2) = g^2, so the step is lr * g / (|g| + eps)
+    exp, _, _ = mlp_ref.adam_step(p0.astype(np.float64), 0.0, 0.0, gref, 1, lr)
+    big = np.abs(gref) > 1e-3 * np.abs(gref).max()
+    np.testing.assert_allclose(p1[big], exp[big], rtol=0, atol=lr * 1e-2)
+    losses = [l0] + [ctx.mlp_train_step(x, t, lr) for _ in range(400)]
+    assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
+    ctx.close()
+
+
+def _host_features(bvh, fi, fe, roots, frame_age=0.0):
+    """Features 1, 3, 4 and 6-15 recomputed from the exported BVH and fronts."""
+    rx, ry, par = bvh["int_rx"], bvh["int_ry"], bvh["int_par"]
+    ext_per_leaf = np.bincount(fe[:, 1], minlength=len(bvh["leaf_par"]))
+    int_per_node = np.bincount(fi[:, 1], minlength=len(rx))
+    rows = []
+    for r in roots:
+        a, b = rx[r], ry[r]
+        m = b - a + 1
+        f = np.zeros(16)
+        f[1] = (int_per_node[r:r + m - 1].sum() + ext_per_leaf[a:b + 1].sum()) / m
+        d, q = 0, par[r]
+        while q >= 0:
+            d, q = d + 1, par[q]
+        f[3] = d / 32
+        f[4] = np.log2(m) / 12
+        c = ext_per_leaf[a:b + 1]
+        bins = np.minimum(9, np.floor(np.log2(1 + c)).astype(int))
+        f[6:] = np.bincount(bins, minlength=10) / m
+        rows.append(f)
+    return np.array(rows)
+
+
+def test_features_match_host_recount(pkg):
+    print("[gpu] features", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=96, frames=30)
+    ctx = _ctx(pkg, sc)
+    ctx.update_vertices(sc.frame(0))
+    ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
+    feats, roots = ctx.mlp_features()
+    bvh = ctx.export_bvh()
+    rx, ry, par = bvh["int_rx"], bvh["int_ry"], bvh["int_par"]
+    cnt = ry - rx + 1
+    # candidates: maximal subtrees with <= 4096 leaves, disjoint and covering every leaf
+    assert np.all(cnt[roots] <= 4096)
+    assert all(par[r] < 0 or cnt[par[r]] > 4096 for r in roots)
+    cover = np.zeros(len(bvh["leaf_par"]), np.int32)
+    for r in roots:
+        cover[rx[r]:ry[r] + 1] += 1
+    assert np.all(cover == 1)
+    fi, fe = ctx.export_front(0), ctx.export_front(1)
+    host = _host_features(bvh, fi, fe, roots)
+    cols = [1, 3, 4] + list(range(6, 16))
+    np.testing.assert_allclose(feats[:, cols], host[:, cols], rtol=1e-5, atol=1e-6)
+    # right after a build: surface areas, leaf boxes and age equal their references
+    assert np.all(feats[:, [0, 2, 5]] == 0)
+    # five frames of refit + keep: age = frames stepped since the build (6 steps in all) / 12
+    for fr in range(1, 6):
+        ctx.update_vertices(sc.frame(fr))
+        ctx.step(pkg.BVH_REFIT, pkg.FRONT_KEEP)
+    f2, r2 = ctx.mlp_features()
+    np.testing.assert_array_equal(r2, roots)
+    np.testing.assert_allclose(f2[:, 5], 6 / 12, rtol=1e-6)
+    assert np.all(np.isfinite(f2))
+    ctx.close()
+
+
+def test_explicit_restructure_matches_oracle(pkg, oracle):
+    print("[gpu] explicit restructure", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=80, frames=40)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc)
+    orc = oracle.OracleContext(f, sc.num_verts)
+    for fr in range(12):
+        v = sc.frame(fr * 3)
+        ctx.update_vertices(v)
+        orc.update_vertices(v)
+        op = (0, 1) if fr == 0 else (1, 4)
+        ctx.step(*op)
+        orc.step(*op)
+    bvh = ctx.export_bvh()
+    rx, ry = bvh["int_rx"], bvh["int_ry"]
+    cnt = ry - rx + 1
+    # disjoint roots: the maximal subtrees of <= 300 leaves, every third one
+    par = bvh["int_par"]
+    cand = [i for i in range(len(rx)) if cnt[i] <= 300 and cnt[i] >= 2 and (par[i] < 0 or cnt[par[i]] > 300)]
+    roots = np.array(cand[::3], np.int32)
+    assert roots.size > 3
+    ctx.restructure(roots)
+    orc.restructure(roots)
+    got, ref = ctx.export_bvh(), orc.export_bvh()
+    for k in ["codes", "prim_idx", "prim_map", "int_lc", "int_rc", "int_par", "int_mark", "int_rx", "int_ry",
+              "leaf_par", "leaf_mark", "leaf_lca"]:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    assert np.array_equal(got["int_box"], ref["int_box"]) and np.array_equal(got["leaf_box"], ref["leaf_box"])
+    # overlapping or bad roots are rejected
+    with pytest.raises(pkg.BvhcdError):
+        ctx.restructure(np.array([0, roots[0]], np.int32))
+    with pytest.raises(pkg.BvhcdError):
+        ctx.restructure(np.array([len(rx)], np.int32))
+    # the next frame regenerates the front: pairs and fronts equal
+    v = sc.frame(37)
+    ctx.update_vertices(v)
+    orc.update_vertices(v)
+    ctx.step(1, 4)
+    orc.step(1, 4)
+    assert ctx.stats()["last_front_op"] == pkg.FRONT_GENERATE
+    np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), oracle.pair_set(orc.pairs(True)))
+    np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), oracle.pair_set(orc.pairs(False)))
+    ctx.close()
+
+
+def test_learned_schedule_keeps_pairs(pkg, oracle):
+    """Random weights, threshold set so that some subtrees are picked: every frame's pair set
+    equals the oracle's (plain schedule)."""
+    print("[gpu] learned schedule parity", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=64, frames=36)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc, scheduler=pkg.SCHED_LEARNED)
+    ctx.mlp_set_weights(mlp_ref.init_params(21))
+    ctx.mlp_set_threshold(-1e30)   # every candidate
+    orc = oracle.OracleContext(f, sc.num_verts)
+    restr = 0
+    for fr in range(sc.frames):
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        orc.update_vertices(v)
+        ctx.step_scheduled()
+        orc.step_scheduled()
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), oracle.pair_set(orc.pairs(True)),
+                                      err_msg=f"np {fr}")
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), oracle.pair_set(orc.pairs(False)),
+                                      err_msg=f"bp {fr}")
+        restr += ctx.stats()["last_bvh_op"] == pkg.BVH_RESTRUCTURE
+    print(f"[gpu] learned restructure frames: {restr}", flush=True)
+    ctx.close()
