@@ -28,7 +28,7 @@ LIB_PATH = os.path.join(_HERE, "libbvhcd.so")
 OK, E_INVALID, E_OOM, E_CAPACITY, E_DEVICE, E_NOT_BUILT = 0, -1, -2, -3, -4, -5
 MEM_HOST, MEM_DEVICE = 0, 1
 BVH_BUILD, BVH_REFIT, BVH_RESTRUCTURE, BVH_NONE = 0, 1, 2, 3
-FRONT_PURE_BVH_CD, FRONT_GENERATE, FRONT_UPDATE, FRONT_REORDER, FRONT_KEEP = 0, 1, 2, 3, 4
+FRONT_PURE_BVH_CD, FRONT_GENERATE, FRONT_UPDATE, FRONT_REORDER, FRONT_KEEP, FRONT_RESTRUCTURE = 0, 1, 2, 3, 4, 5
 SCHED_PLAIN, SCHED_LEARNED, SCHED_GENERATE = 0, 1, 2
 SCENE_TWO_SPHERES, SCENE_CLOTH, SCENE_LAYERS, SCENE_SHEET_SPHERE = 0, 1, 2, 3
 MLP_NPARAM = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1

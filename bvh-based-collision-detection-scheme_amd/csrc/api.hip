@@ -34,6 +34,8 @@ struct bvhcd_ctx {
     int64_t T = 0, V = 0;
     bool built = false, front_valid = false, restr_pending = false;
     bool learned_frame = false;  // set by the learned scheduler for its decision frames
+    bool repair_pending = false; // front classified for the incremental front restructure
+    uint32_t repair_tasks = 0;   // its (query, subtree) walks
     int64_t frame = 0;
     int last_bvh = -1, last_front = -1;
     int64_t restr_sub = 0, restr_leaf = 0;
@@ -177,6 +179,8 @@ int allocate(bvhcd_ctx* c) {
     d.scan_tmp_n = (size_t)((std::max<int64_t>((int64_t)d.sort_hist_n, T + 2) + 4095) / 4096 + 1);
     ok &= dalloc(c, d.sort_hist, d.sort_hist_n);
     ok &= dalloc(c, d.scan_tmp, d.scan_tmp_n);
+    d.rtask_cap = std::min<int64_t>(4 * T + (1 << 20), lim);
+    ok &= dalloc(c, d.rtasks, d.rtask_cap);
     ok &= dalloc(c, d.bp, d.pair_cap);
     ok &= dalloc(c, d.np, d.pair_cap);
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
@@ -231,6 +235,7 @@ void frame_bytes(double T, double V, int bvh_op, int front_op, double visits, do
         out[S_BUILD] = 24 * V + 16 * T + (4 * T + 16.0 * 4 * T) + (4 * T + 24 * I + 12 * T + 24 * (T + I));
     if (bvh_op == BVHCD_BVH_REFIT || bvh_op == BVHCD_BVH_BUILD) out[S_REFIT] = 12 * V + 40 * T + 88 * I;
     if (bvh_op == BVHCD_BVH_RESTRUCTURE) out[S_RESTR] = 12 * V + 40 * T + 88 * I + 16.0 * 7 * restr_leaves;
+    if (front_op == BVHCD_FRONT_RESTRUCTURE) out[S_RESTR] += 16 * (fi0 + fe0);  // front classification
     const bool whole = front_op == BVHCD_FRONT_PURE_BVH_CD || front_op == BVHCD_FRONT_GENERATE;
     const double nin = whole ? T : fi0 + fe0;
     const double nout = (front_op == BVHCD_FRONT_KEEP || front_op == BVHCD_FRONT_PURE_BVH_CD) ? 0 : fi1 + fe1;
@@ -278,14 +283,30 @@ int build_and_refit(bvhcd_ctx* c) {
     return BVHCD_OK;
 }
 
-int apply_restructure(bvhcd_ctx* c, int64_t nsub, int64_t nleaf) {
+// Rebuilds the marked subtrees. repair: keep the front through an incremental front
+// restructure (applied by the next front op) instead of regenerating it; afterwards
+// c->front_valid says whether the front survived.
+int apply_restructure(bvhcd_ctx* c, int64_t nsub, int64_t nleaf, bool repair) {
+    if (c->repair_pending) c->repair_pending = c->front_valid = false;  // one repair at a time
+    const bool prep = repair && c->front_valid;
     begin(c, S_RESTR);
-    int st = restructure_subtrees(c->d, (int32_t)c->T, c->stream);
+    int st = restructure_subtrees(c->d, (int32_t)c->T, c->stream, prep);
     if (st) return fail(c, st, "restructure failed");
     update_sa_ref(c->d, (int32_t)c->T, 0, (int32_t)c->frame, c->stream);
     end(c, S_RESTR);
     c->restr_sub = nsub;
     c->restr_leaf = nleaf;
+    c->front_valid = false;
+    if (prep) {
+        uint32_t cnt = 0;
+        HIPCHK(c, hipMemcpyAsync(&cnt, c->d.ctr + C_TASKS, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if ((int64_t)cnt <= c->d.rtask_cap) {  // else the walks did not fit: regenerate
+            c->repair_pending = true;
+            c->repair_tasks = cnt;
+            c->front_valid = true;
+        }
+    }
     return BVHCD_OK;
 }
 
@@ -320,9 +341,11 @@ int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed, bool use_lear
         end(c, S_REFIT);
         bvh_op = BVHCD_BVH_REFIT;
     } else {
-        int st = apply_restructure(c, nsub, nleaf);
+        // the learned schedule repairs the front; the plain one regenerates it, as the
+        // reference's intra front does after a restructure
+        int st = apply_restructure(c, nsub, nleaf, learned);
         if (st) return st;
-        topo_changed = true;
+        topo_changed = !c->front_valid;
     }
     return BVHCD_OK;
 }
@@ -371,6 +394,10 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     DevState& d = c->d;
     const int32_t n = (int32_t)c->T;
     hipStream_t s = c->stream;
+    if (c->repair_pending) {
+        if (front_op == BVHCD_FRONT_PURE_BVH_CD || !c->front_valid) c->repair_pending = c->front_valid = false;
+        else front_op = BVHCD_FRONT_RESTRUCTURE;
+    }
     if (front_op != BVHCD_FRONT_PURE_BVH_CD && !c->front_valid) front_op = BVHCD_FRONT_GENERATE;
     c->restr_pending = false;  // the quality flag lives for one frame
     const bool quality = c->cfg.enable_restructure != 0;
@@ -404,6 +431,12 @@ int run_front(bvhcd_ctx* c, int& front_op) {
             begin(c, S_FRONT);
             launch_keep(d, n, s);
             end(c, S_FRONT);
+            break;
+        case BVHCD_FRONT_RESTRUCTURE:
+            begin(c, S_FRONT);
+            launch_front_restr_apply(d, n, c->repair_tasks, s);
+            end(c, S_FRONT);
+            c->repair_pending = false;
             break;
         default:
             return fail(c, BVHCD_E_INVALID, "bad front_op");
@@ -536,7 +569,7 @@ int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V,
     if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
     c->T = T;
     c->V = V;
-    c->built = c->front_valid = c->restr_pending = c->learned_frame = false;
+    c->built = c->front_valid = c->restr_pending = c->learned_frame = c->repair_pending = false;
     c->frame = 0;
     c->d.fcur[0] = c->d.fcur[1] = 0;
     HIPCHK(c, hipMemsetAsync(c->d.fsize, 0, 4 * sizeof(uint32_t), c->stream));
@@ -829,10 +862,9 @@ int bvhcd_restructure(bvhcd_ctx* c, const int32_t* roots, int64_t k) {
     if (st) return st;
     const int64_t tcap = c->tcap;  // outside a frame: keep it out of the stage timing
     c->tcap = 0;
-    st = apply_restructure(c, nsub, nleaf);
+    st = apply_restructure(c, nsub, nleaf, true);  // the next step repairs the front
     c->tcap = tcap;
     if (st) return st;
-    c->front_valid = false;
     c->restr_pending = false;
     return check_overflow(c);
 }

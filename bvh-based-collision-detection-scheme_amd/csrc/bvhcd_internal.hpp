@@ -181,6 +181,9 @@ struct DevState {
     uint32_t* r_keep_mark = nullptr;
     uint64_t* r_keys = nullptr;     // T
     uint64_t* r_keys2 = nullptr;    // T
+    // incremental front restructure: (query, rebuilt subtree root) walks
+    int2* rtasks = nullptr;
+    int64_t rtask_cap = 0;
     // pairs
     int2* bp = nullptr;             // broad-phase pairs
     int2* np = nullptr;             // narrow-phase hits
@@ -193,12 +196,12 @@ enum Counter : int {
     C_BP = 0,          // broad-phase pair count
     C_NP = 1,          // narrow-phase hit count
     C_OVF = 2,         // overflow bits: 1 int front, 2 ext front, 4 bp pairs, 8 np pairs
-    C_VISIT_LO = 3,    // node tests (low 32 bits, wraps; C_VISIT_HI counts wraps)
-    C_VISIT_HI = 4,
     C_RT_SUB = 5,      // restructure: subtree count
     C_RT_INT = 6,      // restructure: internal node count
     C_RT_LEAF = 7,     // restructure: leaf count
-    C_WORK = 8,        // dynamic work-fetch cursor of the running front kernel
+    C_TASKS = 8,       // incremental front restructure: (query, subtree) walks
+    C_KEPT_I = 9,      // incremental front restructure: kept internal / external entries
+    C_KEPT_E = 10,
     C_NUM = 16         // words 16..31 hold 64-bit counters (see Counter64)
 };
 enum Counter64 : int {

@@ -483,6 +483,118 @@ void launch_keep(DevState& d, int32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_keep, dim3(stream_grid()), dim3(kBlock), 0, s, d, cur_front(d));
 }
 
+// ---------------------------------------------------------------- incremental front restructure
+// The intent of BvttFrontLooseIntra::restructure (collision/bvtt_front/BvttFrontLooseIntra.cu:330-400,
+// separateFronts :172-211, BvttFrontRestructureKernels.cu:12-90; unreachable in the shipped
+// intra path): after a subtree rebuild keep the front entries it does not touch and re-walk
+// only what it invalidates, instead of regenerating the whole front.
+// Each query's front is a cut of the leaves to its right. For a rebuilt subtree R, leaves [a, b]:
+//  * an entry strictly inside R of a query outside [a, b] is one piece of that query's cut of R;
+//    the piece whose leftmost leaf is a becomes the walk (query, R), the others are dropped;
+//  * an entry strictly inside R of a query inside [a, b] is dropped; that query's part inside
+//    R is re-walked from lca[q + 1] to R's escape lca[b + 1];
+//  * all other entries (outside every rebuilt subtree, at an ancestor, or at R) stay.
+// Classification runs on the old topology (rt_of_leaf and the old rangex), before emission.
+__global__ void __launch_bounds__(kBlock) k_restr_classify(DevState d, FrontIn in, FrontSink fi, FrontSink fe,
+                                                           PairSink tasks) {
+    __shared__ Stages st;
+    stages_init(st);
+    const Out out = make_out(st, fi, fe, d);
+    const uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]), total = ni + ne;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+        bool task = false;
+        int32_t q, r;
+        if (k < ni) {
+            q = in.query[0][k];
+            const int32_t x = (int32_t)in.node[0][k];
+            const int32_t lx = d.int_rx[x];
+            r = d.rt_of_leaf[lx];
+            const bool inside = r >= 0 && x > r;  // x < r: an ancestor of r
+            stage_push(fi, out.ovf, out.sb[0], out.sf[0], !inside, make_int2(q, x));
+            task = inside && d.rt_of_leaf[q] != r && lx == d.int_rx[r];
+        } else {
+            q = in.query[1][k - ni];
+            const int32_t j = (int32_t)in.node[1][k - ni];
+            r = d.rt_of_leaf[j];
+            const bool inside = r >= 0;
+            stage_push(fe, out.ovf, out.sb[1], out.sf[1], !inside, make_int2(q, j));
+            task = inside && d.rt_of_leaf[q] != r && j == d.int_rx[r];
+        }
+        stage_push(tasks, out.ovf, out.sb[2], out.sf[2], task, make_int2(q, r));
+    }
+    stage_final(fi, out.ovf, out.sb[0], out.sf[0]);
+    stage_final(fe, out.ovf, out.sb[1], out.sf[1]);
+    stage_final(tasks, out.ovf, out.sb[2], out.sf[2]);
+}
+
+void launch_front_restr_prepare(DevState& d, int32_t n, hipStream_t s) {
+    const FrontIn in = cur_front(d);
+    FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
+    hipMemsetAsync(fi.count, 0, sizeof(uint32_t), s);
+    hipMemsetAsync(fe.count, 0, sizeof(uint32_t), s);
+    hipMemsetAsync(d.ctr + C_TASKS, 0, sizeof(uint32_t), s);
+    // walks beyond the capacity are counted, not written (ovf bit 0): the caller compares
+    // the count with the capacity and regenerates the front instead
+    const PairSink tasks{d.ctr + C_TASKS, d.rtasks, (uint32_t)std::min<int64_t>(d.rtask_cap, 0xFFFFFFFFll), 0u};
+    if (n >= 2) hipLaunchKernelGGL(k_restr_classify, dim3(stream_grid()), dim3(kBlock), 0, s, d, in, fi, fe, tasks);
+    d.fcur[0] ^= 1;
+    d.fcur[1] ^= 1;
+}
+
+// Walks of the repaired front: (query, rebuilt root) walks first, then every query inside a
+// rebuilt run walks its part of that run. Entries append after the kept ones.
+__global__ void __launch_bounds__(kBlock) k_restr_walk(DevState d, uint32_t ntasks, int32_t n, FrontSink fi,
+                                                       FrontSink fe) {
+    __shared__ Stages st;
+    stages_init(st);
+    const Out out = make_out(st, fi, fe, d);
+    const uint32_t total = ntasks + (uint32_t)n;
+    uint32_t visits = 0;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+        int32_t q, start, stop;
+        if (k < ntasks) {
+            const int2 t = d.rtasks[k];
+            q = t.x;
+            start = t.y << 1;
+            stop = d.leaf_lca[d.int_ry[t.y] + 1];
+        } else {
+            q = (int32_t)(k - ntasks);
+            const int32_t r = d.rt_of_leaf[q];
+            if (r < 0) continue;
+            const int32_t b = d.int_ry[r];
+            if (q >= b) continue;
+            start = d.leaf_lca[q + 1];
+            stop = d.leaf_lca[b + 1];
+        }
+        const float4 lo = d.leaf[q].lo, hi = d.leaf[q].hi;
+        visits += stackless<true>(d, out, q, rec_box(lo, hi), start, stop);
+    }
+    out.final_flush();
+    add_visits(d, visits);
+}
+
+void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream_t s) {
+    reset_step_counters(d, s);
+    if (n < 2) return;
+    // pairs of the kept entries (KEEP over the kept part only)
+    hipMemcpyAsync(d.ctr + C_KEPT_I, d.fsize + d.fcur[0], sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(d.ctr + C_KEPT_E, d.fsize + 2 + d.fcur[1], sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    FrontIn kept = cur_front(d);
+    kept.size[0] = d.ctr + C_KEPT_I;
+    kept.size[1] = d.ctr + C_KEPT_E;
+    hipLaunchKernelGGL(k_keep, dim3(stream_grid()), dim3(kBlock), 0, s, d, kept);
+    // the walks append to the current buffers
+    FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
+    for (int k = 0; k < 2; ++k) {
+        FrontSink& f = k ? fe : fi;
+        const int b = d.fcur[k];
+        f.count = d.fsize + 2 * k + b;
+        f.node = d.fnode[k][b];
+        f.query = d.fquery[k][b];
+    }
+    hipLaunchKernelGGL(k_restr_walk, dim3(stream_grid()), dim3(kBlock), 0, s, d, ntasks, n, fi, fe);
+}
+
 // ---------------------------------------------------------------- REORDER (balance)
 // Pairs as in KEEP from the unchanged cut; then the internal entries of each node bucket are
 // re-ordered with the non-overlapping ones first (sort key node * 2 + overlap).

@@ -32,6 +32,11 @@ void launch_generate(DevState& d, int32_t n, hipStream_t s);          // travers
 void launch_update(DevState& d, int32_t n, int quality, hipStream_t s);// prune/sprout, quality, ordering
 void launch_keep(DevState& d, int32_t n, hipStream_t s);
 void launch_reorder(DevState& d, int32_t n, hipStream_t s);
+// Incremental front restructure: classify the current fronts against the runs in rt_of_leaf
+// (old topology; kept entries to the other buffers, walks to d.rtasks with their count in
+// ctr[C_TASKS]), then, after the rebuild, pairs of the kept entries plus the walks.
+void launch_front_restr_prepare(DevState& d, int32_t n, hipStream_t s);
+void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream_t s);
 void launch_order_fronts(DevState& d, int32_t n, hipStream_t s);      // radix sort by node + bucket starts
 void launch_snapshot(DevState& d, int32_t n, hipStream_t s);
 void launch_narrow(DevState& d, hipStream_t s);
@@ -39,6 +44,6 @@ void launch_digest(DevState& d, int filtered, hipStream_t s);
 void launch_restr_decision(DevState& d, int32_t n, hipStream_t s);    // counts into ctr[C_RT_*]
 
 // restructure.hip
-int restructure_subtrees(DevState& d, int32_t n, hipStream_t s);      // uses rt marks; returns status
+int restructure_subtrees(DevState& d, int32_t n, hipStream_t s, bool prepare_front);      // uses rt marks; returns status
 
 }  // namespace bvhcd

@@ -43,12 +43,14 @@ __global__ void __launch_bounds__(kBlock) k_restr_unpack(int32_t n, const uint64
     codes[x] = (uint32_t)(keys[x] & 0x3FFFFFFFull);
 }
 
-int restructure_subtrees(DevState& d, int32_t n, hipStream_t s) {
+int restructure_subtrees(DevState& d, int32_t n, hipStream_t s, bool prepare_front) {
     if (n < 3) return BVHCD_OK;
     // per-leaf subtree root from the difference marks left by launch_restr_decision
     launch_exscan(d, reinterpret_cast<const uint32_t*>(d.r_leaf_lo), reinterpret_cast<uint32_t*>(d.r_leaf_hi), n,
                   nullptr, s);
     hipLaunchKernelGGL(k_rt_of_leaf, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.r_leaf_hi, d.r_leaf_lo, d.rt_of_leaf);
+    // the front is classified while the old topology is still in place
+    if (prepare_front) launch_front_restr_prepare(d, n, s);
     // fresh scene box and codes, stable sort inside each run
     launch_scene_box(d, n, s);
     hipLaunchKernelGGL(k_restr_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, d.r_keys, d.vals_tmp);

@@ -67,6 +67,9 @@ extern "C" {
 #define BVHCD_FRONT_UPDATE      2   /* prune + sprout, then quality check and ordering */
 #define BVHCD_FRONT_REORDER     3   /* "balance": keep + re-bucket int front */
 #define BVHCD_FRONT_KEEP        4
+#define BVHCD_FRONT_RESTRUCTURE 5   /* incremental front restructure (BvttFrontLooseIntra::restructure,
+                                       BvttFrontLooseIntra.cu:330-400): run by the library on the step
+                                       after a repairing subtree rebuild; not accepted by bvhcd_step */
 
 /* ---- scheduler (bvhcd_step_scheduled) ---- */
 #define BVHCD_SCHED_PLAIN   0   /* reference STATIC_MANDATORY schedule + restructure heuristic */

@@ -378,6 +378,11 @@ struct Ctx {
     int last_bvh_op = -1, last_front_op = -1;
     int64_t visits = 0, restr_subtrees = 0, restr_leaves = 0;
     int narrow = 1;
+    // incremental front restructure: subtree root of each leaf in a rebuilt run (-1 none),
+    // and the (query, subtree root) walks to redo once the rebuilt subtrees are in place
+    bool repair_pending = false;
+    std::vector<int32_t> run_root;
+    std::vector<Entry> tasks;
 };
 
 inline void emit_pair(Sink& s, const Bvh& b, int64_t i, int64_t j) {
@@ -748,6 +753,81 @@ void restructure_subtree(Bvh& bv, int32_t R, const Box& scene, const int32_t* fa
     }
 }
 
+// ---------------------------------------------------------------- incremental front restructure
+// The intent of the reference's front restructure (BvttFrontLooseIntra::restructure,
+// collision/bvtt_front/BvttFrontLooseIntra.cu:330-400, with separateFronts :172-211 and
+// BvttFrontRestructureKernels.cu:12-90; unreachable in the shipped intra path): keep the front
+// entries the rebuilt subtrees do not touch and re-traverse only what they invalidate.
+// Every query's front is a cut of the leaves to its right, so for a rebuilt subtree R with
+// leaves [a, b]:
+//  * an entry strictly inside R of a query q outside [a, b] is one piece of q's cut of R; the
+//    piece covering leaf a stands for the whole cut and becomes the walk (q, R);
+//  * an entry strictly inside R of a query q inside [a, b] is dropped; q's part inside R is
+//    re-walked from lca[q + 1] up to R's escape lca[b + 1];
+//  * every other entry (outside all rebuilt subtrees, or R itself) stays valid.
+// Prepare runs on the old topology, apply after the rebuild and refit.
+void front_restr_prepare(Ctx& c, const std::vector<int32_t>& roots) {
+    const Bvh& b = c.bvh;
+    c.run_root.assign(b.n, -1);
+    for (int32_t R : roots)
+        for (int32_t j = b.rx[R]; j <= b.ry[R]; ++j) c.run_root[j] = R;
+    std::vector<Entry> fi, fe;
+    c.tasks.clear();
+    for (const Entry& e : c.fi) {
+        const int32_t r = c.run_root[b.rx[e.node]];
+        if (r < 0 || e.node == r) { fi.push_back(e); continue; }  // outside, ancestor or the root itself
+        if (e.node < r) { fi.push_back(e); continue; }            // ancestor of r (its range holds r's)
+        if (c.run_root[e.q] != r && b.rx[e.node] == b.rx[r]) c.tasks.push_back({e.q, r});
+    }
+    for (const Entry& e : c.fe) {
+        const int32_t r = c.run_root[e.node];
+        if (r < 0) { fe.push_back(e); continue; }
+        if (c.run_root[e.q] != r && e.node == b.rx[r]) c.tasks.push_back({e.q, r});
+    }
+    c.fi.swap(fi);
+    c.fe.swap(fe);
+    c.repair_pending = true;
+}
+
+void front_restr_apply(Ctx& c, int threads) {
+    const Bvh& b = c.bvh;
+    const int64_t n = b.n;
+    // pairs of the entries that stayed (as KEEP), then the walks append entries and pairs
+    keep(c, threads);
+    std::vector<Pair> kept_pairs;
+    kept_pairs.swap(c.bp);
+    const int64_t kept_visits = c.visits;
+    Counters ic(n), ec(n + 1);
+    for (auto& x : ic) x = 0;
+    for (auto& x : ec) x = 0;
+    const int64_t nt = (int64_t)c.tasks.size();
+    std::vector<Sink> sinks;
+    run_sinks(nt + n, threads, sinks, [&](int64_t k, Sink& s) {
+        if (k < nt) {
+            const Entry t = c.tasks[k];
+            stackless<true>(b, t.q, t.node << 1, b.llca[b.ry[t.node] + 1], s, nullptr, &ic, &ec);
+        } else {
+            const int64_t q = k - nt;
+            const int32_t r = c.run_root[q];
+            if (r >= 0 && q < b.ry[r]) stackless<true>(b, q, b.llca[q + 1], b.llca[b.ry[r] + 1], s, nullptr, &ic, &ec);
+        }
+    });
+    std::vector<Entry> nfi, nfe;
+    int64_t v = 0;
+    gather_sinks(sinks, &nfi, &nfe, c.bp, v);
+    c.bp.insert(c.bp.end(), kept_pairs.begin(), kept_pairs.end());
+    c.visits = kept_visits + v;
+    c.fi.insert(c.fi.end(), nfi.begin(), nfi.end());
+    c.fe.insert(c.fe.end(), nfe.begin(), nfe.end());
+    // per-node counts of the new front (the offsets keep the last ordering's values)
+    std::fill(c.log.intcnt.begin(), c.log.intcnt.end(), 0u);
+    std::fill(c.log.extcnt.begin(), c.log.extcnt.end(), 0u);
+    for (const Entry& e : c.fi) c.log.intcnt[e.node]++;
+    for (const Entry& e : c.fe) c.log.extcnt[e.node]++;
+    c.repair_pending = false;
+    c.tasks.clear();
+}
+
 int do_restructure(Ctx& c, int threads, int& tag) {
     std::vector<int32_t> roots;
     int64_t num_int = 0;
@@ -816,6 +896,11 @@ int step(Ctx& c, int bvh_op, int front_op, int threads) {
     }
     c.last_bvh_op = bvh_op == 2 ? (tag == 2 ? 0 : tag == 0 ? 1 : 2) : bvh_op;
     if (topo_changed) c.front_valid = false;
+    if (c.repair_pending && (!c.front_valid || front_op == 0)) {
+        c.repair_pending = false;  // a pure traversal frame or a new topology drops the repair
+        c.front_valid = false;
+    }
+    if (c.repair_pending) front_op = 5;
     // The restructure flag lives for one frame (BvttFrontLooseIntra.cu:70 clears it at every
     // front maintain; checkQuality sets it again).
     c.restr_pending = false;
@@ -826,6 +911,7 @@ int step(Ctx& c, int bvh_op, int front_op, int threads) {
         case 2: update(c, threads); break;
         case 3: reorder_balance(c, threads); break;
         case 4: keep(c, threads); break;
+        case 5: front_restr_apply(c, threads); break;
         default: return -1;
     }
     c.last_front_op = front_op;
@@ -1045,8 +1131,9 @@ void orc_export_restr_log(void* p, int32_t* intrange, int32_t* rtroot) {
 }
 
 // Explicit-root restructure (same rebuild as the heuristic path, roots given by the caller):
-// validated disjoint preorder roots, one shared scene box, then a full refit; the front must be
-// regenerated afterwards. Returns 0, or -1 for a bad root / overlapping ranges.
+// validated disjoint preorder roots, one shared scene box, then a full refit. A valid front is
+// prepared for the incremental front restructure, which the next step applies (front op 5).
+// Returns 0, or -1 for a bad root / overlapping ranges.
 int orc_restructure_roots(void* p, const int32_t* roots, int64_t k, int threads) {
     Ctx* c = static_cast<Ctx*>(p);
     if (!c->built || c->T < 2) return -5;
@@ -1059,10 +1146,11 @@ int orc_restructure_roots(void* p, const int32_t* roots, int64_t k, int threads)
     for (size_t i = 1; i < rg.size(); ++i)
         if (rg[i].first <= rg[i - 1].second) return -1;
     if (k == 0) return 0;
+    if (c->repair_pending) c->front_valid = c->repair_pending = false;  // one repair at a time
+    if (c->front_valid) front_restr_prepare(*c, std::vector<int32_t>(roots, roots + k));
     Box scene = scene_box(c->T, c->faces.data(), c->verts.data());
     for (int64_t i = 0; i < k; ++i) restructure_subtree(c->bvh, roots[i], scene, c->faces.data(), c->verts.data());
     refit_bvh(c->bvh, c->faces.data(), c->verts.data(), threads);
-    c->front_valid = false;
     c->restr_pending = false;
     return 0;
 }

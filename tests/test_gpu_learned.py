@@ -26,6 +26,11 @@ def _ctx(pkg, sc, **kw):
     return ctx
 
 
+def _rows(a):
+    a = np.asarray(a).reshape(-1, 2)
+    return a[np.lexsort((a[:, 1], a[:, 0]))]
+
+
 def _feature_rows(n, seed):
     rng = np.random.default_rng(seed)
     x = rng.normal(size=(n, 16)).astype(np.float32)
@@ -137,8 +142,10 @@ def test_features_match_host_recount(pkg):
     host = _host_features(bvh, fi, fe, roots)
     cols = [1, 3, 4] + list(range(6, 16))
     np.testing.assert_allclose(feats[:, cols], host[:, cols], rtol=1e-5, atol=1e-6)
-    # right after a build: surface areas, leaf boxes and age equal their references
-    assert np.all(feats[:, [0, 2, 5]] == 0)
+    # right after a build: surface areas and leaf boxes equal their references; age counts the
+    # frames stepped since the rebuild (1) / 12
+    assert np.all(feats[:, [0, 2]] == 0)
+    np.testing.assert_allclose(feats[:, 5], 1 / 12, rtol=1e-6)
     # five frames of refit + keep: age = frames stepped since the build (6 steps in all) / 12
     for fr in range(1, 6):
         ctx.update_vertices(sc.frame(fr))
@@ -183,15 +190,58 @@ def test_explicit_restructure_matches_oracle(pkg, oracle):
         ctx.restructure(np.array([0, roots[0]], np.int32))
     with pytest.raises(pkg.BvhcdError):
         ctx.restructure(np.array([len(rx)], np.int32))
-    # the next frame regenerates the front: pairs and fronts equal
-    v = sc.frame(37)
-    ctx.update_vertices(v)
-    orc.update_vertices(v)
-    ctx.step(1, 4)
-    orc.step(1, 4)
-    assert ctx.stats()["last_front_op"] == pkg.FRONT_GENERATE
-    np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), oracle.pair_set(orc.pairs(True)))
-    np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), oracle.pair_set(orc.pairs(False)))
+    # the next frames: incremental front restructure, then the ordinary front ops; pairs and
+    # fronts (as multisets) equal the oracle's
+    for fr, fo in [(37, 4), (38, 4), (39, 2), (40, 3), (41, 4)]:
+        v = sc.frame(min(fr, sc.frames - 1))
+        ctx.update_vertices(v)
+        orc.update_vertices(v)
+        ctx.step(1, fo)
+        orc.step(1, fo)
+        if fr == 37:
+            assert ctx.stats()["last_front_op"] == pkg.FRONT_RESTRUCTURE
+            assert orc.stats()["last_front_op"] == pkg.FRONT_RESTRUCTURE
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), oracle.pair_set(orc.pairs(True)),
+                                      err_msg=str(fr))
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), oracle.pair_set(orc.pairs(False)),
+                                      err_msg=str(fr))
+        for w in (0, 1):
+            np.testing.assert_array_equal(_rows(ctx.export_front(w)), _rows(orc.front(w)), err_msg=f"{fr} {w}")
+    ctx.close()
+
+
+@pytest.mark.parametrize("max_leaves,every", [(40, 2), (2000, 1)])
+def test_front_restructure_matches_oracle(pkg, oracle, max_leaves, every):
+    """Incremental front restructure after rebuilding many small subtrees / a few large ones."""
+    print(f"[gpu] front restructure {max_leaves} {every}", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=120, frames=60)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc)
+    orc = oracle.OracleContext(f, sc.num_verts)
+    for fr in range(8):
+        v = sc.frame(fr * 6)
+        ctx.update_vertices(v)
+        orc.update_vertices(v)
+        op = (0, 1) if fr == 0 else (1, 2 if fr % 4 == 0 else 4)
+        ctx.step(*op)
+        orc.step(*op)
+    b = ctx.export_bvh()
+    rx, ry, par = b["int_rx"], b["int_ry"], b["int_par"]
+    cnt = ry - rx + 1
+    cand = [i for i in range(len(rx)) if cnt[i] <= max_leaves and (par[i] < 0 or cnt[par[i]] > max_leaves)]
+    roots = np.array(cand[::every], np.int32)
+    ctx.restructure(roots)
+    orc.restructure(roots)
+    for fr in (50, 51):
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        orc.update_vertices(v)
+        ctx.step(1, 4)
+        orc.step(1, 4)
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), oracle.pair_set(orc.pairs(False)))
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), oracle.pair_set(orc.pairs(True)))
+        for w in (0, 1):
+            np.testing.assert_array_equal(_rows(ctx.export_front(w)), _rows(orc.front(w)), err_msg=f"{fr} {w}")
     ctx.close()
 
 

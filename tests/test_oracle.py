@@ -352,6 +352,57 @@ def test_oracle_explicit_ops_and_restructure(oracle):
         check_invariants(ctx.export_bvh(), sc.num_tris)
 
 
+def _disjoint_roots(b, max_leaves, every):
+    rx, ry, par = b["int_rx"], b["int_ry"], b["int_par"]
+    cnt = ry - rx + 1
+    cand = [i for i in range(len(rx)) if cnt[i] <= max_leaves and (par[i] < 0 or cnt[par[i]] > max_leaves)]
+    return np.array(cand[::every], np.int32)
+
+
+@pytest.mark.parametrize("max_leaves,every", [(64, 2), (500, 3), (100000, 1)])
+def test_oracle_front_restructure(oracle, max_leaves, every):
+    """Incremental front restructure after an explicit subtree rebuild: the repaired front is
+    again a complete cut for every query, its pairs are the brute-force pairs, and later front
+    ops (UPDATE / REORDER / KEEP) stay exact."""
+    pkg = load_package()
+    sc = pkg.Scene.preset("cloth256", n=48, frames=40)
+    f = sc.faces()
+    n = sc.num_tris
+    ctx = oracle.OracleContext(f, sc.num_verts)
+    ctx.update_vertices(sc.frame(0))
+    ctx.step(0, 1)
+    for fr in range(1, 8):
+        ctx.update_vertices(sc.frame(fr * 4))
+        ctx.step(1, 2 if fr % 3 == 0 else 4)
+    before = {w: ctx.front(w) for w in (0, 1)}
+    b0 = ctx.export_bvh()
+    roots = _disjoint_roots(b0, max_leaves, every)
+    assert roots.size >= 1
+    ctx.restructure(roots)
+    b1 = ctx.export_bvh()
+    check_invariants(b1, n)
+    # entries untouched by the rebuilt subtrees survive unchanged
+    inside = np.zeros(n, bool)
+    for r in roots:
+        inside[b0["int_rx"][r]:b0["int_ry"][r] + 1] = True
+    for fr, fo in [(33, 4), (34, 4), (35, 2), (36, 3), (37, 4), (38, 2)]:
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        ctx.step(1, fo)
+        st = ctx.stats()
+        if fr == 33:
+            assert st["last_front_op"] == 5
+            fi = ctx.front(0)
+            keep_i = before[0][~inside[b0["int_rx"][before[0][:, 1]]] & ~inside[before[0][:, 0]]]
+            got = {tuple(x) for x in fi.tolist()}
+            assert all(tuple(x) in got for x in keep_i.tolist())
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)),
+                                      oracle.pair_set(oracle.bruteforce_pairs(f, v, False)), err_msg=str(fr))
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)),
+                                      oracle.pair_set(oracle.bruteforce_pairs(f, v, True)), err_msg=str(fr))
+        front_partitions(ctx.front(0), ctx.front(1), ctx.export_bvh(), n)
+
+
 def test_plain_schedule_table(oracle):
     pkg = load_package()
     got = [oracle.plain_schedule(f) for f in range(26)]
