@@ -25,7 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 STAGES = ["build", "refit", "restructure", "front", "order+quality", "narrow", "schedule", "total"]
-FRONT_KERNEL = {0: "k_traverse<false>", 1: "k_traverse<true>", 2: "k_update", 3: "k_keep+k_reorder_int", 4: "k_keep"}
+FRONT_KERNEL = {0: "k_traverse<false>", 1: "k_traverse<true>", 2: "k_update", 3: "k_keep+k_reorder_keys", 4: "k_keep",
+                5: "k_keep+k_restr_walk"}
 
 
 def load_pkg():
@@ -54,11 +55,30 @@ def upload_frames(pkg, sc, first, count, torch):
     return dev
 
 
-def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights=None, digests_upto=0):
+def make_ctx(pkg, sc, faces, scheduler, weights):
     ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts, scheduler=scheduler)
     ctx.set_topology(faces, sc.num_verts)
     if weights is not None:
-        ctx.mlp_set_weights(weights)
+        ctx.mlp_set_weights(weights[0])
+        ctx.mlp_set_threshold(weights[1])
+    return ctx
+
+
+def frame_digests(pkg, sc, faces, frames_dev, count, scheduler, weights):
+    """Untimed replay: the narrow-phase pair-set digest of every frame."""
+    ctx = make_ctx(pkg, sc, faces, scheduler, weights)
+    nbytes = sc.num_verts * 12
+    out = []
+    for f in range(count):
+        ctx.update_vertices(frames_dev.data_ptr() + f * nbytes, pkg.MEM_DEVICE)
+        ctx.step_scheduled()
+        out.append(tuple(ctx.digest(True)))
+    ctx.close()
+    return out
+
+
+def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights=None, digests_upto=0):
+    ctx = make_ctx(pkg, sc, faces, scheduler, weights)
     nbytes = sc.num_verts * 12
     base = frames_dev.data_ptr()
     digests = []
@@ -81,7 +101,10 @@ def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights
     st = ctx.stats()
     last_digest = ctx.digest(True)
     ctx.close()
-    return dict(wall_s=t1 - t0, ms=ms, bytes=by, ops=ops, stats=st, digests=digests, last_digest=last_digest)
+    restructured = int((ops[:, 0] == pkg.BVH_RESTRUCTURE).sum())
+    updates = int((ops[:, 1] == pkg.FRONT_UPDATE).sum())
+    return dict(wall_s=t1 - t0, ms=ms, bytes=by, ops=ops, stats=st, digests=digests, last_digest=last_digest,
+                restructured=restructured, updates=updates)
 
 
 def roofline(res):
@@ -103,7 +126,7 @@ def roofline(res):
     achieved = b / (t * 1e-3) / 1e9
     stage_summary = {}
     for (s2, op2), (t2, b2, n2) in sorted(groups.items()):
-        label = STAGES[s2] + (f"/{['PURE', 'GENERATE', 'UPDATE', 'REORDER', 'KEEP'][op2]}" if op2 >= 0 else "")
+        label = STAGES[s2] + (f"/{['PURE', 'GENERATE', 'UPDATE', 'REORDER', 'KEEP', 'RESTRUCTURE'][op2]}" if op2 >= 0 else "")
         stage_summary[label] = {"frames": n2, "avg_ms": round(t2 / n2, 4),
                                 "GBps": round(b2 / (t2 * 1e-3) / 1e9, 1),
                                 "frac_peak": round(b2 / (t2 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
@@ -142,7 +165,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--weights", default=os.path.join(ROOT, "bvh-based-collision-detection-scheme_amd",
-                                                      "mlp_weights.npy"))
+                                                      "mlp_scheduler.npz"),
+                    help="trained scheduler (params + threshold) written by train_scheduler.py")
+    ap.add_argument("--no-verify", action="store_true", help="skip the per-frame pair-set comparison pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
@@ -166,12 +191,18 @@ def main():
 
     learned = None
     weights = None
+    identical = None
     if os.path.exists(args.weights):
-        weights = np.load(args.weights, allow_pickle=False)
+        z = np.load(args.weights, allow_pickle=False)
+        weights = (z["params"], float(z["threshold"]))
         learned = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED, torch, weights=weights)
-        log(f"learned: {steps / learned['wall_s']:.1f} frames/s")
-        if learned["last_digest"] != plain["last_digest"]:
-            log("WARNING: learned and plain pair sets differ on the last frame")
+        log(f"learned: {steps / learned['wall_s']:.1f} frames/s "
+            f"({learned['restructured']} restructure frames, {learned['updates']} UPDATE frames)")
+        if not args.no_verify:  # untimed replay: pair sets of every frame, learned vs plain
+            dp = frame_digests(pkg, sc, faces, frames_dev, warmup + steps, pkg.SCHED_PLAIN, None)
+            dl = frame_digests(pkg, sc, faces, frames_dev, warmup + steps, pkg.SCHED_LEARNED, weights)
+            identical = dp == dl
+            log(f"pair sets identical on all {len(dp)} frames: {identical}")
 
     rep = learned if learned is not None else plain
     value = steps / rep["wall_s"]
@@ -207,6 +238,8 @@ def main():
         "plain_fps": round(plain_fps, 3),
         "learned_fps": round(steps / learned["wall_s"], 3) if learned is not None else None,
         "learned_vs_plain": round((steps / learned["wall_s"]) / plain_fps, 4) if learned is not None else None,
+        "learned_pairs_identical_all_frames": identical,
+        "learned_restructure_frames": learned["restructured"] if learned is not None else None,
         "pairs_last_frame": rep["last_digest"][1],
         "roofline": roof,
         "stages": stages,

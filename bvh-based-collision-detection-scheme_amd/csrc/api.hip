@@ -23,6 +23,9 @@ namespace {
 enum Stage { S_BUILD = 0, S_REFIT = 1, S_RESTR = 2, S_FRONT = 3, S_ORDER = 4, S_NARROW = 5, S_SCHED = 6, S_TOTAL = 7 };
 constexpr int kEv = 16;   // begin/end event per stage
 constexpr int kLog = 8;   // device log words per frame
+// Learned schedule: a KEEP frame whose node tests exceed this multiple of the front size
+// (each entry is one test unless its internal node overlaps and is descended) asks for UPDATE.
+constexpr double kLearnedUpdateRatio = 1.25;
 }  // namespace
 
 struct bvhcd_ctx {
@@ -597,16 +600,28 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
     if (c->cfg.scheduler == BVHCD_SCHED_GENERATE) return do_step(c, BVHCD_BVH_BUILD, BVHCD_FRONT_PURE_BVH_CD);
     plain_schedule(c->frame, c->cfg.bvh_cycle, c->cfg.front_cycle, bvh_op, front_op);
     if (c->cfg.scheduler == BVHCD_SCHED_LEARNED && c->built && learned_ready(c->learned)) {
-        // Learned schedule: the mandatory rebuild and the heuristic's restructure frames become
-        // MLP decisions per subtree (restructure the chosen ones, refit the rest); the front
-        // keeps the plain cycle (a rebuild frame's forced GENERATE becomes that frame's UPDATE).
-        if (bvh_op == BVHCD_BVH_BUILD) {
-            front_op = BVHCD_FRONT_UPDATE;
-            c->learned_frame = true;
-        } else if (c->cfg.enable_restructure && c->restr_pending) {
-            c->learned_frame = true;
-        }
+        // Learned schedule: the periodic global maintenance of the plain schedule (rebuild +
+        // regenerate every bvh_cycle frames, UPDATE / REORDER every front_cycle) is replaced by
+        // per-subtree decisions. Every frame refits and keeps the front; every bvh_cycle frames
+        // (and on the heuristic's restructure flag) the MLP scores the candidate subtrees, and
+        // the chosen ones are rebuilt with an incremental front restructure. At front-cycle
+        // points a front whose KEEP work has outgrown its size (descents below internal entries
+        // that now overlap) gets an UPDATE. The pair set is the same for every schedule.
         bvh_op = BVHCD_BVH_REFIT;
+        front_op = BVHCD_FRONT_KEEP;
+        const int bc = c->cfg.bvh_cycle > 0 ? c->cfg.bvh_cycle : 12;
+        const int fc = c->cfg.front_cycle > 0 ? c->cfg.front_cycle : 6;
+        if (c->frame % bc == 0 || (c->cfg.enable_restructure && c->restr_pending)) c->learned_frame = true;
+        if (c->frame % fc == 0 && c->last_front == BVHCD_FRONT_KEEP) {
+            uint64_t visits = 0;
+            uint32_t sz[4] = {0, 0, 0, 0};
+            HIPCHK(c, hipMemcpyAsync(&visits, ctr64(c->d.ctr) + C64_VISITS, sizeof(visits), hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipMemcpyAsync(sz, c->d.fsize, sizeof(sz), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            const double entries = (double)sz[c->d.fcur[0]] + (double)sz[2 + c->d.fcur[1]];
+            if ((double)visits > kLearnedUpdateRatio * entries + 1024.0) front_op = BVHCD_FRONT_UPDATE;
+        }
         return do_step(c, bvh_op, front_op);
     }
     if (bvh_op == BVHCD_BVH_REFIT && c->cfg.enable_restructure && c->restr_pending) bvh_op = BVHCD_BVH_RESTRUCTURE;

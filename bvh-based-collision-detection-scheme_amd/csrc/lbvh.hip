@@ -182,33 +182,59 @@ constexpr int kSortItems = 16;
 constexpr int kSortTile = kBlock * kSortItems;  // 4096 keys per tile
 constexpr int kRadix = 256;
 
+// Digit counts of one 4096-key tile. Each wave takes 64 keys at a time and groups equal digits
+// with a wave64 multisplit (8 ballots), so one LDS atomic per distinct digit per wave replaces
+// one per key: partially sorted input (later passes, front node ids) has long runs of equal
+// digits that would otherwise serialise on the same LDS word.
 template <class K>
 __global__ void __launch_bounds__(kBlock) k_sort_hist(const K* __restrict__ keys, int64_t n, const uint32_t* dn,
                                                       int shift, uint32_t* __restrict__ hist, int64_t tiles) {
     __shared__ uint32_t h[kRadix];
     if (dn) n = min(n, (int64_t)*dn);
-    h[threadIdx.x] = 0;
-    __syncthreads();
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
-    for (int i = 0; i < kSortItems; ++i) {
-        int64_t j = base + (int64_t)i * kBlock + threadIdx.x;
-        if (j < n) atomicAdd(&h[(uint32_t)(keys[j] >> shift) & 0xFFu], 1u);
+    if (base >= n) {  // block-uniform: an empty tile contributes zero counts
+        hist[(int64_t)tid * tiles + blockIdx.x] = 0;
+        return;
+    }
+    h[tid] = 0;
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    constexpr int kPerWave = kSortTile / (kBlock / 64);
+    for (int r = 0; r < kPerWave / 64; ++r) {
+        const int64_t j = base + w * kPerWave + r * 64 + lane;
+        const bool ok = j < n;
+        const uint32_t dig = ok ? (uint32_t)(keys[j] >> shift) & 0xFFu : 0u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot(ok && ((dig >> b) & 1u));
+            peers &= ((dig >> b) & 1u) ? m : ~m;
+        }
+        if (ok && (peers & below) == 0) atomicAdd(&h[dig], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+    hist[(int64_t)tid * tiles + blockIdx.x] = h[tid];
 }
 
-// Stable scatter of one 4096-key tile. Items are ranked in index order (round-major, then
-// wave, then lane) with a wave64 multisplit: 8 ballots give the peers with an equal digit.
+// Stable scatter of one 4096-key tile. Wave w owns the tile's items [1024 w, 1024 w + 1024),
+// taken 64 at a time in index order; a wave64 multisplit (8 ballots) gives each item its rank
+// among the wave's earlier items with the same digit, using only the wave's own LDS counters
+// (no block barrier inside the loop). One block-level pass then turns the per-wave digit counts
+// into offsets, so an item's tile position is
+//   tile_start[digit] + (same-digit items of earlier waves) + (its rank inside the wave),
+// i.e. the stable order by index. Keys stay in registers between the two steps.
 template <class K>
 __global__ void __launch_bounds__(kBlock) k_sort_scatter(const K* __restrict__ keys, const int32_t* __restrict__ vals,
                                                          int64_t n, const uint32_t* dn, int shift,
                                                          const uint32_t* __restrict__ hist, int64_t tiles,
                                                          K* __restrict__ okeys, int32_t* __restrict__ ovals) {
-    __shared__ uint32_t tile_start[kRadix];   // exclusive scan of the tile's digit counts
-    __shared__ uint32_t gofs[kRadix];         // global offset of the tile's first key per digit
-    __shared__ uint32_t run[kRadix];          // per-digit running count over rounds
-    __shared__ uint32_t whist[kBlock / 64][kRadix];
+    constexpr int kWavesS = kBlock / 64;
+    constexpr int kPerWave = kSortTile / kWavesS;   // 1024
+    constexpr int kRounds = kPerWave / 64;          // 16
+    __shared__ uint32_t wcnt[kWavesS][kRadix];      // per-wave digit counts, then offsets
+    __shared__ uint32_t tile_start[kRadix];
+    __shared__ uint32_t gofs[kRadix];
     __shared__ K skeys[kSortTile];
     __shared__ int32_t svals[kSortTile];
     __shared__ uint8_t sdig[kSortTile];
@@ -219,58 +245,56 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(const K* __restrict__ k
     if (base >= n) return;  // whole block uniform: before any barrier
     const int valid = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
 
-    // tile digit counts
-    run[tid] = 0;
-    __syncthreads();
-    for (int i = 0; i < kSortItems; ++i) {
-        int j = i * kBlock + tid;
-        if (j < valid) atomicAdd(&run[(uint32_t)(keys[base + j] >> shift) & 0xFFu], 1u);
-    }
-    __syncthreads();
-    {
-        uint32_t c = run[tid], ex;
-        block_exscan(c, ex);
-        tile_start[tid] = ex;
-        gofs[tid] = hist[(int64_t)tid * tiles + blockIdx.x];
-        run[tid] = 0;
-    }
-    __syncthreads();
-
-    for (int r = 0; r < kSortItems; ++r) {
-        const int j = r * kBlock + tid;
+    for (int k = lane; k < kRadix; k += 64) wcnt[w][k] = 0;
+    K key[kRounds];
+    uint32_t rk[kRounds];
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int j = w * kPerWave + r * 64 + lane;
         const bool ok = j < valid;
-        K key = ok ? keys[base + j] : (K)0;
-        const uint32_t dig = ok ? (uint32_t)(key >> shift) & 0xFFu : 0xFFFFFFFFu;
-        // peers with the same digit among valid lanes of this wave
+        key[r] = ok ? keys[base + j] : (K)0;
+        const uint32_t dig = (uint32_t)(key[r] >> shift) & 0xFFu;
         uint64_t peers = __ballot(ok);
+#pragma unroll
         for (int b = 0; b < 8; ++b) {
             const uint64_t m = __ballot(ok && ((dig >> b) & 1u));
             peers &= ((dig >> b) & 1u) ? m : ~m;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
-        // first lane of each peer group publishes the group size
-        for (int k = 0; k < kRadix; k += 64) whist[w][k + lane] = 0;
-        __syncthreads();
-        if (ok && rank == 0) whist[w][dig] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {   // per digit: prefix over waves, then advance the running count
-            uint32_t acc = run[tid];
-            for (int ww = 0; ww < kBlock / 64; ++ww) {
-                uint32_t c = whist[ww][tid];
-                whist[ww][tid] = acc;
-                acc += c;
-            }
-            run[tid] = acc;
+        const uint32_t before = ok ? wcnt[w][dig] : 0u;
+        rk[r] = before + (uint32_t)__popcll(peers & below);
+        // the lowest lane of each peer group advances the wave's count (its read above was
+        // issued before this write, and LDS operations of one wave complete in order)
+        if (ok && (peers & below) == 0) wcnt[w][dig] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();  // keep the next round's reads after this write
+    }
+    __syncthreads();
+    {   // digit tid: offsets of each wave's group, then the tile-wide exclusive scan
+        uint32_t acc = 0;
+#pragma unroll
+        for (int ww = 0; ww < kWavesS; ++ww) {
+            const uint32_t c = wcnt[ww][tid];
+            wcnt[ww][tid] = acc;
+            acc += c;
         }
-        __syncthreads();
-        if (ok) {
-            const uint32_t pos = tile_start[dig] + whist[w][dig] + rank;
-            skeys[pos] = key;
+        uint32_t ex;
+        block_exscan(acc, ex);   // contains a barrier; every thread reaches it
+        tile_start[tid] = ex;
+        gofs[tid] = hist[(int64_t)tid * tiles + blockIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int j = w * kPerWave + r * 64 + lane;
+        if (j < valid) {
+            const uint32_t dig = (uint32_t)(key[r] >> shift) & 0xFFu;
+            const uint32_t pos = tile_start[dig] + wcnt[w][dig] + rk[r];
+            skeys[pos] = key[r];
             svals[pos] = vals[base + j];
             sdig[pos] = (uint8_t)dig;
         }
-        __syncthreads();
     }
+    __syncthreads();
     // coalesced write-out in tile-sorted order
     for (int i = 0; i < kSortItems; ++i) {
         const int j = i * kBlock + tid;

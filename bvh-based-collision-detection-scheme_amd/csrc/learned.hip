@@ -11,11 +11,12 @@
 // Candidates ("treelets"): maximal subtrees with at most kTreeletLeaves leaves.
 // Features (16 per treelet):
 //   0  log(sum of internal-node surface areas now / at the last rebuild)   (SAH growth)
-//   1  front entries (internal + external) inside the subtree per leaf
+//   1  front entries (internal + external) per leaf of the subtree; an entry counts for the
+//      subtree holding its node's leftmost leaf
 //   2  mean over leaves of log(leaf box area now / at the last rebuild)    (box expansion)
 //   3  depth of the subtree root / 32
 //   4  log2(leaf count) / 12
-//   5  frames since the subtree was last rebuilt / 12
+//   5  min(1, frames since the subtree was last rebuilt / 12)
 //   6-15 fraction of leaves whose external-front count c falls in bin floor(log2(1 + c)) (last bin 9+)
 // Numerics: weights rounded to bf16 (nearest even), layer inputs rounded to bf16 before the two
 // hidden products, fp32 accumulation (v_mfma_f32_16x16x16_bf16 for inference), fp32 biases
@@ -64,6 +65,9 @@ struct LearnedState {
     uint32_t* flags = nullptr;
     uint32_t* pos = nullptr;
     uint32_t* total = nullptr;
+    int32_t* tl = nullptr;        // candidate index of every leaf
+    uint32_t* tcount = nullptr;   // front entries per candidate
+    uint32_t* lcnt = nullptr;     // external-front entries per leaf
     int64_t node_cap = 0;
     float* feats = nullptr;
     float threshold = 0.0f;
@@ -97,7 +101,7 @@ void learned_destroy(LearnedState* L) {
     if (!L) return;
     for (void* p : {(void*)L->params, (void*)L->wbf, (void*)L->grad, (void*)L->m, (void*)L->v, (void*)L->loss,
                     (void*)L->x, (void*)L->y, (void*)L->t, (void*)L->roots, (void*)L->flags, (void*)L->pos,
-                    (void*)L->total, (void*)L->feats})
+                    (void*)L->total, (void*)L->feats, (void*)L->tl, (void*)L->tcount, (void*)L->lcnt})
         if (p) (void)hipFree(p);
     delete L;
 }
@@ -402,9 +406,45 @@ __global__ void k_treelet_compact(int32_t I, const uint32_t* __restrict__ flags,
     if (i < I && flags[i]) roots[pos[i]] = i;
 }
 
+__global__ void k_treelet_of_leaf(const int32_t* __restrict__ roots, DevState d, int32_t* __restrict__ tl) {
+    const int32_t r = roots[blockIdx.x];
+    for (int32_t j = d.int_rx[r] + (int32_t)threadIdx.x; j <= d.int_ry[r]; j += blockDim.x) tl[j] = (int32_t)blockIdx.x;
+}
+
+// Adds one per lane with `pred` to counter[key], one atomic per group of equal keys in the wave
+// (front entries are ordered by node, so neighbouring lanes mostly share a key).
+__device__ inline void wave_count(uint32_t* counter, int32_t key, bool pred) {
+    uint64_t todo = __ballot(pred);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int32_t k = __shfl(key, leader);
+        const uint64_t grp = __ballot(pred && key == k);
+        if (lane_id() == leader) atomicAdd(&counter[k], (uint32_t)__popcll(grp));
+        todo &= ~grp;
+    }
+}
+
+// Current front entries per candidate subtree (an entry belongs to the subtree holding its
+// node's leftmost leaf) and external entries per leaf.
+__global__ void __launch_bounds__(kBlock) k_count_front(DevState d, const int32_t* __restrict__ tl,
+                                                        uint32_t* __restrict__ tcount, uint32_t* __restrict__ lcnt) {
+    const int c0 = d.fcur[0], c1 = d.fcur[1];
+    const uint32_t ni = (uint32_t)min((int64_t)d.fsize[c0], d.front_cap[0]);
+    const uint32_t ne = (uint32_t)min((int64_t)d.fsize[2 + c1], d.front_cap[1]);
+    const uint32_t total = ni + ne;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+        const bool ext = k >= ni;
+        const int32_t leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
+        const int32_t t = tl[leaf];  // -1: a leaf hanging directly below a node too big to be a candidate
+        wave_count(tcount, t, t >= 0);
+        wave_count(lcnt, leaf, ext);
+    }
+}
+
 // One workgroup per treelet.
 __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__ roots, DevState d, int32_t frame,
-                                                     float frame_scale, float* __restrict__ feats) {
+                                                     float frame_scale, const uint32_t* __restrict__ tcount,
+                                                     const uint32_t* __restrict__ lcnt, float* __restrict__ feats) {
     __shared__ float red[4][kBlock];
     __shared__ uint32_t hist[10];
     const int32_t r = roots[blockIdx.x];
@@ -420,7 +460,7 @@ __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__
         const float now = rec_area(d.leaf[j]), ref = d.sa_ref_leaf[j];
         leaf_exp += logf((now + 1e-12f) / (ref + 1e-12f));
         age += (float)(frame - d.built_frame[j]);
-        const uint32_t c = d.extbeg[j + 1] - d.extbeg[j];
+        const uint32_t c = lcnt[j];
         const int bin = c >= 511u ? 9 : 31 - __clz(c + 1u);  // floor(log2(1 + c)), c + 1 in [1, 511]
         atomicAdd(&hist[bin], 1u);
     }
@@ -438,15 +478,13 @@ __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__
     if (threadIdx.x == 0) {
         float* f = feats + (int64_t)blockIdx.x * kIn;
         f[0] = logf((red[0][0] + 1e-12f) / (red[1][0] + 1e-12f));
-        const uint32_t fi = d.intbeg[r + m - 1] - d.intbeg[r];
-        const uint32_t fe = d.extbeg[b + 1] - d.extbeg[a];
-        f[1] = (float)(fi + fe) / (float)m;
+        f[1] = (float)tcount[blockIdx.x] / (float)m;
         f[2] = red[2][0] / (float)m;
         int depth = 0;
         for (int32_t p = d.int_par[r]; p >= 0 && depth < 4096; p = d.int_par[p]) ++depth;
         f[3] = (float)depth / 32.f;
         f[4] = log2f((float)m) / 12.f;
-        f[5] = red[3][0] / (float)m * frame_scale;
+        f[5] = fminf(red[3][0] / (float)m * frame_scale, 1.0f);
         for (int k = 0; k < 10; ++k) f[6 + k] = (float)hist[k] / (float)m;
     }
 }
@@ -457,10 +495,15 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
     *rows = 0;
     if (I < 1) return BVHCD_OK;
     if (L->node_cap < n) {
-        for (void* p : {(void*)L->roots, (void*)L->flags, (void*)L->pos, (void*)L->total, (void*)L->feats})
+        for (void* p : {(void*)L->roots, (void*)L->flags, (void*)L->pos, (void*)L->total, (void*)L->feats,
+                        (void*)L->tl, (void*)L->tcount, (void*)L->lcnt})
             if (p) (void)hipFree(p);
         L->roots = nullptr; L->flags = nullptr; L->pos = nullptr; L->total = nullptr; L->feats = nullptr;
-        if (hipMalloc(&L->roots, sizeof(int32_t) * n) != hipSuccess ||
+        L->tl = nullptr; L->tcount = nullptr; L->lcnt = nullptr;
+        if (hipMalloc(&L->tl, sizeof(int32_t) * n) != hipSuccess ||
+            hipMalloc(&L->tcount, sizeof(uint32_t) * n) != hipSuccess ||
+            hipMalloc(&L->lcnt, sizeof(uint32_t) * n) != hipSuccess ||
+            hipMalloc(&L->roots, sizeof(int32_t) * n) != hipSuccess ||
             hipMalloc(&L->flags, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->pos, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->total, sizeof(uint32_t) * 2) != hipSuccess ||
@@ -475,9 +518,15 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
     if (hipMemcpyAsync(&cnt, L->total, sizeof(cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return BVHCD_E_DEVICE;
-    if (cnt > 0)
+    if (cnt > 0) {
+        fill_i32(L->tl, n, -1, s);
+        hipLaunchKernelGGL(k_treelet_of_leaf, dim3(cnt), dim3(kBlock), 0, s, L->roots, d, L->tl);
+        hipMemsetAsync(L->tcount, 0, sizeof(uint32_t) * cnt, s);
+        hipMemsetAsync(L->lcnt, 0, sizeof(uint32_t) * (size_t)n, s);
+        hipLaunchKernelGGL(k_count_front, dim3(2048), dim3(kBlock), 0, s, d, L->tl, L->tcount, L->lcnt);
         hipLaunchKernelGGL(k_features, dim3(cnt), dim3(kBlock), 0, s, L->roots, d, (int32_t)frame, 1.0f / 12.0f,
-                           L->feats);
+                           L->tcount, L->lcnt, L->feats);
+    }
     *rows = cnt;
     return hipGetLastError() == hipSuccess ? BVHCD_OK : BVHCD_E_DEVICE;
 }

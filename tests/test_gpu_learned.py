@@ -101,14 +101,17 @@ def test_mlp_training_step_and_convergence(pkg):
 def _host_features(bvh, fi, fe, roots, frame_age=0.0):
     """Features 1, 3, 4 and 6-15 recomputed from the exported BVH and fronts."""
     rx, ry, par = bvh["int_rx"], bvh["int_ry"], bvh["int_par"]
-    ext_per_leaf = np.bincount(fe[:, 1], minlength=len(bvh["leaf_par"]))
-    int_per_node = np.bincount(fi[:, 1], minlength=len(rx))
+    n = len(bvh["leaf_par"])
+    ext_per_leaf = np.bincount(fe[:, 1], minlength=n)
+    # an entry counts for the candidate holding its node's leftmost leaf
+    leftmost = np.concatenate([rx[fi[:, 1]], fe[:, 1]])
+    per_leaf = np.bincount(leftmost, minlength=n)
     rows = []
     for r in roots:
         a, b = rx[r], ry[r]
         m = b - a + 1
         f = np.zeros(16)
-        f[1] = (int_per_node[r:r + m - 1].sum() + ext_per_leaf[a:b + 1].sum()) / m
+        f[1] = per_leaf[a:b + 1].sum() / m
         d, q = 0, par[r]
         while q >= 0:
             d, q = d + 1, par[q]
