@@ -34,7 +34,7 @@ SCENE_TWO_SPHERES, SCENE_CLOTH, SCENE_LAYERS, SCENE_SHEET_SPHERE = 0, 1, 2, 3
 MLP_NPARAM = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
 BVH_OP_NAMES = {0: "BUILD", 1: "REFIT", 2: "RESTRUCTURE", 3: "NONE"}
-FRONT_OP_NAMES = {0: "PURE_BVH_CD", 1: "GENERATE", 2: "UPDATE", 3: "REORDER", 4: "KEEP"}
+FRONT_OP_NAMES = {0: "PURE_BVH_CD", 1: "GENERATE", 2: "UPDATE", 3: "REORDER", 4: "KEEP", 5: "RESTRUCTURE"}
 
 
 class BvhcdError(RuntimeError):
@@ -120,6 +120,7 @@ def lib():
         "bvhcd_batch_pair_counts": (C.c_int, [P, P]),
         "bvhcd_batch_last_error": (C.c_char_p, [P]),
         "bvhcd_autotune": (C.c_int, [P]),
+        "bvhcd_batch_autotune": (C.c_int, [P]),
         "bvhcd_autotune_report": (C.c_int, [P, I64]),
         "bvhcd_scene_create": (C.c_int, [C.c_int, C.POINTER(SceneParams), C.POINTER(P)]),
         "bvhcd_scene_destroy": (None, [P]),
@@ -170,6 +171,20 @@ PRESETS = {
     "layers4": SceneSpec(SCENE_LAYERS, seed=5, n=1024, frames=200, layers=4, amplitude=0.05,
                          compress=0.5, spacing=1e-3),
 }
+
+
+def batch_scene_specs(num=64, seed=3, frames=150, t_min=2e3, t_max=2e5):
+    """The batched workload: `num` folding sheets with triangle counts log-uniform in
+    [t_min, t_max] (an n x n sheet has 2 (n-1)^2 triangles; n = ceil(sqrt(T / 2)) + 1), each
+    with a rigid icosphere (subdivision 3, radius 0.08) on a seeded straight path through it."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(num):
+        t = float(np.exp(rng.uniform(np.log(t_min), np.log(t_max))))
+        n = int(np.ceil(np.sqrt(t / 2.0))) + 1
+        specs.append(SceneSpec(SCENE_SHEET_SPHERE, seed=seed * 1000 + k, n=n, frames=frames, subdiv=3,
+                               radius=0.08, amplitude=0.15, compress=0.5))
+    return specs
 
 
 class Scene:
@@ -390,9 +405,88 @@ class CollisionContext:
 
     def restructure(self, roots):
         """Restructure the subtrees under the given preorder internal-node ids now; the next
-        step regenerates the front."""
+        step repairs the front (incremental front restructure)."""
         r = np.ascontiguousarray(roots, dtype=np.int32).ravel()
         self._chk(lib().bvhcd_restructure(self._h, _ptr(r) if r.size else None, r.size), "restructure")
+
+    def autotune(self):
+        """Tune the grids of KEEP and the narrow phase on the current data (needs a step first)."""
+        self._chk(lib().bvhcd_autotune(self._h), "autotune")
+
+
+def autotune_report() -> str:
+    n = lib().bvhcd_autotune_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().bvhcd_autotune_report(buf, n + 1)
+    return buf.value.decode()
+
+
+# ----------------------------------------------------------------------------- batched scenes
+class Batch:
+    """Many independent scenes per launch: one tree whose subtrees are the scenes, a persistent
+    self-CD traversal, per-scene narrow-phase hits, and (use_graph) one hipGraph per frame."""
+
+    def __init__(self, tris_per_scene, verts_per_scene, *, pair_cap=0, device=0, stream=None):
+        t = np.ascontiguousarray(tris_per_scene, dtype=np.int64)
+        v = np.ascontiguousarray(verts_per_scene, dtype=np.int64)
+        self.S = int(t.size)
+        self.toff = np.concatenate([[0], np.cumsum(t)])
+        self.voff = np.concatenate([[0], np.cumsum(v)])
+        cfg = Config(1, 1, 0, 0, int(pair_cap), 1, SCHED_PLAIN, 0, 12, 6, int(device), stream)
+        h = C.c_void_p()
+        st = lib().bvhcd_batch_create(C.byref(cfg), self.S, _ptr(t), _ptr(v), C.byref(h))
+        if st != OK:
+            raise BvhcdError(st, "bvhcd_batch_create failed")
+        self._h = h
+
+    def _chk(self, st, what):
+        if st != OK:
+            msg = lib().bvhcd_batch_last_error(self._h)
+            raise BvhcdError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bvhcd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_topology(self, faces):
+        """faces: concatenated per-scene faces with scene-local vertex ids (host)."""
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+        self._chk(lib().bvhcd_batch_set_topology(self._h, _ptr(f), MEM_HOST), "batch_set_topology")
+
+    def update_vertices(self, xyz, where=MEM_HOST):
+        if where == MEM_HOST:
+            v = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+            self._chk(lib().bvhcd_batch_update_vertices(self._h, _ptr(v), MEM_HOST), "batch_update_vertices")
+        else:
+            self._chk(lib().bvhcd_batch_update_vertices(self._h, C.c_void_p(int(xyz)), MEM_DEVICE),
+                      "batch_update_vertices")
+
+    def step(self, bvh_op, use_graph=True):
+        self._chk(lib().bvhcd_batch_step(self._h, int(bvh_op), 1 if use_graph else 0), "batch_step")
+
+    def pair_counts(self) -> np.ndarray:
+        c = np.empty(self.S, np.int64)
+        self._chk(lib().bvhcd_batch_pair_counts(self._h, _ptr(c)), "batch_pair_counts")
+        return c
+
+    def pairs(self, scene) -> np.ndarray:
+        n = C.c_int64()
+        self._chk(lib().bvhcd_batch_get_pairs(self._h, int(scene), None, 0, C.byref(n)), "batch_get_pairs")
+        out = np.empty((n.value, 2), np.int32)
+        if n.value:
+            self._chk(lib().bvhcd_batch_get_pairs(self._h, int(scene), _ptr(out), n.value, C.byref(n)),
+                      "batch_get_pairs")
+        return out
+
+    def autotune(self):
+        self._chk(lib().bvhcd_batch_autotune(self._h), "batch_autotune")
 
 
 def plain_schedule(frame, bvh_cycle=12, front_cycle=6):

@@ -37,6 +37,7 @@ struct bvhcd_ctx {
     int64_t T = 0, V = 0;
     bool built = false, front_valid = false, restr_pending = false;
     bool learned_frame = false;  // set by the learned scheduler for its decision frames
+    bool bp_tris = false;        // d.bp already converted from leaf ids to triangle ids
     bool repair_pending = false; // front classified for the incremental front restructure
     uint32_t repair_tasks = 0;   // its (query, subtree) walks
     int64_t frame = 0;
@@ -106,6 +107,7 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.int_flag, T);
     ok &= dalloc(c, d.itopo, T);
     ok &= dalloc(c, d.tri_finite, T);
+    ok &= dalloc(c, d.lfaces, 3 * T);
     ok &= dalloc(c, d.sa_ref, T);
     ok &= dalloc(c, d.sa_ref_leaf, T);
     ok &= dalloc(c, d.built_frame, T);
@@ -456,6 +458,7 @@ int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     if (bvh_op < 0 || bvh_op > BVHCD_BVH_NONE || front_op < 0 || front_op > BVHCD_FRONT_KEEP)
         return fail(c, BVHCD_E_INVALID, "bad op");
     begin(c, S_TOTAL);
+    c->bp_tris = false;
     bool topo = false;
     int st = run_bvh(c, bvh_op, topo);
     if (st) return st;
@@ -481,6 +484,13 @@ void plain_schedule(int64_t f, int bc, int fc, int& bvh_op, int& front_op) {
     else front_op = BVHCD_FRONT_KEEP;
     if (bvh_op == BVHCD_BVH_BUILD) front_op = BVHCD_FRONT_GENERATE;
     if ((f + 1) % bc == 0) front_op = BVHCD_FRONT_KEEP;
+}
+
+// The broad-phase list holds leaf ids until it is read out as triangle pairs.
+void bp_as_tris(bvhcd_ctx* c) {
+    if (c->bp_tris || !c->built) return;
+    launch_bp_to_tris(c->d, c->stream);
+    c->bp_tris = true;
 }
 
 int check_overflow(bvhcd_ctx* c) {
@@ -630,6 +640,7 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
 
 int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int where, int filtered) {
     if (!c || !num) return BVHCD_E_INVALID;
+    if (!filtered) bp_as_tris(c);
     uint32_t cnt = 0;
     HIPCHK(c, hipMemcpyAsync(&cnt, c->d.ctr + (filtered ? C_NP : C_BP), sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
     int st = check_overflow(c);
@@ -648,6 +659,7 @@ int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int w
 
 int bvhcd_pair_digest(bvhcd_ctx* c, int filtered, uint64_t* digest, int64_t* num) {
     if (!c || !digest) return BVHCD_E_INVALID;
+    if (!filtered) bp_as_tris(c);
     launch_digest(c->d, filtered, c->stream);
     uint64_t dg = 0;
     uint32_t cnt = 0;
@@ -861,6 +873,64 @@ int bvhcd_mlp_grad(bvhcd_ctx* c, const float* feats, const float* targets, int64
     if (!c->learned) return fail(c, BVHCD_E_INVALID, "no weights");
     return learned_grad(c->learned, feats, targets, rows, grad, loss, where, c->stream, &c->err);
 }
+// ---- autotuner: grid sizes of KEEP and the narrow phase on the context's current data
+int bvhcd_autotune(bvhcd_ctx* c) {
+    if (!c) return BVHCD_E_INVALID;
+    if (!c->built || !c->front_valid || c->repair_pending)
+        return fail(c, BVHCD_E_NOT_BUILT, "autotune needs a built BVH and a front (run a step first)");
+    DevState& d = c->d;
+    const int32_t n = (int32_t)c->T;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    const unsigned cand[] = {512u, 1024u, 2048u, 4096u, 8192u, 16384u};
+    for (int k : {(int)TK_KEEP, (int)TK_NARROW}) {
+        float best_ms = 1e30f;
+        unsigned best = tuned_grid(k, k == TK_KEEP ? c->T : d.maxT);
+        for (unsigned g : cand) {
+            set_tuned_grid(k, k == TK_KEEP ? c->T : d.maxT, g);
+            auto run = [&]() {
+                if (k == TK_KEEP) launch_keep(d, n, c->stream);  // rewrites the same pair list
+                else launch_narrow(d, c->stream);
+            };
+            run();  // warm-up
+            (void)hipEventRecord(e0, c->stream);
+            for (int r = 0; r < 3; ++r) run();
+            (void)hipEventRecord(e1, c->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return fail(c, BVHCD_E_DEVICE, "autotune run failed");
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            record_tuning(k, k == TK_KEEP ? c->T : d.maxT, g, ms / 3.0f);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = g;
+            }
+        }
+        set_tuned_grid(k, k == TK_KEEP ? c->T : d.maxT, best);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // leave a consistent result: the current front's pairs and their narrow phase
+    launch_keep(d, n, c->stream);
+    c->bp_tris = false;
+    if (c->cfg.narrow_phase) launch_narrow(d, c->stream);
+    return check_overflow(c);
+}
+
+int bvhcd_autotune_report(char* buf, int64_t cap) {
+    const std::string r = tuning_report();
+    if (buf && cap > 0) {
+        const size_t k = std::min<size_t>((size_t)cap - 1, r.size());
+        std::memcpy(buf, r.data(), k);
+        buf[k] = 0;
+    }
+    return (int)r.size();
+}
+
 int bvhcd_mlp_set_threshold(bvhcd_ctx* c, float t) {
     if (!c || !(t == t)) return BVHCD_E_INVALID;
     if (!c->learned) c->learned = learned_create(c->stream);

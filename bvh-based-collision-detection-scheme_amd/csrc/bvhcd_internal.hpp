@@ -128,7 +128,8 @@ struct DevState {
     float* sa_ref = nullptr;        // T-1 internal-node surface area at its last rebuild
     float* sa_ref_leaf = nullptr;   // T leaf-box surface area at its last rebuild
     int32_t* built_frame = nullptr; // T frame at which each leaf's subtree was last rebuilt
-    uint8_t* tri_finite = nullptr;  // T (by triangle id): all 9 |coordinates| <= 1e8 (set by refit)
+    uint8_t* tri_finite = nullptr;  // T (by leaf): all 9 |coordinates| <= 1e8 (set by refit)
+    int32_t* lfaces = nullptr;      // 3T: the faces in leaf (sorted) order, rebuilt with prim_idx
     float* gpart = nullptr;         // 12 floats per internal node: child boxes handed over
                                     // between workgroups at chunk-spanning nodes (left, right)
     int32_t* lroot = nullptr;       // T maximal chunk-local subtree root of each leaf (-1 none)

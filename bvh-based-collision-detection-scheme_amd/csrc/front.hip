@@ -23,6 +23,10 @@
 //  * Appends go through a per-wave LDS staging buffer: lanes take slots with a ballot
 //    prefix, and one global atomic reserves space for a whole buffer when it is flushed.
 //    Capacity is checked before every global write (overflow bit instead of a write).
+#include <cstdio>
+#include <string>
+#include <vector>
+
 #include "bvhcd_internal.hpp"
 #include "kernels.hpp"
 
@@ -116,10 +120,10 @@ struct Out {
     uint32_t* ovf;
     volatile int2* sb[3];
     volatile uint32_t* sf[3];
-    const int32_t* prim_idx;
+    // Broad-phase pairs are kept as leaf (sorted-slot) ids, query first (qi < j): the narrow
+    // phase then reads leaf-ordered faces with locality, and maps to triangle ids itself.
     __device__ void pair(int32_t qi, int32_t j) const {
-        const int32_t a = prim_idx[qi], b = prim_idx[j];
-        stage_push(bp, ovf, sb[2], sf[2], true, a < b ? make_int2(a, b) : make_int2(b, a));
+        stage_push(bp, ovf, sb[2], sf[2], true, make_int2(qi, j));
     }
     __device__ void int_entry(int32_t qi, int32_t node) const {
         stage_push(fi, ovf, sb[0], sf[0], true, make_int2(qi, node));
@@ -145,7 +149,6 @@ __device__ inline Out make_out(Stages& st, const FrontSink& fi, const FrontSink&
         o.sb[k] = st.buf[k][w];
         o.sf[k] = &st.fill[k][w];
     }
-    o.prim_idx = d.prim_idx;
     return o;
 }
 
@@ -251,6 +254,49 @@ static void reset_step_counters(DevState& d, hipStream_t s) {
 
 static unsigned stream_grid() { return 2048; }
 
+// ---------------------------------------------------------------- launch-shape tuning
+// The reference picks one block size per kernel from the occupancy API
+// (system/CudaDevice/CudaDevice.cu:60-94). Here block sizes are fixed at one wave64 multiple
+// (256, the LDS staging is sized for it) and what is tuned is the grid of the grid-stride and
+// persistent kernels, per kernel and size class ceil(log2 n), by timing candidates on live data
+// (bvhcd_autotune / bvhcd_batch_autotune). The table is process-wide and not locked.
+namespace {
+constexpr int kClasses = 40;
+const unsigned kDefaultGrid[TK_COUNT] = {2048u, 4096u, 2048u};
+unsigned g_grid[TK_COUNT][kClasses] = {};
+struct TuneTrial {
+    int kernel, cls;
+    unsigned grid;
+    float ms;
+};
+std::vector<TuneTrial> g_trials;
+int size_class(int64_t n) {
+    int c = 0;
+    while (c < kClasses - 1 && ((int64_t)1 << c) < n) ++c;
+    return c;
+}
+}  // namespace
+
+unsigned tuned_grid(int kernel, int64_t n) {
+    const unsigned g = g_grid[kernel][size_class(n)];
+    return g ? g : kDefaultGrid[kernel];
+}
+void set_tuned_grid(int kernel, int64_t n, unsigned grid) { g_grid[kernel][size_class(n)] = grid; }
+void record_tuning(int kernel, int64_t n, unsigned grid, float ms) {
+    g_trials.push_back({kernel, size_class(n), grid, ms});
+}
+std::string tuning_report() {
+    static const char* names[TK_COUNT] = {"k_keep", "k_narrow", "k_batch_cd"};
+    std::string r = "kernel size_class grid ms_per_launch chosen\n";
+    char line[160];
+    for (const TuneTrial& t : g_trials) {
+        snprintf(line, sizeof(line), "%s 2^%d %u %.4f %s\n", names[t.kernel], t.cls, t.grid, t.ms,
+                 g_grid[t.kernel][t.cls] == t.grid ? "*" : "");
+        r += line;
+    }
+    return r;
+}
+
 // ---------------------------------------------------------------- pure traversal / generate
 template <bool FRONT>
 __global__ void __launch_bounds__(kBlock) k_traverse(int32_t n, DevState d, FrontSink fi, FrontSink fe) {
@@ -276,6 +322,47 @@ void launch_pure_cd(DevState& d, int32_t n, hipStream_t s) {
     if (n >= 2)
         hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, null_sink(),
                            null_sink());
+}
+
+// ---------------------------------------------------------------- batched scenes
+// Self-CD over many scenes in one tree: the sort keys carry the scene id above the 30 Morton
+// bits, so every scene is one subtree and its leaves are contiguous. Query x of the scene whose
+// last leaf is b walks the leaves (x, b] only: from lca[x + 1] to that subtree's escape
+// lca[b + 1]. Persistent: each wave takes 64 queries at a time from a global counter until
+// all are taken (uneven scenes and uneven per-query work balance themselves).
+__global__ void __launch_bounds__(kBlock) k_batch_cd(int32_t n, DevState d, const uint64_t* __restrict__ keys,
+                                                     const int32_t* __restrict__ scene_last, uint32_t* queue) {
+    __shared__ Stages st;
+    stages_init(st);
+    const Out out = make_out(st, null_sink(), null_sink(), d);
+    const int lane = lane_id();
+    uint32_t v = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(queue, 64u);
+        base = __shfl(base, 0);
+        if (base >= (uint32_t)n) break;  // wave-uniform exit: the queue is drained
+        const int32_t x = (int32_t)(base + lane);
+        if (x < n) {
+            const int32_t last = scene_last[(int32_t)(keys[x] >> 30)];
+            if (x < last) {
+                const float4 lo = d.leaf[x].lo, hi = d.leaf[x].hi;
+                v += stackless<false>(d, out, x, rec_box(lo, hi), f2i(hi.w), d.leaf_lca[last + 1]);
+            }
+        }
+    }
+    stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
+    add_visits(d, v);
+}
+
+// Counters are cleared with kernels (no memset nodes) so that the sequence can be captured in
+// a graph; `grid` workgroups stay resident and drain the query queue.
+void launch_batch_cd(DevState& d, int32_t n, const uint64_t* keys, const int32_t* scene_last, uint32_t* queue,
+                     unsigned grid, hipStream_t s) {
+    fill_u32(d.ctr, 3, 0u, s);
+    fill_u32(reinterpret_cast<uint32_t*>(ctr64(d.ctr) + C64_VISITS), 2, 0u, s);
+    fill_u32(queue, 1, 0u, s);
+    if (n >= 2) hipLaunchKernelGGL(k_batch_cd, dim3(grid), dim3(kBlock), 0, s, n, d, keys, scene_last, queue);
 }
 
 // ---------------------------------------------------------------- ordering
@@ -480,7 +567,7 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in) {
 
 void launch_keep(DevState& d, int32_t n, hipStream_t s) {
     reset_step_counters(d, s);
-    hipLaunchKernelGGL(k_keep, dim3(stream_grid()), dim3(kBlock), 0, s, d, cur_front(d));
+    hipLaunchKernelGGL(k_keep, dim3(tuned_grid(TK_KEEP, n)), dim3(kBlock), 0, s, d, cur_front(d));
 }
 
 // ---------------------------------------------------------------- incremental front restructure
@@ -713,9 +800,16 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
     const PairSink sink{&d.ctr[C_NP], d.np, (uint32_t)min((int64_t)d.pair_cap, (int64_t)0xFFFFFFFF), 8u};
     const uint32_t npairs = min(d.ctr[C_BP], (uint32_t)d.pair_cap);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += gridDim.x * blockDim.x) {
-        const int2 pr = d.bp[k];  // pr.x < pr.y
-        const int32_t* fa = d.faces + 3 * (int64_t)pr.x;
-        const int32_t* fb = d.faces + 3 * (int64_t)pr.y;
+        const int2 lv = d.bp[k];  // leaves
+        // canonical order: the triangle with the lower id is P
+        int32_t ta = d.prim_idx[lv.x], tb = d.prim_idx[lv.y], la = lv.x, lb = lv.y;
+        if (ta > tb) {
+            const int32_t t = ta; ta = tb; tb = t;
+            const int32_t l = la; la = lb; lb = l;
+        }
+        const int2 pr = make_int2(ta, tb);
+        const int32_t* fa = d.lfaces + 3 * (int64_t)la;
+        const int32_t* fb = d.lfaces + 3 * (int64_t)lb;
         const int32_t a0 = fa[0], a1 = fa[1], a2 = fa[2], b0 = fb[0], b1 = fb[1], b2 = fb[2];
         // A shared vertex has bitwise-equal translated coordinates on both sides, so while
         // every product stays finite (all |coordinates| <= 1e8) its projection lies in both
@@ -723,7 +817,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
         const bool share = a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 ||
                            a2 == b1 || a2 == b2;
         bool hit;
-        if (share && d.tri_finite[pr.x] && d.tri_finite[pr.y]) {
+        if (share && d.tri_finite[la] && d.tri_finite[lb]) {
             hit = true;
         } else {
             hit = sat_tri_tri(ld3(d.verts, a0), ld3(d.verts, a1), ld3(d.verts, a2), ld3(d.verts, b0),
@@ -734,9 +828,22 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
     stage_final(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w]);
 }
 
+// Broad-phase list from leaf ids to triangle ids (lower id first), in place, for export.
+__global__ void __launch_bounds__(kBlock) k_bp_to_tris(DevState d) {
+    const uint32_t m = min(d.ctr[C_BP], (uint32_t)d.pair_cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const int2 lv = d.bp[k];
+        const int32_t a = d.prim_idx[lv.x], b = d.prim_idx[lv.y];
+        d.bp[k] = a < b ? make_int2(a, b) : make_int2(b, a);
+    }
+}
+void launch_bp_to_tris(DevState& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_bp_to_tris, dim3(stream_grid()), dim3(kBlock), 0, s, d);
+}
+
 void launch_narrow(DevState& d, hipStream_t s) {
-    hipMemsetAsync(d.ctr + C_NP, 0, sizeof(uint32_t), s);
-    hipLaunchKernelGGL(k_narrow, dim3(stream_grid() * 2), dim3(kBlock), 0, s, d);
+    fill_u32(d.ctr + C_NP, 1, 0u, s);  // a kernel, not a memset: the batched frame graph records it
+    hipLaunchKernelGGL(k_narrow, dim3(tuned_grid(TK_NARROW, d.maxT)), dim3(kBlock), 0, s, d);
 }
 
 // ---------------------------------------------------------------- pair-set digest

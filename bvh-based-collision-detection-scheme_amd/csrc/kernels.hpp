@@ -1,5 +1,7 @@
 // Host-side launchers of the gfx950 kernels (stream-ordered, no host synchronisation).
 #pragma once
+#include <string>
+
 #include "bvhcd_internal.hpp"
 
 namespace bvhcd {
@@ -19,6 +21,22 @@ bool radix_sort_u32_dev(DevState& d, uint32_t* keys, int32_t* vals, uint32_t* tk
 void radix_sort_u64(DevState& d, uint64_t* keys, int32_t* vals, uint64_t* tkeys, int32_t* tvals, int64_t n, int nbits,
                     hipStream_t s);
 void launch_emit_full(DevState& d, int32_t n, hipStream_t s);
+// Launch-shape tuning: grid sizes (workgroups) of the grid-stride / persistent kernels per size
+// class ceil(log2 n), process-wide; 0 entries fall back to the defaults.
+enum TunedKernel { TK_KEEP = 0, TK_NARROW = 1, TK_BATCH_CD = 2, TK_COUNT = 3 };
+unsigned tuned_grid(int kernel, int64_t n);
+void set_tuned_grid(int kernel, int64_t n, unsigned grid);
+void record_tuning(int kernel, int64_t n, unsigned grid, float ms);
+std::string tuning_report();
+// Emission from sorted 64-bit keys (batched scenes: scene id above the Morton bits).
+void launch_emit_full64(DevState& d, const uint64_t* keys, int32_t n, hipStream_t s);
+// Stable LSD sort of 64-bit keys over bits [0, nbits); returns true when the result sits in
+// (tkeys, tvals).
+bool radix_sort_u64_dev(DevState& d, uint64_t* keys, int32_t* vals, uint64_t* tkeys, int32_t* tvals, int64_t n,
+                        int nbits, hipStream_t s);
+// Batched self-CD (persistent, work queue) over per-scene subtrees; pairs into d.bp.
+void launch_batch_cd(DevState& d, int32_t n, const uint64_t* keys, const int32_t* scene_last, uint32_t* queue,
+                     unsigned grid, hipStream_t s);
 void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t* leaf_lo, int32_t* leaf_hi,
                       int32_t* split_lo, int32_t* split_hi, int32_t* base, int32_t* keep_par, uint32_t* keep_mark,
                       hipStream_t s);
@@ -40,6 +58,8 @@ void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream
 void launch_order_fronts(DevState& d, int32_t n, hipStream_t s);      // radix sort by node + bucket starts
 void launch_snapshot(DevState& d, int32_t n, hipStream_t s);
 void launch_narrow(DevState& d, hipStream_t s);
+// Broad-phase pairs are kept as leaf ids; converts d.bp in place to (lower, higher) triangle ids.
+void launch_bp_to_tris(DevState& d, hipStream_t s);
 void launch_digest(DevState& d, int filtered, hipStream_t s);
 void launch_restr_decision(DevState& d, int32_t n, hipStream_t s);    // counts into ctr[C_RT_*]
 

@@ -352,16 +352,27 @@ void radix_sort_u32(DevState& d, uint32_t* keys, int32_t* vals, uint32_t* tkeys,
                     hipStream_t s) {
     radix_sort_impl<uint32_t>(d, keys, vals, tkeys, tvals, n, nbits, s);
 }
+bool radix_sort_u64_dev(DevState& d, uint64_t* keys, int32_t* vals, uint64_t* tkeys, int32_t* tvals, int64_t n,
+                        int nbits, hipStream_t s) {
+    return radix_sort_passes<uint64_t>(d, keys, vals, tkeys, tvals, n, nullptr, nbits, s);
+}
 void radix_sort_u64(DevState& d, uint64_t* keys, int32_t* vals, uint64_t* tkeys, int32_t* tvals, int64_t n, int nbits,
                     hipStream_t s) {
     radix_sort_impl<uint64_t>(d, keys, vals, tkeys, tvals, n, nbits, s);
 }
 
 // ======================================================================= emission
+// delta(i) = bit length of c[i] ^ c[i+1]; the last split position holds a sentinel above every
+// real value (33 for 32-bit codes, 65 for 64-bit keys).
 __device__ inline int delta_at(const uint32_t* __restrict__ c, int32_t i, int32_t n) {
     if (i >= n - 1) return 33;
     const uint32_t x = c[i] ^ c[i + 1];
     return x ? 32 - __clz(x) : 0;
+}
+__device__ inline int delta_at(const uint64_t* __restrict__ c, int32_t i, int32_t n) {
+    if (i >= n - 1) return 65;
+    const uint64_t x = c[i] ^ c[i + 1];
+    return x ? 64 - __clzll((long long)x) : 0;
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -388,32 +399,34 @@ void fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
 // run act as delta = +inf, so a run is built exactly as a standalone tree over its leaves.
 // Range [a, b]: a-1 = nearest split left of k with delta >= d (a = lo if none); b = nearest
 // split right of k with delta > d (b = hi if none).
-__device__ inline void split_range(const uint32_t* __restrict__ c, int32_t k, int32_t lo, int32_t hi, int d,
+template <class C>
+__device__ inline void split_range(const C* __restrict__ c, int32_t k, int32_t lo, int32_t hi, int d,
                                    int32_t& a, int32_t& b) {
+    constexpr int kBits = 8 * (int)sizeof(C);
     // a: smallest index in [lo, k] whose code agrees with c[k] on bits >= d-1 (d >= 1).
     if (d == 0) {
         a = k;
     } else {
-        const uint32_t ck = c[k];
+        const C ck = c[k];
         const int sh = d - 1;
         int32_t L = lo, R = k;  // answer in [L, R]
         while (L < R) {
             const int32_t m = L + ((R - L) >> 1);
-            if (((c[m] ^ ck) >> sh) == 0u) R = m;
+            if (((c[m] ^ ck) >> sh) == (C)0) R = m;
             else L = m + 1;
         }
         a = L;
     }
     // b: largest index in [k+1, hi] whose code agrees with c[k+1] on bits >= d.
     {
-        const uint32_t ck1 = c[k + 1];
+        const C ck1 = c[k + 1];
         int32_t L = k + 1, R = hi;
-        if (d >= 32) {
+        if (d >= kBits) {
             L = hi;
         } else {
             while (L < R) {
                 const int32_t m = L + ((R - L + 1) >> 1);
-                if (((c[m] ^ ck1) >> d) == 0u) L = m;
+                if (((c[m] ^ ck1) >> d) == (C)0) L = m;
                 else R = m - 1;
             }
         }
@@ -423,7 +436,8 @@ __device__ inline void split_range(const uint32_t* __restrict__ c, int32_t k, in
 
 // One thread per split k of the sorted codes; segments: runs [seg_lo, seg_hi] given per split
 // through seg_of (nullptr = one run [0, n-1]).
-__global__ void __launch_bounds__(kBlock) k_emit_internal(int32_t n, const uint32_t* __restrict__ c,
+template <class C>
+__global__ void __launch_bounds__(kBlock) k_emit_internal(int32_t n, const C* __restrict__ c,
                                                           const int32_t* __restrict__ run_lo,
                                                           const int32_t* __restrict__ run_hi,
                                                           int32_t* __restrict__ s_a, int32_t* __restrict__ s_b,
@@ -458,7 +472,8 @@ __global__ void __launch_bounds__(kBlock) k_emit_internal(int32_t n, const uint3
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_emit_leaves(int32_t n, const uint32_t* __restrict__ c,
+template <class C>
+__global__ void __launch_bounds__(kBlock) k_emit_leaves(int32_t n, const C* __restrict__ c,
                                                         const int32_t* __restrict__ leaf_lo,
                                                         const int32_t* __restrict__ leaf_hi,
                                                         int32_t* __restrict__ s_lpar, int32_t* __restrict__ s_lc,
@@ -576,12 +591,13 @@ __global__ void k_fix_int_escape(int32_t I, const int32_t* __restrict__ int_ry, 
 }
 
 // Full-tree emission from sorted codes d.codes[0..n-1] (n >= 2).
-void launch_emit_full(DevState& d, int32_t n, hipStream_t s) {
+template <class C>
+static void emit_full(DevState& d, const C* codes, int32_t n, hipStream_t s) {
     fill_u32(d.depths, n, 0u, s);
     fill_i32(d.s_top, n, -1, s);
-    hipLaunchKernelGGL(k_emit_internal, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d.codes, nullptr, nullptr, d.s_a,
-                       d.s_b, d.s_par, d.s_lc, d.s_rc, d.s_top, d.depths);
-    hipLaunchKernelGGL(k_emit_leaves, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.codes, nullptr, nullptr, d.s_lpar,
+    hipLaunchKernelGGL(k_emit_internal<C>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, codes, nullptr, nullptr,
+                       d.s_a, d.s_b, d.s_par, d.s_lc, d.s_rc, d.s_top, d.depths);
+    hipLaunchKernelGGL(k_emit_leaves<C>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, codes, nullptr, nullptr, d.s_lpar,
                        d.s_lc, d.s_rc);
     launch_exscan(d, d.depths, d.offsets, n, nullptr, s);
     hipLaunchKernelGGL(k_assign_preorder, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.s_top, d.depths, d.offsets,
@@ -590,6 +606,11 @@ void launch_emit_full(DevState& d, int32_t n, hipStream_t s) {
                        d.s_lc, d.s_rc, d.s_top, d.tkmap, nullptr, nullptr, d);
     hipLaunchKernelGGL(k_write_leaves, dim3(grid_for(n)), dim3(kBlock), 0, s, n, nullptr, d.s_lpar, d.s_top, d.tkmap,
                        d);
+}
+
+void launch_emit_full(DevState& d, int32_t n, hipStream_t s) { emit_full<uint32_t>(d, d.codes, n, s); }
+void launch_emit_full64(DevState& d, const uint64_t* keys, int32_t n, hipStream_t s) {
+    emit_full<uint64_t>(d, keys, n, s);
 }
 
 // ======================================================================= restructure emission
@@ -632,9 +653,9 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
                        leaf_hi, split_lo, split_hi);
     fill_u32(d.depths, n, 0u, s);
     fill_i32(d.s_top, n, -1, s);
-    hipLaunchKernelGGL(k_emit_internal, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d.codes, split_lo, split_hi,
+    hipLaunchKernelGGL(k_emit_internal<uint32_t>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d.codes, split_lo, split_hi,
                        d.s_a, d.s_b, d.s_par, d.s_lc, d.s_rc, d.s_top, d.depths);
-    hipLaunchKernelGGL(k_emit_leaves, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.codes, leaf_lo, leaf_hi, d.s_lpar,
+    hipLaunchKernelGGL(k_emit_leaves<uint32_t>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.codes, leaf_lo, leaf_hi, d.s_lpar,
                        d.s_lc, d.s_rc);
     launch_exscan(d, d.depths, d.offsets, n, nullptr, s);
     hipLaunchKernelGGL(k_run_base, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rt_of_leaf, d.int_rx, d.offsets, base);
@@ -745,8 +766,7 @@ __global__ void __launch_bounds__(kBlock) k_refit(int32_t n, DevState d) {
     for (int i = threadIdx.x; i < kRefitChunk; i += kBlock) flag[i] = 0;
     // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase
     for (int32_t x = s + threadIdx.x; x < e; x += kBlock) {
-        const int32_t t = d.prim_idx[x];
-        const int32_t* f = d.faces + 3 * (int64_t)t;
+        const int32_t* f = d.lfaces + 3 * (int64_t)x;
         bool fin = true;
         Box b = empty_box();
         for (int k = 0; k < 3; ++k) {
@@ -757,7 +777,7 @@ __global__ void __launch_bounds__(kBlock) k_refit(int32_t n, DevState d) {
             fin = fin && fabsf(vx) <= 1e8f && fabsf(vy) <= 1e8f && fabsf(vz) <= 1e8f;
             box_add(b, vx, vy, vz);
         }
-        d.tri_finite[t] = fin ? 1 : 0;
+        d.tri_finite[x] = fin ? 1 : 0;
         leafbox[x - s] = b;
         store_box(&d.leaf[x], b);
     }
@@ -810,8 +830,20 @@ __global__ void k_prim_map(int32_t n, const int32_t* __restrict__ prim_idx, int3
     if (s >= n) return;
     prim_map[prim_idx[s]] = s;
 }
+// Inverse permutation, and the faces gathered into leaf order once per sort so that refit and
+// the narrow phase read them sequentially instead of gathering by triangle id every frame.
+__global__ void k_leaf_faces(int32_t n, const int32_t* __restrict__ prim_idx, const int32_t* __restrict__ faces,
+                             int32_t* __restrict__ lfaces) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int64_t t = prim_idx[x];
+    lfaces[3 * (int64_t)x] = faces[3 * t];
+    lfaces[3 * (int64_t)x + 1] = faces[3 * t + 1];
+    lfaces[3 * (int64_t)x + 2] = faces[3 * t + 2];
+}
 void launch_prim_map(DevState& d, int32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_prim_map, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.prim_idx, d.prim_map);
+    hipLaunchKernelGGL(k_leaf_faces, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.prim_idx, d.faces, d.lfaces);
 }
 
 }  // namespace bvhcd

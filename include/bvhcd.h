@@ -183,8 +183,12 @@ int  bvhcd_mlp_set_threshold(bvhcd_ctx* ctx, float threshold);
    scheduler. */
 int  bvhcd_restructure(bvhcd_ctx* ctx, const int32_t* roots, int64_t k);
 
-/* ---- batched scenes (many scenes per launch; one hipGraph per frame) ---- */
+/* ---- batched scenes (many scenes per launch; one hipGraph per frame) ----
+   No counterpart object in the reference, which runs one scene per LBvh / BvttFront pair
+   (Frameworks/CudaProject/CudaProjectApp.cpp:28-34); each scene here gets exactly the pairs a single-scene context
+   would report for it (PURE_BVH_CD traversal + narrow phase). */
 typedef struct bvhcd_batch bvhcd_batch;
+/* cfg: device, stream and pair_cap are used (0 = derived from free HBM). */
 int  bvhcd_batch_create(const bvhcd_config* cfg, int32_t num_scenes, const int64_t* tris_per_scene,
                         const int64_t* verts_per_scene, bvhcd_batch** out);
 int  bvhcd_batch_destroy(bvhcd_batch* b);
@@ -192,18 +196,27 @@ int  bvhcd_batch_destroy(bvhcd_batch* b);
 int  bvhcd_batch_set_topology(bvhcd_batch* b, const int32_t* faces, int where);
 /* xyz: concatenated per-scene vertices. */
 int  bvhcd_batch_update_vertices(bvhcd_batch* b, const float* xyz, int where);
-/* One frame for every scene: build or refit + pure traversal + narrow phase, captured in a
-   hipGraph after the first frame (use_graph = 1). */
+/* One frame for every scene: bvh_op BUILD or REFIT, then the persistent self-CD traversal,
+   the narrow phase and the per-scene grouping of the hits. use_graph = 1: the frame is captured
+   in a hipGraph on first use (one graph per bvh_op) and replayed afterwards. */
 int  bvhcd_batch_step(bvhcd_batch* b, int bvh_op, int use_graph);
-/* Pairs of scene s (scene-local triangle ids), narrow-phase filtered. */
+/* Pairs of scene s (scene-local triangle ids, a < b), narrow-phase filtered; out = NULL only
+   returns the count. Synchronises the stream. */
 int  bvhcd_batch_get_pairs(bvhcd_batch* b, int32_t scene, int32_t* out, int64_t cap, int64_t* n);
 int  bvhcd_batch_pair_counts(bvhcd_batch* b, int64_t* counts /* num_scenes */);
 const char* bvhcd_batch_last_error(const bvhcd_batch* b);
 
-/* ---- autotuner ---- */
-/* Time candidate block sizes for every tuned kernel on the context's current data and keep
-   the fastest (cached per kernel and size class for the process). */
+/* ---- autotuner (replaces CudaDevice::launchConfig / evalOptimalBlockSize, CudaDevice.cu:60-94) ----
+   Block sizes are fixed at 256 (wave64 x 4); the grids of the grid-stride and persistent kernels
+   are tuned by timing candidates on the current data, per kernel and size class ceil(log2 n),
+   in a process-wide table used by every later launch. */
+/* KEEP and the narrow phase of a context with a built BVH and front; leaves the current front's
+   pairs in place. */
 int  bvhcd_autotune(bvhcd_ctx* ctx);
+/* The persistent batched traversal; drops the captured frame graphs (re-captured on next use). */
+int  bvhcd_batch_autotune(bvhcd_batch* b);
+/* Text table of every timed candidate (kernel, size class, grid, ms, chosen); returns its
+   length (copies at most cap - 1 bytes plus a terminator). */
 int  bvhcd_autotune_report(char* buf, int64_t cap);
 
 /* ---- seeded synthetic scenes (host) ---- */

@@ -102,6 +102,24 @@ def test_context_without_gpu_fails_loudly():
     pkg = load_package()
     with pytest.raises(pkg.BvhcdError):
         pkg.CollisionContext(100, 100)
+    with pytest.raises(pkg.BvhcdError):
+        pkg.Batch([10, 20], [30, 60])
+
+
+def test_batch_workload_specs():
+    """The batched workload: log-uniform triangle counts in [2e3, 2e5], sheet + icosphere."""
+    pkg = load_package()
+    specs = pkg.batch_scene_specs()
+    assert len(specs) == 64
+    tris = []
+    for s in specs[:16]:
+        sc = pkg.Scene(s)
+        sheet = 2 * (s.n - 1) ** 2
+        assert sc.num_tris == sheet + 1280  # icosphere, subdivision 3
+        assert 2e3 <= 2 * (s.n - 2) ** 2 <= 2e5 or 2e3 <= sheet <= 2.1e5
+        tris.append(sc.num_tris)
+    assert len(set(tris)) > 8
+    assert pkg.batch_scene_specs() == specs  # seeded
 
 
 def test_bench_rejects_multi_gpu():
