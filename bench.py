@@ -168,6 +168,8 @@ def main():
                                                       "mlp_scheduler.npz"),
                     help="trained scheduler (params + threshold) written by train_scheduler.py")
     ap.add_argument("--no-verify", action="store_true", help="skip the per-frame pair-set comparison pass")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="finish with three 1 GiB device-to-device copies (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
@@ -246,6 +248,13 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.calibrate_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration: 3 copies of 1 GiB done")
 
 
 if __name__ == "__main__":

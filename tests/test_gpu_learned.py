@@ -248,6 +248,44 @@ def test_front_restructure_matches_oracle(pkg, oracle, max_leaves, every):
     ctx.close()
 
 
+def _timed_run(pkg, sc, frames, weights=None, threshold=0.0):
+    ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts,
+                               scheduler=pkg.SCHED_LEARNED if weights is not None else pkg.SCHED_PLAIN)
+    ctx.set_topology(sc.faces(), sc.num_verts)
+    if weights is not None:
+        ctx.mlp_set_weights(weights)
+        ctx.mlp_set_threshold(threshold)
+    ctx.set_timing(len(frames))
+    dig = []
+    for v in frames:
+        ctx.update_vertices(v)
+        ctx.step_scheduled()
+        dig.append(ctx.digest(True))
+    ms, _, ops = ctx.timing_frames()
+    ctx.close()
+    return float(ms[12:, 7].sum()), dig, ops
+
+
+def test_trained_scheduler_gains_over_plain(pkg):
+    """The committed trained network on a sequence it never saw (256^2 cloth, seed 977): the
+    learned schedule is faster than the plain one with identical pair sets on every frame, and
+    faster than the same schedule told to rebuild every candidate (what its decisions avoid)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(pkg.__file__), "mlp_scheduler.npz"), allow_pickle=False)
+    params, thr = z["params"], float(z["threshold"])
+    sc = pkg.Scene.preset("cloth256", seed=977, frames=240)
+    frames = [sc.frame(f) for f in range(sc.frames)]
+    t_plain, d_plain, _ = _timed_run(pkg, sc, frames)
+    t_learn, d_learn, ops = _timed_run(pkg, sc, frames, params, thr)
+    t_all, d_all, _ = _timed_run(pkg, sc, frames, params, -1e30)
+    print(f"[gpu] plain {t_plain:.1f} ms, learned {t_learn:.1f} ms ({t_plain / t_learn:.3f}x), "
+          f"rebuild-all {t_all:.1f} ms, learned restructure frames {(ops[:, 0] == pkg.BVH_RESTRUCTURE).sum()}",
+          flush=True)
+    assert d_learn == d_plain and d_all == d_plain
+    assert t_learn < 0.9 * t_plain
+    assert t_learn < t_all
+
+
 def test_learned_schedule_keeps_pairs(pkg, oracle):
     """Random weights, threshold set so that some subtrees are picked: every frame's pair set
     equals the oracle's (plain schedule)."""
