@@ -77,7 +77,8 @@ def frame_digests(pkg, sc, faces, frames_dev, count, scheduler, weights):
     return out
 
 
-def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights=None, digests_upto=0):
+def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights=None, digests_upto=0,
+            autotune=True):
     ctx = make_ctx(pkg, sc, faces, scheduler, weights)
     nbytes = sc.num_verts * 12
     base = frames_dev.data_ptr()
@@ -87,6 +88,8 @@ def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights
         ctx.step_scheduled()
         if f < digests_upto:
             digests.append(ctx.digest(True))
+    if autotune:  # launch shapes tuned on this workload before the timed frames (process-wide table)
+        ctx.autotune()
     torch.cuda.synchronize()
     ctx.set_timing(steps)
     torch.cuda.synchronize()
@@ -168,6 +171,7 @@ def main():
                                                       "mlp_scheduler.npz"),
                     help="trained scheduler (params + threshold) written by train_scheduler.py")
     ap.add_argument("--no-verify", action="store_true", help="skip the per-frame pair-set comparison pass")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with three 1 GiB device-to-device copies (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -187,7 +191,8 @@ def main():
     frames_dev = upload_frames(pkg, sc, 0, warmup + steps, torch)
 
     ndig = min(warmup, 12)
-    plain = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_PLAIN, torch, digests_upto=ndig)
+    plain = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_PLAIN, torch, digests_upto=ndig,
+                    autotune=not args.no_autotune)
     plain_fps = steps / plain["wall_s"]
     log(f"plain: {plain_fps:.1f} frames/s ({1e3 * plain['wall_s'] / steps:.3f} ms/frame)")
 
@@ -197,7 +202,8 @@ def main():
     if os.path.exists(args.weights):
         z = np.load(args.weights, allow_pickle=False)
         weights = (z["params"], float(z["threshold"]))
-        learned = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED, torch, weights=weights)
+        learned = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED, torch, weights=weights,
+                          autotune=not args.no_autotune)
         log(f"learned: {steps / learned['wall_s']:.1f} frames/s "
             f"({learned['restructured']} restructure frames, {learned['updates']} UPDATE frames)")
         if not args.no_verify:  # untimed replay: pair sets of every frame, learned vs plain

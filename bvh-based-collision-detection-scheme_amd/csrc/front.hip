@@ -33,7 +33,8 @@
 namespace bvhcd {
 
 constexpr int kWaves = kBlock / 64;
-constexpr uint32_t kStage = 256;  // staged entries per wave per output list
+constexpr uint32_t kStage = 256;   // staged entries per wave, single-list kernels
+constexpr uint32_t kStage3 = 128;  // per list when a kernel stages three lists (LDS for 8 waves/SIMD)
 
 __device__ inline uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
 
@@ -79,13 +80,13 @@ __device__ void stage_flush(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, 
 // Called by the currently active lanes of a wave; `pred` selects the lanes that emit `v`.
 template <class Sink>
 __device__ inline void stage_push(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
-                                  bool pred, int2 v) {
+                                  uint32_t cap, bool pred, int2 v) {
     const uint64_t active = __ballot(1);
     const uint64_t m = __ballot(pred);
     if (m == 0) return;
     uint32_t f = *sfill;
     const uint32_t cnt = (uint32_t)__popcll(m);
-    if (f + cnt > kStage) {
+    if (f + cnt > cap) {
         stage_flush(sk, ovf, sbuf, f, active);
         f = 0;
     }
@@ -106,7 +107,7 @@ __device__ inline void stage_final(const Sink& sk, uint32_t* ovf, volatile int2*
 
 // Per-block staging storage for up to three lists.
 struct Stages {
-    int2 buf[3][kWaves][kStage];
+    int2 buf[3][kWaves][kStage3];
     uint32_t fill[3][kWaves];
 };
 __device__ inline void stages_init(Stages& st) {
@@ -120,16 +121,17 @@ struct Out {
     uint32_t* ovf;
     volatile int2* sb[3];
     volatile uint32_t* sf[3];
+    uint32_t scap;  // staging entries per wave and list
     // Broad-phase pairs are kept as leaf (sorted-slot) ids, query first (qi < j): the narrow
     // phase then reads leaf-ordered faces with locality, and maps to triangle ids itself.
     __device__ void pair(int32_t qi, int32_t j) const {
-        stage_push(bp, ovf, sb[2], sf[2], true, make_int2(qi, j));
+        stage_push(bp, ovf, sb[2], sf[2], scap, true, make_int2(qi, j));
     }
     __device__ void int_entry(int32_t qi, int32_t node) const {
-        stage_push(fi, ovf, sb[0], sf[0], true, make_int2(qi, node));
+        stage_push(fi, ovf, sb[0], sf[0], scap, true, make_int2(qi, node));
     }
     __device__ void ext_entry(int32_t qi, int32_t leaf) const {
-        stage_push(fe, ovf, sb[1], sf[1], true, make_int2(qi, leaf));
+        stage_push(fe, ovf, sb[1], sf[1], scap, true, make_int2(qi, leaf));
     }
     __device__ void final_flush() const {
         stage_final(fi, ovf, sb[0], sf[0]);
@@ -137,6 +139,30 @@ struct Out {
         stage_final(bp, ovf, sb[2], sf[2]);
     }
 };
+
+// Pair-only kernels (KEEP, pure traversal) stage one list: a third of the LDS, so 8 waves per
+// SIMD instead of 6 hide more of the dependent node loads.
+struct StagesBP {
+    int2 buf[kWaves][kStage];
+    uint32_t fill[kWaves];
+};
+__device__ inline void stages_init(StagesBP& st) {
+    if (lane_id() == 0) st.fill[threadIdx.x >> 6] = 0;
+}
+__device__ inline Out make_out(StagesBP& st, const DevState& d) {
+    Out o;
+    o.fi = FrontSink{nullptr, nullptr, nullptr, 0u, 0u};
+    o.fe = o.fi;
+    o.bp = PairSink{&d.ctr[C_BP], d.bp, (uint32_t)min((int64_t)d.pair_cap, (int64_t)0xFFFFFFFF), 4u};
+    o.ovf = &d.ctr[C_OVF];
+    const int w = threadIdx.x >> 6;
+    o.sb[0] = o.sb[1] = nullptr;
+    o.sf[0] = o.sf[1] = nullptr;
+    o.sb[2] = st.buf[w];
+    o.sf[2] = &st.fill[w];
+    o.scap = kStage;
+    return o;
+}
 
 __device__ inline Out make_out(Stages& st, const FrontSink& fi, const FrontSink& fe, const DevState& d) {
     Out o;
@@ -149,6 +175,7 @@ __device__ inline Out make_out(Stages& st, const FrontSink& fi, const FrontSink&
         o.sb[k] = st.buf[k][w];
         o.sf[k] = &st.fill[k][w];
     }
+    o.scap = kStage3;
     return o;
 }
 
@@ -332,9 +359,9 @@ void launch_pure_cd(DevState& d, int32_t n, hipStream_t s) {
 // all are taken (uneven scenes and uneven per-query work balance themselves).
 __global__ void __launch_bounds__(kBlock) k_batch_cd(int32_t n, DevState d, const uint64_t* __restrict__ keys,
                                                      const int32_t* __restrict__ scene_last, uint32_t* queue) {
-    __shared__ Stages st;
+    __shared__ StagesBP st;
     stages_init(st);
-    const Out out = make_out(st, null_sink(), null_sink(), d);
+    const Out out = make_out(st, d);
     const int lane = lane_id();
     uint32_t v = 0;
     for (;;) {
@@ -535,9 +562,9 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s) {
 
 // ---------------------------------------------------------------- KEEP
 __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in) {
-    __shared__ Stages st;
+    __shared__ StagesBP st;
     stages_init(st);
-    const Out out = make_out(st, null_sink(), null_sink(), d);
+    const Out out = make_out(st, d);
     const uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]), total = ni + ne;
     uint32_t visits = 0;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
@@ -597,17 +624,17 @@ __global__ void __launch_bounds__(kBlock) k_restr_classify(DevState d, FrontIn i
             const int32_t lx = d.int_rx[x];
             r = d.rt_of_leaf[lx];
             const bool inside = r >= 0 && x > r;  // x < r: an ancestor of r
-            stage_push(fi, out.ovf, out.sb[0], out.sf[0], !inside, make_int2(q, x));
+            stage_push(fi, out.ovf, out.sb[0], out.sf[0], out.scap, !inside, make_int2(q, x));
             task = inside && d.rt_of_leaf[q] != r && lx == d.int_rx[r];
         } else {
             q = in.query[1][k - ni];
             const int32_t j = (int32_t)in.node[1][k - ni];
             r = d.rt_of_leaf[j];
             const bool inside = r >= 0;
-            stage_push(fe, out.ovf, out.sb[1], out.sf[1], !inside, make_int2(q, j));
+            stage_push(fe, out.ovf, out.sb[1], out.sf[1], out.scap, !inside, make_int2(q, j));
             task = inside && d.rt_of_leaf[q] != r && j == d.int_rx[r];
         }
-        stage_push(tasks, out.ovf, out.sb[2], out.sf[2], task, make_int2(q, r));
+        stage_push(tasks, out.ovf, out.sb[2], out.sf[2], out.scap, task, make_int2(q, r));
     }
     stage_final(fi, out.ovf, out.sb[0], out.sf[0]);
     stage_final(fe, out.ovf, out.sb[1], out.sf[1]);
@@ -823,7 +850,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
             hit = sat_tri_tri(ld3(d.verts, a0), ld3(d.verts, a1), ld3(d.verts, a2), ld3(d.verts, b0),
                               ld3(d.verts, b1), ld3(d.verts, b2));
         }
-        stage_push(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w], hit, pr);
+        stage_push(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w], kStage, hit, pr);
     }
     stage_final(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w]);
 }

@@ -55,7 +55,7 @@ def test_batch_matches_oracle_per_scene(pkg, oracle):
         for f in range(6):
             orc.update_vertices(sc.frame(f))
             orc.step(0 if f % 3 == 0 else 1, 0)
-            np.testing.assert_array_equal(_set(got[f][k]), oracle.pair_set(orc.pairs(True)),
+            np.testing.assert_array_equal(oracle.pair_set(got[f][k]), oracle.pair_set(orc.pairs(True)),
                                           err_msg=f"scene {k} frame {f}")
         orc.close()
 
@@ -78,7 +78,7 @@ def test_batch_tiny_and_uneven_scenes(pkg, oracle):
     counts = b.pair_counts()
     for k in range(len(tps)):
         ref = oracle.pair_set(oracle.bruteforce_pairs(faces[k], verts[k], True))
-        np.testing.assert_array_equal(_set(b.pairs(k)), ref, err_msg=str(k))
+        np.testing.assert_array_equal(oracle.pair_set(b.pairs(k)), ref, err_msg=str(k))
         assert counts[k] == len(ref)
     b.close()
 
@@ -109,7 +109,7 @@ def test_autotune_keeps_results(pkg, oracle):
     b.step(pkg.BVH_REFIT, use_graph=True)  # re-captured with the tuned grid
     for k, sc in enumerate(scenes):
         ref = oracle.pair_set(oracle.bruteforce_pairs(sc.faces(), sc.frame(2), True))
-        np.testing.assert_array_equal(_set(b.pairs(k)), ref, err_msg=str(k))
+        np.testing.assert_array_equal(oracle.pair_set(b.pairs(k)), ref, err_msg=str(k))
     b.close()
     # single context: KEEP + narrow grids
     sc = pkg.Scene.preset("cloth256", n=96)
