@@ -159,12 +159,57 @@ def cpu_baseline(pkg, sc, faces, budget_s, threads, gpu_digests):
     return done, t_total, match, min(done, len(gpu_digests))
 
 
+def batch_bench(pkg, torch, steps, warmup, no_autotune):
+    """Batched scenes: 64 folding sheets (2e3..2e5 triangles, log-uniform) + icospheres, one
+    tree and one captured graph per frame op; BUILD every 12 frames, REFIT otherwise."""
+    specs = pkg.batch_scene_specs()
+    scenes = [pkg.Scene(s) for s in specs]
+    frames = min(specs[0].frames, warmup + steps)
+    tps = [s.num_tris for s in scenes]
+    vps = [s.num_verts for s in scenes]
+    T, V = sum(tps), sum(vps)
+    log(f"batch: {len(scenes)} scenes, {T} triangles, {V} vertices, {frames} frames")
+    dev = torch.empty((frames, V, 3), dtype=torch.float32, device="cuda")
+    for f in range(frames):
+        dev[f].copy_(torch.from_numpy(np.concatenate([s.frame(f) for s in scenes])))
+    torch.cuda.synchronize()
+    b = pkg.Batch(tps, vps)
+    b.set_topology(np.concatenate([s.faces() for s in scenes]))
+    nb = V * 12
+
+    def frame(f):
+        b.update_vertices(dev.data_ptr() + f * nb, pkg.MEM_DEVICE)
+        b.step(pkg.BVH_BUILD if f % 12 == 0 else pkg.BVH_REFIT, use_graph=True)
+
+    for f in range(warmup):
+        frame(f)
+    if not no_autotune:
+        b.autotune()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, frames):
+        frame(f)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    counts = b.pair_counts()
+    b.close()
+    n = frames - warmup
+    return {"metric": "batched CD triangles/sec (64 scenes, 2e3-2e5 tris each, exact pairs)",
+            "value": round(T * n / dt, 1), "unit": "triangles/s", "n_gpus": 1, "steps": n, "warmup": warmup,
+            "ms_per_step": round(1e3 * dt / n, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"batch64: {len(scenes)} sheet+icosphere scenes, {T} triangles", "graph": True},
+            "pairs_last_frame": int(counts.sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=360)
     ap.add_argument("--warmup", type=int, default=40)
-    ap.add_argument("--workload", default="cloth1024", choices=["cloth1024", "cloth256", "layers4", "spheres"])
+    ap.add_argument("--workload", default="cloth1024",
+                    choices=["cloth1024", "cloth256", "layers4", "spheres", "batch64"],
+                    help="batch64: the batched-scene workload (its own JSON line, triangles/s)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--weights", default=os.path.join(ROOT, "bvh-based-collision-detection-scheme_amd",
@@ -183,6 +228,10 @@ def main():
     import torch
 
     pkg = load_pkg()
+    if args.workload == "batch64":
+        print(json.dumps(batch_bench(pkg, torch, min(args.steps, 138), min(args.warmup, 12), args.no_autotune)),
+              flush=True)
+        return
     sc = pkg.Scene.preset(args.workload)
     steps = min(args.steps, sc.frames - args.warmup)
     warmup = args.warmup

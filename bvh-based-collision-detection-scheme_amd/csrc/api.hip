@@ -37,7 +37,6 @@ struct bvhcd_ctx {
     int64_t T = 0, V = 0;
     bool built = false, front_valid = false, restr_pending = false;
     bool learned_frame = false;  // set by the learned scheduler for its decision frames
-    bool bp_tris = false;        // d.bp already converted from leaf ids to triangle ids
     bool repair_pending = false; // front classified for the incremental front restructure
     uint32_t repair_tasks = 0;   // its (query, subtree) walks
     int64_t frame = 0;
@@ -161,8 +160,8 @@ int allocate(bvhcd_ctx* c) {
     if (ce <= 0) ce = 24 * T + (1 << 20);
     if (cp <= 0) cp = 24 * T + (1 << 20);
     ci = std::min(ci, lim); ce = std::min(ce, lim); cp = std::min(cp, lim);
-    // 2 buffers x 8 B for each front, 2 x 8 B for pairs
-    const double need = 16.0 * (double)(ci + ce) + 16.0 * (double)cp;
+    // 2 buffers x 8 B for each front, 3 x 8 B for pairs (broad, narrow, read-out copy)
+    const double need = 16.0 * (double)(ci + ce) + 24.0 * (double)cp;
     const double budget = 0.7 * (double)free_b;
     if (need > budget && (c->cfg.int_front_cap <= 0 || c->cfg.ext_front_cap <= 0 || c->cfg.pair_cap <= 0)) {
         const double f = budget / need;
@@ -172,7 +171,9 @@ int allocate(bvhcd_ctx* c) {
     }
     d.front_cap[0] = std::max<int64_t>(ci, 1);
     d.front_cap[1] = std::max<int64_t>(ce, 1);
-    d.pair_cap = std::max<int64_t>(cp, 1);
+    // pair lists: kPairSegs equal segments (an explicit capacity is split evenly)
+    d.pair_seg_cap = std::max<int64_t>((std::max<int64_t>(cp, 1) + kPairSegs - 1) / kPairSegs, 1);
+    d.pair_cap = d.pair_seg_cap * kPairSegs;
     for (int k = 0; k < 2; ++k)
         for (int b = 0; b < 2; ++b) {
             ok &= dalloc(c, d.fnode[k][b], d.front_cap[k]);
@@ -188,8 +189,11 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.rtasks, d.rtask_cap);
     ok &= dalloc(c, d.bp, d.pair_cap);
     ok &= dalloc(c, d.np, d.pair_cap);
+    ok &= dalloc(c, d.pair_tmp, d.pair_cap);
+    ok &= dalloc(c, d.pcnt, 2 * kPairSegs * kCntStride);
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
     HIPCHK(c, hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(d.pcnt, 0, 2 * kPairSegs * kCntStride * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(d.fsize, 0, 4 * sizeof(uint32_t), c->stream));
     return BVHCD_OK;
 }
@@ -232,8 +236,12 @@ void log_frame(bvhcd_ctx* c, int bvh_op, int front_op) {
 //   B_front  = 32 N_in + 36 N_visit + 12 N_out + 8 P_bp
 //   B_order  = 16 N_out + 16T
 //   B_narrow = 104 P_bp + 8 P_np
+// Algorithmic bytes per stage (read-once / write-once compulsory traffic). The front stage's
+// event brackets include the node-id sort of GENERATE / UPDATE / REORDER (16 bytes per entry
+// and 16 per node over the passes' reads and writes), so its bytes count there; the
+// order+quality stage only times the quality snapshot (16 bytes per node).
 void frame_bytes(double T, double V, int bvh_op, int front_op, double visits, double fi0, double fe0, double fi1,
-                 double fe1, double bp, double np, double restr_leaves, double* out) {
+                 double fe1, double bp, double np, double restr_leaves, bool ordered, double* out) {
     const double I = T > 1 ? T - 1 : 0;
     for (int i = 0; i < 8; ++i) out[i] = 0;
     if (bvh_op == BVHCD_BVH_BUILD)
@@ -245,8 +253,10 @@ void frame_bytes(double T, double V, int bvh_op, int front_op, double visits, do
     const double nin = whole ? T : fi0 + fe0;
     const double nout = (front_op == BVHCD_FRONT_KEEP || front_op == BVHCD_FRONT_PURE_BVH_CD) ? 0 : fi1 + fe1;
     out[S_FRONT] = 32 * nin + 36 * visits + 12 * nout + 8 * bp;
-    if (front_op == BVHCD_FRONT_GENERATE || front_op == BVHCD_FRONT_UPDATE || front_op == BVHCD_FRONT_REORDER)
-        out[S_ORDER] = 16 * nout + 16 * T;
+    if (ordered && (front_op == BVHCD_FRONT_GENERATE || front_op == BVHCD_FRONT_UPDATE))
+        out[S_FRONT] += 16 * nout + 16 * T;
+    if (front_op == BVHCD_FRONT_REORDER) out[S_FRONT] += 16 * fi1 + 16 * T;  // the internal front only
+    if (ordered && front_op == BVHCD_FRONT_GENERATE) out[S_ORDER] = 16 * T;
     out[S_NARROW] = 104 * bp + 8 * np;
     double tot = 0;
     for (int i = 0; i < 7; ++i) tot += out[i];
@@ -405,7 +415,11 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     }
     if (front_op != BVHCD_FRONT_PURE_BVH_CD && !c->front_valid) front_op = BVHCD_FRONT_GENERATE;
     c->restr_pending = false;  // the quality flag lives for one frame
-    const bool quality = c->cfg.enable_restructure != 0;
+    // The learned schedule neither reads the node-id ordering nor the front-quality heuristic:
+    // its fronts keep emission order (neighbouring queries together, which KEEP streams with
+    // better locality than node order).
+    const bool order = c->cfg.scheduler != BVHCD_SCHED_LEARNED;
+    const bool quality = c->cfg.enable_restructure != 0 && order;
     switch (front_op) {
         case BVHCD_FRONT_PURE_BVH_CD:
             begin(c, S_FRONT);
@@ -414,7 +428,7 @@ int run_front(bvhcd_ctx* c, int& front_op) {
             break;
         case BVHCD_FRONT_GENERATE:
             begin(c, S_FRONT);
-            launch_generate(d, n, s);  // traversal + ordering
+            launch_generate(d, n, s, order);  // traversal + ordering
             end(c, S_FRONT);
             begin(c, S_ORDER);
             if (quality) launch_snapshot(d, n, s);
@@ -423,7 +437,7 @@ int run_front(bvhcd_ctx* c, int& front_op) {
             break;
         case BVHCD_FRONT_UPDATE:
             begin(c, S_FRONT);
-            launch_update(d, n, quality, s);  // prune/sprout + quality + ordering
+            launch_update(d, n, quality, s, order);  // prune/sprout + quality + ordering
             end(c, S_FRONT);
             if (quality) c->restr_pending = true;
             break;
@@ -448,7 +462,8 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     }
     begin(c, S_NARROW);
     if (c->cfg.narrow_phase) launch_narrow(d, s);
-    else (void)hipMemsetAsync(d.ctr + C_NP, 0, sizeof(uint32_t), s);
+    else clear_np(d, s);
+    launch_pair_totals(d, s);
     end(c, S_NARROW);
     return BVHCD_OK;
 }
@@ -458,7 +473,6 @@ int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     if (bvh_op < 0 || bvh_op > BVHCD_BVH_NONE || front_op < 0 || front_op > BVHCD_FRONT_KEEP)
         return fail(c, BVHCD_E_INVALID, "bad op");
     begin(c, S_TOTAL);
-    c->bp_tris = false;
     bool topo = false;
     int st = run_bvh(c, bvh_op, topo);
     if (st) return st;
@@ -484,13 +498,6 @@ void plain_schedule(int64_t f, int bc, int fc, int& bvh_op, int& front_op) {
     else front_op = BVHCD_FRONT_KEEP;
     if (bvh_op == BVHCD_BVH_BUILD) front_op = BVHCD_FRONT_GENERATE;
     if ((f + 1) % bc == 0) front_op = BVHCD_FRONT_KEEP;
-}
-
-// The broad-phase list holds leaf ids until it is read out as triangle pairs.
-void bp_as_tris(bvhcd_ctx* c) {
-    if (c->bp_tris || !c->built) return;
-    launch_bp_to_tris(c->d, c->stream);
-    c->bp_tris = true;
 }
 
 int check_overflow(bvhcd_ctx* c) {
@@ -640,7 +647,6 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
 
 int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int where, int filtered) {
     if (!c || !num) return BVHCD_E_INVALID;
-    if (!filtered) bp_as_tris(c);
     uint32_t cnt = 0;
     HIPCHK(c, hipMemcpyAsync(&cnt, c->d.ctr + (filtered ? C_NP : C_BP), sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
     int st = check_overflow(c);
@@ -648,10 +654,16 @@ int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int w
     if (st) return st;
     if (out && cap > 0) {
         const int64_t k = std::min<int64_t>(cap, cnt);
-        if (k > 0)
-            HIPCHK(c, hipMemcpyAsync(out, filtered ? c->d.np : c->d.bp, sizeof(int2) * k,
-                                     where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                                     c->stream));
+        if (k > 0) {
+            // segments gathered (broad phase: leaf ids mapped to triangle ids) straight into a
+            // device buffer, or into the read-out copy for the host
+            if (where == BVHCD_MEM_DEVICE) {
+                launch_pair_gather(c->d, filtered ? 1 : 0, reinterpret_cast<int2*>(out), k, c->stream);
+            } else {
+                launch_pair_gather(c->d, filtered ? 1 : 0, c->d.pair_tmp, k, c->stream);
+                HIPCHK(c, hipMemcpyAsync(out, c->d.pair_tmp, sizeof(int2) * k, hipMemcpyDeviceToHost, c->stream));
+            }
+        }
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return BVHCD_OK;
@@ -659,7 +671,6 @@ int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int w
 
 int bvhcd_pair_digest(bvhcd_ctx* c, int filtered, uint64_t* digest, int64_t* num) {
     if (!c || !digest) return BVHCD_E_INVALID;
-    if (!filtered) bp_as_tris(c);
     launch_digest(c->d, filtered, c->stream);
     uint64_t dg = 0;
     uint32_t cnt = 0;
@@ -709,7 +720,8 @@ int bvhcd_timing_frames(bvhcd_ctx* c, int64_t cap, double* ms, double* bytes, in
         }
         if (bytes)
             frame_bytes((double)c->T, (double)c->V, bo, fo, (double)L[0], fi0, fe0, (double)L[3], (double)L[4],
-                        (double)L[1], (double)L[2], (double)c->trestr[f], bytes + 8 * f);
+                        (double)L[1], (double)L[2], (double)c->trestr[f], c->cfg.scheduler != BVHCD_SCHED_LEARNED,
+                        bytes + 8 * f);
         if (ops) { ops[2 * f] = bo; ops[2 * f + 1] = fo; }
         fi0 = (double)L[3];
         fe0 = (double)L[4];
@@ -735,7 +747,9 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     out->np_pairs = ctr[C_NP];
     out->required_int_front = out->int_front;
     out->required_ext_front = out->ext_front;
-    out->required_pairs = ctr[C_BP];
+    // pair lists are kPairSegs equal segments: what fits every segment
+    out->required_pairs = std::max<int64_t>((int64_t)ctr[C_BP],
+                                            (int64_t)std::max(ctr[C_BP_SEGMAX], ctr[C_NP_SEGMAX]) * kPairSegs);
     out->node_visits = (int64_t)c64[C64_VISITS];
     out->restructured_subtrees = c->restr_sub;
     out->restructured_leaves = c->restr_leaf;
@@ -750,7 +764,7 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     }
     frame_bytes((double)c->T, (double)c->V, c->last_bvh, c->last_front, (double)c64[C64_VISITS], 0, 0,
                 (double)out->int_front, (double)out->ext_front, (double)out->bp_pairs, (double)out->np_pairs,
-                (double)c->restr_leaf, out->stage_bytes);
+                (double)c->restr_leaf, c->cfg.scheduler != BVHCD_SCHED_LEARNED, out->stage_bytes);
     if (ctr[C_OVF]) {
         if (ctr[C_OVF] & 16u) return fail(c, BVHCD_E_DEVICE, "traversal step bound exceeded (corrupted tree)");
         out->required_int_front = std::max<int64_t>(out->int_front, c->d.front_cap[0] + 1);
@@ -916,8 +930,9 @@ int bvhcd_autotune(bvhcd_ctx* c) {
     (void)hipEventDestroy(e1);
     // leave a consistent result: the current front's pairs and their narrow phase
     launch_keep(d, n, c->stream);
-    c->bp_tris = false;
     if (c->cfg.narrow_phase) launch_narrow(d, c->stream);
+    else clear_np(d, c->stream);
+    launch_pair_totals(d, c->stream);
     return check_overflow(c);
 }
 

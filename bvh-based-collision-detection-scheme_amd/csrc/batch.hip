@@ -197,11 +197,12 @@ __device__ inline void wave_add(uint32_t* counter, int32_t key, bool pred, uint3
 
 __global__ void __launch_bounds__(kBlock) k_pair_scene_count(DevState d, const int32_t* __restrict__ tri_scene,
                                                              uint32_t* __restrict__ scnt) {
-    const uint32_t m = (uint32_t)min((int64_t)d.ctr[C_NP], d.pair_cap);
+    __shared__ uint32_t pre[kPairSegs + 1];
+    const uint32_t m = seg_prefix(d.pcnt + kPairSegs * kCntStride, (uint32_t)d.pair_seg_cap, pre);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m + 63u; k += gridDim.x * blockDim.x) {
         if ((k & ~63u) >= m) break;  // wave-uniform
         const bool ok = k < m;
-        const int32_t s = ok ? tri_scene[d.np[k].x] : 0;
+        const int32_t s = ok ? tri_scene[d.np[seg_slot(pre, k, d.pair_seg_cap)].x] : 0;
         wave_add(scnt, s, ok, nullptr);
     }
 }
@@ -210,11 +211,12 @@ __global__ void __launch_bounds__(kBlock) k_pair_scene_scatter(DevState d, const
                                                                const int64_t* __restrict__ toff,
                                                                const uint32_t* __restrict__ soff,
                                                                uint32_t* __restrict__ scur, int2* __restrict__ out) {
-    const uint32_t m = (uint32_t)min((int64_t)d.ctr[C_NP], d.pair_cap);
+    __shared__ uint32_t pre[kPairSegs + 1];
+    const uint32_t m = seg_prefix(d.pcnt + kPairSegs * kCntStride, (uint32_t)d.pair_seg_cap, pre);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m + 63u; k += gridDim.x * blockDim.x) {
         if ((k & ~63u) >= m) break;
         const bool ok = k < m;
-        const int2 p = ok ? d.np[k] : make_int2(0, 0);
+        const int2 p = ok ? d.np[seg_slot(pre, k, d.pair_seg_cap)] : make_int2(0, 0);
         const int32_t s = ok ? tri_scene[p.x] : 0;
         uint32_t slot = 0;
         wave_add(scur, s, ok, &slot);
@@ -251,6 +253,7 @@ void record_frame(bvhcd_batch* b, int bvh_op, unsigned grid) {
     launch_refit(d, n, s);
     launch_batch_cd(d, n, b->keys, b->scene_last, b->queue, grid, s);
     launch_narrow(d, s);
+    launch_pair_totals(d, s);
     fill_u32(b->scnt, b->S + 1, 0u, s);
     fill_u32(b->scur, b->S + 1, 0u, s);
     hipLaunchKernelGGL(k_pair_scene_count, dim3(1024), dim3(kBlock), 0, s, d, b->tri_scene, b->scnt);
@@ -318,7 +321,8 @@ int bvhcd_batch_create(const bvhcd_config* cfg, int32_t S, const int64_t* tps, c
     int64_t cp = cfg->pair_cap > 0 ? cfg->pair_cap : 24 * T + (1 << 20);
     cp = std::min<int64_t>(cp, 0x7FFFFFFFll);
     if (cfg->pair_cap <= 0) cp = std::min<int64_t>(cp, (int64_t)(0.5 * (double)free_b / 24.0));
-    d.pair_cap = std::max<int64_t>(cp, 1);
+    d.pair_seg_cap = std::max<int64_t>((std::max<int64_t>(cp, 1) + kPairSegs - 1) / kPairSegs, 1);
+    d.pair_cap = d.pair_seg_cap * kPairSegs;
     const int64_t tiles = (T + 4095) / 4096;
     d.sort_hist_n = (size_t)(256 * tiles);
     d.scan_tmp_n = (size_t)((std::max<int64_t>((int64_t)d.sort_hist_n, T + 2) + 4095) / 4096 + 1);
@@ -362,6 +366,7 @@ int bvhcd_batch_create(const bvhcd_config* cfg, int32_t S, const int64_t* tps, c
     ok &= balloc(b, d.bp, d.pair_cap);
     ok &= balloc(b, d.np, d.pair_cap);
     ok &= balloc(b, b->grouped, d.pair_cap);
+    ok &= balloc(b, d.pcnt, 2 * kPairSegs * kCntStride);
     ok &= balloc(b, d.intbeg, 2 * (S + 1));  // holds the device copy of the triangle offsets
     ok &= balloc(b, b->tri_scene, T);
     ok &= balloc(b, b->scene_last, S);
@@ -388,7 +393,8 @@ int bvhcd_batch_create(const bvhcd_config* cfg, int32_t S, const int64_t* tps, c
                               b->stream) == hipSuccess &&
                hipMemcpyAsync(b->voff_dev, b->voff.data(), sizeof(int64_t) * (S + 1), hipMemcpyHostToDevice,
                               b->stream) == hipSuccess &&
-               hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), b->stream) == hipSuccess;
+               hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), b->stream) == hipSuccess &&
+               hipMemsetAsync(d.pcnt, 0, 2 * kPairSegs * kCntStride * sizeof(uint32_t), b->stream) == hipSuccess;
     if (okc) {
         hipLaunchKernelGGL(k_tri_scene, dim3(std::min<unsigned>(grid_for(T), 4096)), dim3(kBlock), 0, b->stream, T, S,
                            toff_dev_of(b), b->tri_scene);

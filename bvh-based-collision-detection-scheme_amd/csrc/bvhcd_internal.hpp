@@ -12,6 +12,9 @@
 
 namespace bvhcd {
 
+constexpr int kPairSegs = 64;   // pair-list segments (one append counter each)
+constexpr int kCntStride = 32;  // words between two segment counters (128 bytes)
+
 // ------------------------------------------------------------------ node records (HBM layout)
 // One 32-byte record per node, read with two 16-byte loads by the traversal kernels:
 //   internal node n (preorder): lo = (minx, miny, minz, rangex), hi = (maxx, maxy, maxz, escape)
@@ -93,6 +96,35 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
     if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Segmented pair lists read by a virtual index: seg_prefix builds, per workgroup, the exclusive
+// prefix of the kPairSegs segment counts (clamped to the segment capacity) in LDS `pre`
+// (kPairSegs + 1 words; all threads call it, it holds a barrier) and returns the total;
+// seg_slot maps virtual index k < total to its array slot.
+static_assert(kPairSegs == 64, "one wave scans the segment counts");
+__device__ inline uint32_t seg_prefix(const uint32_t* __restrict__ cnt, uint32_t seg_cap, uint32_t* pre) {
+    if (threadIdx.x < 64) {
+        const uint32_t v = min(cnt[threadIdx.x * kCntStride], seg_cap);
+        uint32_t x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if ((int)threadIdx.x >= off) x += y;
+        }
+        pre[threadIdx.x + 1] = x;
+        if (threadIdx.x == 0) pre[0] = 0;
+    }
+    __syncthreads();
+    return pre[kPairSegs];
+}
+__device__ inline int64_t seg_slot(const uint32_t* pre, uint32_t k, int64_t seg_cap) {
+    int lo = 0, hi = kPairSegs - 1;  // last segment g with pre[g] <= k
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (pre[m] <= k) lo = m;
+        else hi = m - 1;
+    }
+    return (int64_t)lo * seg_cap + (int64_t)(k - pre[lo]);
 }
 
 // ------------------------------------------------------------------ device buffers
@@ -186,9 +218,15 @@ struct DevState {
     int2* rtasks = nullptr;
     int64_t rtask_cap = 0;
     // pairs
-    int2* bp = nullptr;             // broad-phase pairs
-    int2* np = nullptr;             // narrow-phase hits
-    int64_t pair_cap = 0;
+    // Pair lists are split into kPairSegs segments with one counter each (128 bytes apart), a
+    // workgroup appending to segment blockIdx % kPairSegs: one shared counter serialises the
+    // appends of the whole device. Segment g holds [g * pair_seg_cap, g * pair_seg_cap + count).
+    int2* bp = nullptr;             // broad-phase pairs (leaf ids, query leaf first)
+    int2* np = nullptr;             // narrow-phase hits (triangle ids, lower first)
+    int2* pair_tmp = nullptr;       // gathered contiguous copy for read-out
+    int64_t pair_cap = 0;           // kPairSegs * pair_seg_cap
+    int64_t pair_seg_cap = 0;
+    uint32_t* pcnt = nullptr;       // [2][kPairSegs * kCntStride]: bp, np segment counts
     // counters block: see Counter enum
     uint32_t* ctr = nullptr;        // 32 words
 };
@@ -203,6 +241,8 @@ enum Counter : int {
     C_TASKS = 8,       // incremental front restructure: (query, subtree) walks
     C_KEPT_I = 9,      // incremental front restructure: kept internal / external entries
     C_KEPT_E = 10,
+    C_BP_SEGMAX = 11,  // largest broad-phase segment count requested (capacity reporting)
+    C_NP_SEGMAX = 12,
     C_NUM = 16         // words 16..31 hold 64-bit counters (see Counter64)
 };
 enum Counter64 : int {

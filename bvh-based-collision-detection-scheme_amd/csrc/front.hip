@@ -34,7 +34,10 @@ namespace bvhcd {
 
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kStage = 256;   // staged entries per wave, single-list kernels
-constexpr uint32_t kStage3 = 128;  // per list when a kernel stages three lists (LDS for 8 waves/SIMD)
+// Per list when a kernel stages three lists. 128 would fit 8 waves/SIMD in LDS, but every flush
+// is one atomic on the list's single counter and those serialise (measured: GENERATE 2.9 ms at
+// 256, 5.5 ms at 128 on the 2M-triangle cloth), so fewer, larger flushes win.
+constexpr uint32_t kStage3 = 256;
 
 __device__ inline uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
 
@@ -140,6 +143,13 @@ struct Out {
     }
 };
 
+// Segment of a pair list that this workgroup appends to (kind 0 broad phase, 1 narrow phase).
+__device__ inline PairSink pair_segment(const DevState& d, int kind, uint32_t ovf_bit) {
+    const int g = blockIdx.x % kPairSegs;
+    return PairSink{d.pcnt + (kind * kPairSegs + g) * kCntStride, (kind ? d.np : d.bp) + (int64_t)g * d.pair_seg_cap,
+                    (uint32_t)min(d.pair_seg_cap, (int64_t)0xFFFFFFFF), ovf_bit};
+}
+
 // Pair-only kernels (KEEP, pure traversal) stage one list: a third of the LDS, so 8 waves per
 // SIMD instead of 6 hide more of the dependent node loads.
 struct StagesBP {
@@ -153,7 +163,7 @@ __device__ inline Out make_out(StagesBP& st, const DevState& d) {
     Out o;
     o.fi = FrontSink{nullptr, nullptr, nullptr, 0u, 0u};
     o.fe = o.fi;
-    o.bp = PairSink{&d.ctr[C_BP], d.bp, (uint32_t)min((int64_t)d.pair_cap, (int64_t)0xFFFFFFFF), 4u};
+    o.bp = pair_segment(d, 0, 4u);
     o.ovf = &d.ctr[C_OVF];
     const int w = threadIdx.x >> 6;
     o.sb[0] = o.sb[1] = nullptr;
@@ -168,7 +178,7 @@ __device__ inline Out make_out(Stages& st, const FrontSink& fi, const FrontSink&
     Out o;
     o.fi = fi;
     o.fe = fe;
-    o.bp = PairSink{&d.ctr[C_BP], d.bp, (uint32_t)min((int64_t)d.pair_cap, (int64_t)0xFFFFFFFF), 4u};
+    o.bp = pair_segment(d, 0, 4u);
     o.ovf = &d.ctr[C_OVF];
     const int w = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
@@ -273,10 +283,12 @@ static FrontIn cur_front(DevState& d) {
 }
 __host__ __device__ inline FrontSink null_sink() { return FrontSink{nullptr, nullptr, nullptr, 0u, 0u}; }
 
+// Pair totals, overflow bits, visits and the broad-phase segment counters (fill kernels, no
+// memset nodes: the batched frame graph records this too).
 static void reset_step_counters(DevState& d, hipStream_t s) {
-    // pair counts, overflow, visits
-    hipMemsetAsync(d.ctr, 0, sizeof(uint32_t) * 3, s);
-    hipMemsetAsync(ctr64(d.ctr) + C64_VISITS, 0, sizeof(uint64_t), s);
+    fill_u32(d.ctr, 3, 0u, s);
+    fill_u32(reinterpret_cast<uint32_t*>(ctr64(d.ctr) + C64_VISITS), 2, 0u, s);
+    fill_u32(d.pcnt, kPairSegs * kCntStride, 0u, s);
 }
 
 static unsigned stream_grid() { return 2048; }
@@ -386,8 +398,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_cd(int32_t n, DevState d, cons
 // a graph; `grid` workgroups stay resident and drain the query queue.
 void launch_batch_cd(DevState& d, int32_t n, const uint64_t* keys, const int32_t* scene_last, uint32_t* queue,
                      unsigned grid, hipStream_t s) {
-    fill_u32(d.ctr, 3, 0u, s);
-    fill_u32(reinterpret_cast<uint32_t*>(ctr64(d.ctr) + C64_VISITS), 2, 0u, s);
+    reset_step_counters(d, s);
     fill_u32(queue, 1, 0u, s);
     if (n >= 2) hipLaunchKernelGGL(k_batch_cd, dim3(grid), dim3(kBlock), 0, s, n, d, keys, scene_last, queue);
 }
@@ -436,7 +447,7 @@ void launch_order_fronts(DevState& d, int32_t n, hipStream_t s) {
                        d.fsize + 2 + d.fcur[1], (uint32_t)std::min<int64_t>(d.front_cap[1], 0xFFFFFFFFll), n, d.extbeg);
 }
 
-void launch_generate(DevState& d, int32_t n, hipStream_t s) {
+void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order) {
     reset_step_counters(d, s);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
     hipMemsetAsync(fi.count, 0, sizeof(uint32_t), s);
@@ -444,7 +455,7 @@ void launch_generate(DevState& d, int32_t n, hipStream_t s) {
     if (n >= 2) hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, fi, fe);
     d.fcur[0] ^= 1;
     d.fcur[1] ^= 1;
-    if (n >= 2) launch_order_fronts(d, n, s);
+    if (order && n >= 2) launch_order_fronts(d, n, s);
 }
 
 // ---------------------------------------------------------------- UPDATE (prune / sprout)
@@ -543,7 +554,8 @@ void launch_snapshot(DevState& d, int32_t n, hipStream_t s) {
     if (n >= 2) hipLaunchKernelGGL(k_snapshot, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d);
 }
 
-void launch_update(DevState& d, int32_t n, int quality, hipStream_t s) {
+void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool order) {
+    if (!order) quality = 0;  // the quality check reads the ordering's bucket offsets
     reset_step_counters(d, s);
     const FrontIn in = cur_front(d);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
@@ -557,7 +569,7 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s) {
         fill_i32(d.rtroot, n, INT32_MAX, s);
         hipLaunchKernelGGL(k_quality, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d);
     }
-    if (n >= 2) launch_order_fronts(d, n, s);
+    if (order && n >= 2) launch_order_fronts(d, n, s);
 }
 
 // ---------------------------------------------------------------- KEEP
@@ -822,12 +834,13 @@ __device__ bool sat_tri_tri(float3 P1, float3 P2, float3 P3, float3 Q1, float3 Q
 __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
     __shared__ int2 sbuf[kWaves][kStage];
     __shared__ uint32_t sfill[kWaves];
+    __shared__ uint32_t pre[kPairSegs + 1];
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) sfill[w] = 0;
-    const PairSink sink{&d.ctr[C_NP], d.np, (uint32_t)min((int64_t)d.pair_cap, (int64_t)0xFFFFFFFF), 8u};
-    const uint32_t npairs = min(d.ctr[C_BP], (uint32_t)d.pair_cap);
+    const PairSink sink = pair_segment(d, 1, 8u);
+    const uint32_t npairs = seg_prefix(d.pcnt, (uint32_t)d.pair_seg_cap, pre);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += gridDim.x * blockDim.x) {
-        const int2 lv = d.bp[k];  // leaves
+        const int2 lv = d.bp[seg_slot(pre, k, d.pair_seg_cap)];  // leaves
         // canonical order: the triangle with the lower id is P
         int32_t ta = d.prim_idx[lv.x], tb = d.prim_idx[lv.y], la = lv.x, lb = lv.y;
         if (ta > tb) {
@@ -855,22 +868,51 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
     stage_final(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w]);
 }
 
-// Broad-phase list from leaf ids to triangle ids (lower id first), in place, for export.
-__global__ void __launch_bounds__(kBlock) k_bp_to_tris(DevState d) {
-    const uint32_t m = min(d.ctr[C_BP], (uint32_t)d.pair_cap);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
-        const int2 lv = d.bp[k];
-        const int32_t a = d.prim_idx[lv.x], b = d.prim_idx[lv.y];
-        d.bp[k] = a < b ? make_int2(a, b) : make_int2(b, a);
-    }
-}
-void launch_bp_to_tris(DevState& d, hipStream_t s) {
-    hipLaunchKernelGGL(k_bp_to_tris, dim3(stream_grid()), dim3(kBlock), 0, s, d);
+void launch_narrow(DevState& d, hipStream_t s) {
+    // kernels, not memsets: the batched frame graph records this
+    fill_u32(d.pcnt + kPairSegs * kCntStride, kPairSegs * kCntStride, 0u, s);
+    hipLaunchKernelGGL(k_narrow, dim3(tuned_grid(TK_NARROW, d.maxT)), dim3(kBlock), 0, s, d);
 }
 
-void launch_narrow(DevState& d, hipStream_t s) {
-    fill_u32(d.ctr + C_NP, 1, 0u, s);  // a kernel, not a memset: the batched frame graph records it
-    hipLaunchKernelGGL(k_narrow, dim3(tuned_grid(TK_NARROW, d.maxT)), dim3(kBlock), 0, s, d);
+// Totals of both segmented pair lists (requested counts, so an overflowing list reports what it
+// needed) and their largest segment: ctr[C_BP], ctr[C_NP], ctr[C_BP_SEGMAX], ctr[C_NP_SEGMAX].
+__global__ void k_pair_totals(DevState d) {
+    const int kind = threadIdx.x >> 6, g = lane_id();
+    uint32_t v = d.pcnt[(kind * kPairSegs + g) * kCntStride], m = v;
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_down(v, off);
+        m = max(m, __shfl_down(m, off));
+    }
+    if (g == 0) {
+        d.ctr[kind ? C_NP : C_BP] = v;
+        d.ctr[kind ? C_NP_SEGMAX : C_BP_SEGMAX] = m;
+    }
+}
+void launch_pair_totals(DevState& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_pair_totals, dim3(1), dim3(128), 0, s, d);
+}
+void clear_np(DevState& d, hipStream_t s) {
+    fill_u32(d.pcnt + kPairSegs * kCntStride, kPairSegs * kCntStride, 0u, s);
+}
+
+// Contiguous copy of a segmented pair list (at most cap pairs) into dst; the broad-phase list
+// (leaf ids) is mapped to triangle ids, lower first.
+__global__ void __launch_bounds__(kBlock) k_pair_gather(DevState d, int kind, int2* __restrict__ dst, uint64_t cap) {
+    __shared__ uint32_t pre[kPairSegs + 1];
+    const uint32_t n = seg_prefix(d.pcnt + kind * kPairSegs * kCntStride, (uint32_t)d.pair_seg_cap, pre);
+    const uint32_t m = (uint32_t)min((uint64_t)n, cap);
+    const int2* src = kind ? d.np : d.bp;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        int2 p = src[seg_slot(pre, k, d.pair_seg_cap)];
+        if (kind == 0) {
+            const int32_t a = d.prim_idx[p.x], b = d.prim_idx[p.y];
+            p = a < b ? make_int2(a, b) : make_int2(b, a);
+        }
+        dst[k] = p;
+    }
+}
+void launch_pair_gather(DevState& d, int kind, int2* dst, int64_t cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_pair_gather, dim3(2048), dim3(kBlock), 0, s, d, kind, dst, (uint64_t)cap);
 }
 
 // ---------------------------------------------------------------- pair-set digest
@@ -879,12 +921,17 @@ __device__ inline uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
-__global__ void __launch_bounds__(kBlock) k_digest(const int2* __restrict__ pairs, const uint32_t* __restrict__ cnt,
-                                                   uint64_t cap, uint64_t* out) {
-    const uint32_t n = (uint32_t)min((uint64_t)*cnt, cap);
+__global__ void __launch_bounds__(kBlock) k_digest(DevState d, int kind, uint64_t* out) {
+    __shared__ uint32_t pre[kPairSegs + 1];
+    const uint32_t n = seg_prefix(d.pcnt + kind * kPairSegs * kCntStride, (uint32_t)d.pair_seg_cap, pre);
+    const int2* src = kind ? d.np : d.bp;
     uint64_t acc = 0;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int2 p = pairs[k];
+        int2 p = src[seg_slot(pre, k, d.pair_seg_cap)];
+        if (kind == 0) {
+            const int32_t a = d.prim_idx[p.x], b = d.prim_idx[p.y];
+            p = a < b ? make_int2(a, b) : make_int2(b, a);
+        }
         acc += mix64(((uint64_t)(uint32_t)p.x << 32) | (uint32_t)p.y);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
@@ -893,9 +940,8 @@ __global__ void __launch_bounds__(kBlock) k_digest(const int2* __restrict__ pair
 
 void launch_digest(DevState& d, int filtered, hipStream_t s) {
     uint64_t* out = ctr64(d.ctr) + C64_DIGEST;
-    hipMemsetAsync(out, 0, sizeof(uint64_t), s);
-    hipLaunchKernelGGL(k_digest, dim3(1024), dim3(kBlock), 0, s, filtered ? d.np : d.bp,
-                       d.ctr + (filtered ? C_NP : C_BP), (uint64_t)d.pair_cap, out);
+    fill_u32(reinterpret_cast<uint32_t*>(out), 2, 0u, s);
+    hipLaunchKernelGGL(k_digest, dim3(1024), dim3(kBlock), 0, s, d, filtered ? 1 : 0, out);
 }
 
 }  // namespace bvhcd

@@ -46,8 +46,10 @@ void launch_prim_map(DevState& d, int32_t n, hipStream_t s);
 
 // front.hip
 void launch_pure_cd(DevState& d, int32_t n, hipStream_t s);
-void launch_generate(DevState& d, int32_t n, hipStream_t s);          // traversal + ordering
-void launch_update(DevState& d, int32_t n, int quality, hipStream_t s);// prune/sprout, quality, ordering
+// order = false keeps the emission order (entries of neighbouring queries together) and skips
+// the node-id sort and bucket offsets that only the front-quality heuristic reads.
+void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order = true);  // traversal + ordering
+void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool order = true);
 void launch_keep(DevState& d, int32_t n, hipStream_t s);
 void launch_reorder(DevState& d, int32_t n, hipStream_t s);
 // Incremental front restructure: classify the current fronts against the runs in rt_of_leaf
@@ -58,8 +60,12 @@ void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream
 void launch_order_fronts(DevState& d, int32_t n, hipStream_t s);      // radix sort by node + bucket starts
 void launch_snapshot(DevState& d, int32_t n, hipStream_t s);
 void launch_narrow(DevState& d, hipStream_t s);
-// Broad-phase pairs are kept as leaf ids; converts d.bp in place to (lower, higher) triangle ids.
-void launch_bp_to_tris(DevState& d, hipStream_t s);
+// Segmented pair lists: totals (and largest segments) into ctr[C_BP / C_NP / *_SEGMAX]; clearing
+// the narrow-phase list; a contiguous copy of list `kind` (0 broad phase, mapped from leaf to
+// triangle ids; 1 narrow phase) of at most cap pairs into dst.
+void launch_pair_totals(DevState& d, hipStream_t s);
+void clear_np(DevState& d, hipStream_t s);
+void launch_pair_gather(DevState& d, int kind, int2* dst, int64_t cap, hipStream_t s);
 void launch_digest(DevState& d, int filtered, hipStream_t s);
 void launch_restr_decision(DevState& d, int32_t n, hipStream_t s);    // counts into ctr[C_RT_*]
 
