@@ -53,16 +53,25 @@ struct FrontSink {  // SoA front buffer
         query[o] = v.x;
     }
 };
-struct PairSink {  // int2 pair list
+struct PairSink {  // int2 list with one counter
     uint32_t* count;
     int2* out;
     uint32_t cap;
     uint32_t ovf_bit;
     __device__ void write(uint32_t o, int2 v) const { out[o] = v; }
 };
+struct SegPairSink {  // segmented pair list: kPairSegs counters kCntStride words apart
+    uint32_t* cnt;
+    int2* base;
+    int64_t seg_cap;
+    uint32_t ovf_bit;
+};
 
+// The staging word pair of a wave and list: sfill[0] = staged entries, sfill[1] = rotor that
+// picks the segment of the next flush of a segmented list.
 template <class Sink>
-__device__ void stage_flush(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, uint32_t f, uint64_t active) {
+__device__ void stage_flush(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t*, uint32_t f,
+                            uint64_t active) {
     const int lane = lane_id();
     const int leader = __ffsll((long long)active) - 1;
     uint32_t base = 0;
@@ -79,6 +88,32 @@ __device__ void stage_flush(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, 
     }
     if (over) atomicOr(ovf, sk.ovf_bit);
 }
+// Segmented list: successive flushes of a wave rotate over the segments (the rotor starts at
+// the wave's global index), so the appends spread over kPairSegs counters and every segment
+// fills evenly whichever workgroups produce the entries.
+__device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
+                                   uint32_t f, uint64_t active) {
+    const int lane = lane_id();
+    const int leader = __ffsll((long long)active) - 1;
+    uint32_t base = 0, g = 0;
+    if (lane == leader) {
+        g = sfill[1] % kPairSegs;
+        sfill[1] = sfill[1] + 1;
+        base = atomicAdd(sk.cnt + g * kCntStride, f);
+    }
+    base = __shfl(base, leader);
+    g = __shfl(g, leader);
+    const uint32_t rank = (uint32_t)__popcll(active & lanes_below());
+    const uint32_t nact = (uint32_t)__popcll(active);
+    int2* out = sk.base + (int64_t)g * sk.seg_cap;
+    bool over = false;
+    for (uint32_t i = rank; i < f; i += nact) {
+        const uint32_t o = base + i;
+        if (o < sk.seg_cap) out[o] = make_int2(sbuf[i].x, sbuf[i].y);
+        else over = true;
+    }
+    if (over) atomicOr(ovf, sk.ovf_bit);
+}
 
 // Called by the currently active lanes of a wave; `pred` selects the lanes that emit `v`.
 template <class Sink>
@@ -90,7 +125,7 @@ __device__ inline void stage_push(const Sink& sk, uint32_t* ovf, volatile int2* 
     uint32_t f = *sfill;
     const uint32_t cnt = (uint32_t)__popcll(m);
     if (f + cnt > cap) {
-        stage_flush(sk, ovf, sbuf, f, active);
+        stage_flush(sk, ovf, sbuf, sfill, f, active);
         f = 0;
     }
     if (pred) {
@@ -104,23 +139,27 @@ __device__ inline void stage_push(const Sink& sk, uint32_t* ovf, volatile int2* 
 template <class Sink>
 __device__ inline void stage_final(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill) {
     const uint32_t f = *sfill;
-    if (f) stage_flush(sk, ovf, sbuf, f, __ballot(1));
+    if (f) stage_flush(sk, ovf, sbuf, sfill, f, __ballot(1));
     if (lane_id() == 0) *sfill = 0;
 }
 
-// Per-block staging storage for up to three lists.
+// Per-block staging storage for up to three lists; fill[list][wave] = {staged, rotor}.
 struct Stages {
     int2 buf[3][kWaves][kStage3];
-    uint32_t fill[3][kWaves];
+    uint32_t fill[3][kWaves][2];
 };
 __device__ inline void stages_init(Stages& st) {
-    if (lane_id() < 3) st.fill[lane_id()][threadIdx.x >> 6] = 0;
+    const int w = threadIdx.x >> 6;
+    if (lane_id() < 3) {
+        st.fill[lane_id()][w][0] = 0;
+        st.fill[lane_id()][w][1] = blockIdx.x * kWaves + w;
+    }
 }
 
 // Output context of a traversal thread.
 struct Out {
     FrontSink fi, fe;
-    PairSink bp;
+    SegPairSink bp;
     uint32_t* ovf;
     volatile int2* sb[3];
     volatile uint32_t* sf[3];
@@ -143,33 +182,35 @@ struct Out {
     }
 };
 
-// Segment of a pair list that this workgroup appends to (kind 0 broad phase, 1 narrow phase).
-__device__ inline PairSink pair_segment(const DevState& d, int kind, uint32_t ovf_bit) {
-    const int g = blockIdx.x % kPairSegs;
-    return PairSink{d.pcnt + (kind * kPairSegs + g) * kCntStride, (kind ? d.np : d.bp) + (int64_t)g * d.pair_seg_cap,
-                    (uint32_t)min(d.pair_seg_cap, (int64_t)0xFFFFFFFF), ovf_bit};
+// Segmented pair list of kind 0 (broad phase) or 1 (narrow phase).
+__device__ inline SegPairSink pair_segments(const DevState& d, int kind, uint32_t ovf_bit) {
+    return SegPairSink{d.pcnt + kind * kPairSegs * kCntStride, kind ? d.np : d.bp, d.pair_seg_cap, ovf_bit};
 }
 
 // Pair-only kernels (KEEP, pure traversal) stage one list: a third of the LDS, so 8 waves per
 // SIMD instead of 6 hide more of the dependent node loads.
 struct StagesBP {
     int2 buf[kWaves][kStage];
-    uint32_t fill[kWaves];
+    uint32_t fill[kWaves][2];
 };
 __device__ inline void stages_init(StagesBP& st) {
-    if (lane_id() == 0) st.fill[threadIdx.x >> 6] = 0;
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        st.fill[w][0] = 0;
+        st.fill[w][1] = blockIdx.x * kWaves + w;
+    }
 }
 __device__ inline Out make_out(StagesBP& st, const DevState& d) {
     Out o;
     o.fi = FrontSink{nullptr, nullptr, nullptr, 0u, 0u};
     o.fe = o.fi;
-    o.bp = pair_segment(d, 0, 4u);
+    o.bp = pair_segments(d, 0, 4u);
     o.ovf = &d.ctr[C_OVF];
     const int w = threadIdx.x >> 6;
     o.sb[0] = o.sb[1] = nullptr;
     o.sf[0] = o.sf[1] = nullptr;
     o.sb[2] = st.buf[w];
-    o.sf[2] = &st.fill[w];
+    o.sf[2] = st.fill[w];
     o.scap = kStage;
     return o;
 }
@@ -178,12 +219,12 @@ __device__ inline Out make_out(Stages& st, const FrontSink& fi, const FrontSink&
     Out o;
     o.fi = fi;
     o.fe = fe;
-    o.bp = pair_segment(d, 0, 4u);
+    o.bp = pair_segments(d, 0, 4u);
     o.ovf = &d.ctr[C_OVF];
     const int w = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
         o.sb[k] = st.buf[k][w];
-        o.sf[k] = &st.fill[k][w];
+        o.sf[k] = st.fill[k][w];
     }
     o.scap = kStage3;
     return o;
@@ -833,11 +874,14 @@ __device__ bool sat_tri_tri(float3 P1, float3 P2, float3 P3, float3 Q1, float3 Q
 
 __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
     __shared__ int2 sbuf[kWaves][kStage];
-    __shared__ uint32_t sfill[kWaves];
+    __shared__ uint32_t sfill[kWaves][2];
     __shared__ uint32_t pre[kPairSegs + 1];
     const int w = threadIdx.x >> 6;
-    if (lane_id() == 0) sfill[w] = 0;
-    const PairSink sink = pair_segment(d, 1, 8u);
+    if (lane_id() == 0) {
+        sfill[w][0] = 0;
+        sfill[w][1] = blockIdx.x * kWaves + w;
+    }
+    const SegPairSink sink = pair_segments(d, 1, 8u);
     const uint32_t npairs = seg_prefix(d.pcnt, (uint32_t)d.pair_seg_cap, pre);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += gridDim.x * blockDim.x) {
         const int2 lv = d.bp[seg_slot(pre, k, d.pair_seg_cap)];  // leaves
@@ -863,9 +907,9 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
             hit = sat_tri_tri(ld3(d.verts, a0), ld3(d.verts, a1), ld3(d.verts, a2), ld3(d.verts, b0),
                               ld3(d.verts, b1), ld3(d.verts, b2));
         }
-        stage_push(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w], kStage, hit, pr);
+        stage_push(sink, &d.ctr[C_OVF], sbuf[w], sfill[w], kStage, hit, pr);
     }
-    stage_final(sink, &d.ctr[C_OVF], sbuf[w], &sfill[w]);
+    stage_final(sink, &d.ctr[C_OVF], sbuf[w], sfill[w]);
 }
 
 void launch_narrow(DevState& d, hipStream_t s) {
