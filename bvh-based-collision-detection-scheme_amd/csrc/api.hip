@@ -169,8 +169,9 @@ int allocate(bvhcd_ctx* c) {
         if (c->cfg.ext_front_cap <= 0) ce = (int64_t)(ce * f);
         if (c->cfg.pair_cap <= 0) cp = (int64_t)(cp * f);
     }
-    d.front_cap[0] = std::max<int64_t>(ci, 1);
-    d.front_cap[1] = std::max<int64_t>(ce, 1);
+    // fronts are produced in kPairSegs equal segments, then gathered: capacities are multiples
+    d.front_cap[0] = kPairSegs * ((std::max<int64_t>(ci, 1) + kPairSegs - 1) / kPairSegs);
+    d.front_cap[1] = kPairSegs * ((std::max<int64_t>(ce, 1) + kPairSegs - 1) / kPairSegs);
     // pair lists: kPairSegs equal segments (an explicit capacity is split evenly)
     d.pair_seg_cap = std::max<int64_t>((std::max<int64_t>(cp, 1) + kPairSegs - 1) / kPairSegs, 1);
     d.pair_cap = d.pair_seg_cap * kPairSegs;
@@ -191,6 +192,7 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.np, d.pair_cap);
     ok &= dalloc(c, d.pair_tmp, d.pair_cap);
     ok &= dalloc(c, d.pcnt, 2 * kPairSegs * kCntStride);
+    ok &= dalloc(c, d.fcnt, 2 * kPairSegs * kCntStride);
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
     HIPCHK(c, hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(d.pcnt, 0, 2 * kPairSegs * kCntStride * sizeof(uint32_t), c->stream));
@@ -745,8 +747,8 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     out->ext_front = sz[2 + c->d.fcur[1]];
     out->bp_pairs = ctr[C_BP];
     out->np_pairs = ctr[C_NP];
-    out->required_int_front = out->int_front;
-    out->required_ext_front = out->ext_front;
+    out->required_int_front = std::max<int64_t>(out->int_front, ctr[C_FREQ_I]);
+    out->required_ext_front = std::max<int64_t>(out->ext_front, ctr[C_FREQ_E]);
     // pair lists are kPairSegs equal segments: what fits every segment
     out->required_pairs = std::max<int64_t>((int64_t)ctr[C_BP],
                                             (int64_t)std::max(ctr[C_BP_SEGMAX], ctr[C_NP_SEGMAX]) * kPairSegs);

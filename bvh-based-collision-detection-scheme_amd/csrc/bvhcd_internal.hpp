@@ -193,6 +193,8 @@ struct DevState {
     int64_t front_cap[2] = {0, 0};  // entries per buffer, per kind
     int fcur[2] = {0, 0};           // current buffer per kind
     uint32_t* fsize = nullptr;      // [kind*2 + buf] sizes (4 words)
+    uint32_t* fcnt = nullptr;       // [kind][kPairSegs * kCntStride] segment counters of a front
+                                    // being produced (front_cap is kPairSegs equal segments)
     uint32_t* intcnt = nullptr;     // T
     uint32_t* extcnt = nullptr;     // T+1
     uint32_t* intbeg = nullptr;     // T+1
@@ -243,6 +245,8 @@ enum Counter : int {
     C_KEPT_E = 10,
     C_BP_SEGMAX = 11,  // largest broad-phase segment count requested (capacity reporting)
     C_NP_SEGMAX = 12,
+    C_FREQ_I = 13,     // capacity the last produced internal / external front needed
+    C_FREQ_E = 14,
     C_NUM = 16         // words 16..31 hold 64-bit counters (see Counter64)
 };
 enum Counter64 : int {

@@ -42,16 +42,15 @@ constexpr uint32_t kStage3 = 256;
 __device__ inline uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
 
 // ---------------------------------------------------------------- output sinks
-struct FrontSink {  // SoA front buffer
-    uint32_t* count;
+// Front output (SoA node / query), segmented like the pair lists: kPairSegs counters
+// kCntStride words apart, segment g at [g * seg_cap, g * seg_cap + count). The producing kernel
+// appends segment-wise; front_finish then gathers the segments into the current buffer.
+struct FrontSink {
+    uint32_t* cnt;
     uint32_t* node;
     int32_t* query;
-    uint32_t cap;
+    int64_t seg_cap;
     uint32_t ovf_bit;
-    __device__ void write(uint32_t o, int2 v) const {
-        node[o] = (uint32_t)v.y;
-        query[o] = v.x;
-    }
 };
 struct PairSink {  // int2 list with one counter
     uint32_t* count;
@@ -88,21 +87,26 @@ __device__ void stage_flush(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, 
     }
     if (over) atomicOr(ovf, sk.ovf_bit);
 }
-// Segmented list: successive flushes of a wave rotate over the segments (the rotor starts at
+// Segmented lists: successive flushes of a wave rotate over the segments (the rotor starts at
 // the wave's global index), so the appends spread over kPairSegs counters and every segment
 // fills evenly whichever workgroups produce the entries.
-__device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
-                                   uint32_t f, uint64_t active) {
-    const int lane = lane_id();
+__device__ inline uint32_t seg_reserve(uint32_t* cnt, volatile uint32_t* sfill, uint32_t f, uint64_t active,
+                                       uint32_t& g) {
     const int leader = __ffsll((long long)active) - 1;
-    uint32_t base = 0, g = 0;
-    if (lane == leader) {
+    uint32_t base = 0;
+    g = 0;
+    if (lane_id() == leader) {
         g = sfill[1] % kPairSegs;
         sfill[1] = sfill[1] + 1;
-        base = atomicAdd(sk.cnt + g * kCntStride, f);
+        base = atomicAdd(cnt + g * kCntStride, f);
     }
-    base = __shfl(base, leader);
     g = __shfl(g, leader);
+    return __shfl(base, leader);
+}
+__device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
+                                   uint32_t f, uint64_t active) {
+    uint32_t g;
+    const uint32_t base = seg_reserve(sk.cnt, sfill, f, active, g);
     const uint32_t rank = (uint32_t)__popcll(active & lanes_below());
     const uint32_t nact = (uint32_t)__popcll(active);
     int2* out = sk.base + (int64_t)g * sk.seg_cap;
@@ -111,6 +115,25 @@ __device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, volatil
         const uint32_t o = base + i;
         if (o < sk.seg_cap) out[o] = make_int2(sbuf[i].x, sbuf[i].y);
         else over = true;
+    }
+    if (over) atomicOr(ovf, sk.ovf_bit);
+}
+__device__ inline void stage_flush(const FrontSink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
+                                   uint32_t f, uint64_t active) {
+    uint32_t g;
+    const uint32_t base = seg_reserve(sk.cnt, sfill, f, active, g);
+    const uint32_t rank = (uint32_t)__popcll(active & lanes_below());
+    const uint32_t nact = (uint32_t)__popcll(active);
+    const int64_t seg0 = (int64_t)g * sk.seg_cap;
+    bool over = false;
+    for (uint32_t i = rank; i < f; i += nact) {
+        const uint32_t o = base + i;
+        if (o < sk.seg_cap) {
+            sk.node[seg0 + o] = (uint32_t)sbuf[i].y;
+            sk.query[seg0 + o] = sbuf[i].x;
+        } else {
+            over = true;
+        }
     }
     if (over) atomicOr(ovf, sk.ovf_bit);
 }
@@ -306,10 +329,58 @@ struct FrontIn {
     uint32_t cap[2];
 };
 
+// Segmented output into the spare buffer of `kind` (its counters cleared by front_begin).
 static FrontSink next_sink(DevState& d, int kind) {
     const int nb = d.fcur[kind] ^ 1;
-    return FrontSink{d.fsize + 2 * kind + nb, d.fnode[kind][nb], d.fquery[kind][nb],
-                     (uint32_t)std::min<int64_t>(d.front_cap[kind], 0xFFFFFFFFll), kind ? 2u : 1u};
+    return FrontSink{d.fcnt + kind * kPairSegs * kCntStride, d.fnode[kind][nb], d.fquery[kind][nb],
+                     d.front_cap[kind] / kPairSegs, kind ? 2u : 1u};
+}
+static void front_begin(DevState& d, hipStream_t s) { fill_u32(d.fcnt, 2 * kPairSegs * kCntStride, 0u, s); }
+
+// Gathers the segments of the spare buffer into the current buffer of `kind`, after its first
+// `base` entries (base read from *base_word before any block writes, 0 when null); the current
+// buffer's size becomes base + total, clamped to the capacity (overflow bit otherwise).
+__global__ void __launch_bounds__(kBlock) k_front_compact(DevState d, int kind, const uint32_t* __restrict__ src_node,
+                                                          const int32_t* __restrict__ src_query,
+                                                          uint32_t* __restrict__ dst_node,
+                                                          int32_t* __restrict__ dst_query, const uint32_t* base_word,
+                                                          uint32_t* dst_size) {
+    __shared__ uint32_t pre[kPairSegs + 1];
+    const int64_t seg_cap = d.front_cap[kind] / kPairSegs;
+    const uint32_t total = seg_prefix(d.fcnt + kind * kPairSegs * kCntStride, (uint32_t)seg_cap, pre);
+    const uint32_t base = base_word ? *base_word : 0u;
+    const uint32_t cap = (uint32_t)min(d.front_cap[kind], (int64_t)0xFFFFFFFF);
+    const uint32_t room = base < cap ? cap - base : 0u;
+    const uint32_t m = min(total, room);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const int64_t src = seg_slot(pre, k, seg_cap);
+        dst_node[base + k] = src_node[src];
+        dst_query[base + k] = src_query[src];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *dst_size = base + m;
+        if (m < total) atomicOr(&d.ctr[C_OVF], kind ? 2u : 1u);
+        // capacity that would have held every requested entry (segments fill unevenly)
+        uint32_t sum = 0, mx = 0;
+        for (int g = 0; g < kPairSegs; ++g) {
+            const uint32_t v = d.fcnt[(kind * kPairSegs + g) * kCntStride];
+            sum += v;
+            mx = max(mx, v);
+        }
+        d.ctr[kind ? C_FREQ_E : C_FREQ_I] = max(base + sum, mx * (uint32_t)kPairSegs);
+    }
+}
+// append: keep the current buffer's entries and add the new ones after them.
+static void front_finish(DevState& d, int kind, bool append, hipStream_t s) {
+    const int c = d.fcur[kind], nb = c ^ 1;
+    uint32_t* size = d.fsize + 2 * kind + c;
+    uint32_t* base_word = nullptr;
+    if (append) {  // the base is snapshotted: block 0 rewrites *size while others still read
+        base_word = d.ctr + (kind ? C_KEPT_E : C_KEPT_I);
+        hipMemcpyAsync(base_word, size, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    }
+    hipLaunchKernelGGL(k_front_compact, dim3(2048), dim3(kBlock), 0, s, d, kind, d.fnode[kind][nb],
+                       d.fquery[kind][nb], d.fnode[kind][c], d.fquery[kind][c], base_word, size);
 }
 static FrontIn cur_front(DevState& d) {
     FrontIn f;
@@ -490,12 +561,11 @@ void launch_order_fronts(DevState& d, int32_t n, hipStream_t s) {
 
 void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order) {
     reset_step_counters(d, s);
+    front_begin(d, s);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
-    hipMemsetAsync(fi.count, 0, sizeof(uint32_t), s);
-    hipMemsetAsync(fe.count, 0, sizeof(uint32_t), s);
     if (n >= 2) hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, fi, fe);
-    d.fcur[0] ^= 1;
-    d.fcur[1] ^= 1;
+    front_finish(d, 0, false, s);
+    front_finish(d, 1, false, s);
     if (order && n >= 2) launch_order_fronts(d, n, s);
 }
 
@@ -598,13 +668,12 @@ void launch_snapshot(DevState& d, int32_t n, hipStream_t s) {
 void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool order) {
     if (!order) quality = 0;  // the quality check reads the ordering's bucket offsets
     reset_step_counters(d, s);
+    front_begin(d, s);
     const FrontIn in = cur_front(d);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
-    hipMemsetAsync(fi.count, 0, sizeof(uint32_t), s);
-    hipMemsetAsync(fe.count, 0, sizeof(uint32_t), s);
     hipLaunchKernelGGL(k_update, dim3(stream_grid()), dim3(kBlock), 0, s, d, in, fi, fe);
-    d.fcur[0] ^= 1;
-    d.fcur[1] ^= 1;
+    front_finish(d, 0, false, s);  // into the input buffers, which k_update is done with
+    front_finish(d, 1, false, s);
     if (quality && n >= 2) {
         hipMemsetAsync(d.intrange, 0, sizeof(int32_t) * n, s);
         fill_i32(d.rtroot, n, INT32_MAX, s);
@@ -695,17 +764,16 @@ __global__ void __launch_bounds__(kBlock) k_restr_classify(DevState d, FrontIn i
 }
 
 void launch_front_restr_prepare(DevState& d, int32_t n, hipStream_t s) {
+    front_begin(d, s);
     const FrontIn in = cur_front(d);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
-    hipMemsetAsync(fi.count, 0, sizeof(uint32_t), s);
-    hipMemsetAsync(fe.count, 0, sizeof(uint32_t), s);
     hipMemsetAsync(d.ctr + C_TASKS, 0, sizeof(uint32_t), s);
     // walks beyond the capacity are counted, not written (ovf bit 0): the caller compares
     // the count with the capacity and regenerates the front instead
     const PairSink tasks{d.ctr + C_TASKS, d.rtasks, (uint32_t)std::min<int64_t>(d.rtask_cap, 0xFFFFFFFFll), 0u};
     if (n >= 2) hipLaunchKernelGGL(k_restr_classify, dim3(stream_grid()), dim3(kBlock), 0, s, d, in, fi, fe, tasks);
-    d.fcur[0] ^= 1;
-    d.fcur[1] ^= 1;
+    front_finish(d, 0, false, s);
+    front_finish(d, 1, false, s);
 }
 
 // Walks of the repaired front: (query, rebuilt root) walks first, then every query inside a
@@ -750,16 +818,12 @@ void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream
     kept.size[0] = d.ctr + C_KEPT_I;
     kept.size[1] = d.ctr + C_KEPT_E;
     hipLaunchKernelGGL(k_keep, dim3(stream_grid()), dim3(kBlock), 0, s, d, kept);
-    // the walks append to the current buffers
+    // the walks write segments into the spare buffers, then append after the kept entries
+    front_begin(d, s);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
-    for (int k = 0; k < 2; ++k) {
-        FrontSink& f = k ? fe : fi;
-        const int b = d.fcur[k];
-        f.count = d.fsize + 2 * k + b;
-        f.node = d.fnode[k][b];
-        f.query = d.fquery[k][b];
-    }
     hipLaunchKernelGGL(k_restr_walk, dim3(stream_grid()), dim3(kBlock), 0, s, d, ntasks, n, fi, fe);
+    front_finish(d, 0, true, s);
+    front_finish(d, 1, true, s);
 }
 
 // ---------------------------------------------------------------- REORDER (balance)

@@ -436,8 +436,8 @@ __global__ void __launch_bounds__(kBlock) k_count_front(DevState d, const int32_
         const bool ext = k >= ni;
         const int32_t leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
         const int32_t t = tl[leaf];  // -1: a leaf hanging directly below a node too big to be a candidate
-        wave_count(tcount, t, t >= 0);
-        wave_count(lcnt, leaf, ext);
+        wave_count(tcount, t, t >= 0);  // few distinct candidates per wave
+        if (ext) atomicAdd(&lcnt[leaf], 1u);  // leaves of one wave are mostly distinct
     }
 }
 
