@@ -217,6 +217,9 @@ def main():
                     help="trained scheduler (params + threshold) written by train_scheduler.py")
     ap.add_argument("--no-verify", action="store_true", help="skip the per-frame pair-set comparison pass")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--host-upload", action="store_true",
+                    help="also time the reported schedule with every frame's vertices copied from host memory "
+                         "(PCIe-inclusive rate, reported beside value, never as value)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with three 1 GiB device-to-device copies (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -265,6 +268,23 @@ def main():
     value = steps / rep["wall_s"]
     roof, stages = roofline(rep)
 
+    host_fps = None
+    if args.host_upload:  # same frames, vertices handed over from host memory every frame
+        host_frames = [frames_dev[f].cpu().numpy() for f in range(warmup + steps)]
+        ctx = make_ctx(pkg, sc, faces, pkg.SCHED_LEARNED if learned is not None else pkg.SCHED_PLAIN, weights)
+        for f in range(warmup):
+            ctx.update_vertices(host_frames[f])
+            ctx.step_scheduled()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(warmup, warmup + steps):
+            ctx.update_vertices(host_frames[f])
+            ctx.step_scheduled()
+        torch.cuda.synchronize()
+        host_fps = round(steps / (time.perf_counter() - t0), 3)
+        ctx.close()
+        log(f"host-upload (PCIe-inclusive) rate: {host_fps} frames/s")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -296,6 +316,7 @@ def main():
         "learned_fps": round(steps / learned["wall_s"], 3) if learned is not None else None,
         "learned_vs_plain": round((steps / learned["wall_s"]) / plain_fps, 4) if learned is not None else None,
         "learned_pairs_identical_all_frames": identical,
+        "host_upload_fps": host_fps,
         "learned_restructure_frames": learned["restructured"] if learned is not None else None,
         "pairs_last_frame": rep["last_digest"][1],
         "roofline": roof,

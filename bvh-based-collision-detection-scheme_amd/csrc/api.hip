@@ -422,6 +422,7 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     // better locality than node order).
     const bool order = c->cfg.scheduler != BVHCD_SCHED_LEARNED;
     const bool quality = c->cfg.enable_restructure != 0 && order;
+    if (front_op != BVHCD_FRONT_KEEP && front_op != BVHCD_FRONT_PURE_BVH_CD) d.front_version++;
     switch (front_op) {
         case BVHCD_FRONT_PURE_BVH_CD:
             begin(c, S_FRONT);
@@ -592,6 +593,7 @@ int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V,
     c->T = T;
     c->V = V;
     c->built = c->front_valid = c->restr_pending = c->learned_frame = c->repair_pending = false;
+    c->d.front_version++;
     c->frame = 0;
     c->d.fcur[0] = c->d.fcur[1] = 0;
     HIPCHK(c, hipMemsetAsync(c->d.fsize, 0, 4 * sizeof(uint32_t), c->stream));

@@ -192,6 +192,7 @@ struct DevState {
     int32_t* fquery[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int64_t front_cap[2] = {0, 0};  // entries per buffer, per kind
     int fcur[2] = {0, 0};           // current buffer per kind
+    int64_t front_version = 0;      // host-side: bumped whenever a step changes the front's contents
     uint32_t* fsize = nullptr;      // [kind*2 + buf] sizes (4 words)
     uint32_t* fcnt = nullptr;       // [kind][kPairSegs * kCntStride] segment counters of a front
                                     // being produced (front_cap is kPairSegs equal segments)

@@ -65,9 +65,9 @@ struct LearnedState {
     uint32_t* flags = nullptr;
     uint32_t* pos = nullptr;
     uint32_t* total = nullptr;
-    int32_t* tl = nullptr;        // candidate index of every leaf
-    uint32_t* tcount = nullptr;   // front entries per candidate
+    uint32_t* acnt = nullptr;     // front entries per leaf (by the node's leftmost leaf)
     uint32_t* lcnt = nullptr;     // external-front entries per leaf
+    int64_t counted_version = -1; // front version the per-leaf counts belong to
     int64_t node_cap = 0;
     float* feats = nullptr;
     float threshold = 0.0f;
@@ -101,7 +101,7 @@ void learned_destroy(LearnedState* L) {
     if (!L) return;
     for (void* p : {(void*)L->params, (void*)L->wbf, (void*)L->grad, (void*)L->m, (void*)L->v, (void*)L->loss,
                     (void*)L->x, (void*)L->y, (void*)L->t, (void*)L->roots, (void*)L->flags, (void*)L->pos,
-                    (void*)L->total, (void*)L->feats, (void*)L->tl, (void*)L->tcount, (void*)L->lcnt})
+                    (void*)L->total, (void*)L->feats, (void*)L->acnt, (void*)L->lcnt})
         if (p) (void)hipFree(p);
     delete L;
 }
@@ -406,28 +406,11 @@ __global__ void k_treelet_compact(int32_t I, const uint32_t* __restrict__ flags,
     if (i < I && flags[i]) roots[pos[i]] = i;
 }
 
-__global__ void k_treelet_of_leaf(const int32_t* __restrict__ roots, DevState d, int32_t* __restrict__ tl) {
-    const int32_t r = roots[blockIdx.x];
-    for (int32_t j = d.int_rx[r] + (int32_t)threadIdx.x; j <= d.int_ry[r]; j += blockDim.x) tl[j] = (int32_t)blockIdx.x;
-}
-
-// Adds one per lane with `pred` to counter[key], one atomic per group of equal keys in the wave
-// (front entries are ordered by node, so neighbouring lanes mostly share a key).
-__device__ inline void wave_count(uint32_t* counter, int32_t key, bool pred) {
-    uint64_t todo = __ballot(pred);
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int32_t k = __shfl(key, leader);
-        const uint64_t grp = __ballot(pred && key == k);
-        if (lane_id() == leader) atomicAdd(&counter[k], (uint32_t)__popcll(grp));
-        todo &= ~grp;
-    }
-}
-
-// Current front entries per candidate subtree (an entry belongs to the subtree holding its
-// node's leftmost leaf) and external entries per leaf.
-__global__ void __launch_bounds__(kBlock) k_count_front(DevState d, const int32_t* __restrict__ tl,
-                                                        uint32_t* __restrict__ tcount, uint32_t* __restrict__ lcnt) {
+// Front entries per leaf: every entry counts for its node's leftmost leaf (acnt), external
+// entries also for their leaf (lcnt). Leaves of neighbouring entries are mostly distinct, so
+// plain per-lane atomics over 2 T addresses; recomputed only when the front changed.
+__global__ void __launch_bounds__(kBlock) k_count_front(DevState d, uint32_t* __restrict__ acnt,
+                                                        uint32_t* __restrict__ lcnt) {
     const int c0 = d.fcur[0], c1 = d.fcur[1];
     const uint32_t ni = (uint32_t)min((int64_t)d.fsize[c0], d.front_cap[0]);
     const uint32_t ne = (uint32_t)min((int64_t)d.fsize[2 + c1], d.front_cap[1]);
@@ -435,23 +418,22 @@ __global__ void __launch_bounds__(kBlock) k_count_front(DevState d, const int32_
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
         const bool ext = k >= ni;
         const int32_t leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
-        const int32_t t = tl[leaf];  // -1: a leaf hanging directly below a node too big to be a candidate
-        wave_count(tcount, t, t >= 0);  // few distinct candidates per wave
-        if (ext) atomicAdd(&lcnt[leaf], 1u);  // leaves of one wave are mostly distinct
+        atomicAdd(&acnt[leaf], 1u);
+        if (ext) atomicAdd(&lcnt[leaf], 1u);
     }
 }
 
 // One workgroup per treelet.
 __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__ roots, DevState d, int32_t frame,
-                                                     float frame_scale, const uint32_t* __restrict__ tcount,
+                                                     float frame_scale, const uint32_t* __restrict__ acnt,
                                                      const uint32_t* __restrict__ lcnt, float* __restrict__ feats) {
-    __shared__ float red[4][kBlock];
+    __shared__ float red[5][kBlock];
     __shared__ uint32_t hist[10];
     const int32_t r = roots[blockIdx.x];
     const int32_t a = d.int_rx[r], b = d.int_ry[r], m = b - a + 1;
     if (threadIdx.x < 10) hist[threadIdx.x] = 0;
     __syncthreads();
-    float sa_now = 0.f, sa_ref = 0.f, leaf_exp = 0.f, age = 0.f;
+    float sa_now = 0.f, sa_ref = 0.f, leaf_exp = 0.f, age = 0.f, entries = 0.f;
     for (int32_t i = r + threadIdx.x; i < r + m - 1; i += kBlock) {
         sa_now += rec_area(d.inode[i]);
         sa_ref += d.sa_ref[i];
@@ -460,6 +442,7 @@ __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__
         const float now = rec_area(d.leaf[j]), ref = d.sa_ref_leaf[j];
         leaf_exp += logf((now + 1e-12f) / (ref + 1e-12f));
         age += (float)(frame - d.built_frame[j]);
+        entries += (float)acnt[j];
         const uint32_t c = lcnt[j];
         const int bin = c >= 511u ? 9 : 31 - __clz(c + 1u);  // floor(log2(1 + c)), c + 1 in [1, 511]
         atomicAdd(&hist[bin], 1u);
@@ -468,17 +451,18 @@ __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__
     red[1][threadIdx.x] = sa_ref;
     red[2][threadIdx.x] = leaf_exp;
     red[3][threadIdx.x] = age;
+    red[4][threadIdx.x] = entries;
     __syncthreads();
     for (int off = kBlock / 2; off > 0; off >>= 1) {
         if (threadIdx.x < off) {
-            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         float* f = feats + (int64_t)blockIdx.x * kIn;
         f[0] = logf((red[0][0] + 1e-12f) / (red[1][0] + 1e-12f));
-        f[1] = (float)tcount[blockIdx.x] / (float)m;
+        f[1] = red[4][0] / (float)m;
         f[2] = red[2][0] / (float)m;
         int depth = 0;
         for (int32_t p = d.int_par[r]; p >= 0 && depth < 4096; p = d.int_par[p]) ++depth;
@@ -496,12 +480,11 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
     if (I < 1) return BVHCD_OK;
     if (L->node_cap < n) {
         for (void* p : {(void*)L->roots, (void*)L->flags, (void*)L->pos, (void*)L->total, (void*)L->feats,
-                        (void*)L->tl, (void*)L->tcount, (void*)L->lcnt})
+                        (void*)L->acnt, (void*)L->lcnt})
             if (p) (void)hipFree(p);
         L->roots = nullptr; L->flags = nullptr; L->pos = nullptr; L->total = nullptr; L->feats = nullptr;
-        L->tl = nullptr; L->tcount = nullptr; L->lcnt = nullptr;
-        if (hipMalloc(&L->tl, sizeof(int32_t) * n) != hipSuccess ||
-            hipMalloc(&L->tcount, sizeof(uint32_t) * n) != hipSuccess ||
+        L->acnt = nullptr; L->lcnt = nullptr;
+        if (hipMalloc(&L->acnt, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->lcnt, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->roots, sizeof(int32_t) * n) != hipSuccess ||
             hipMalloc(&L->flags, sizeof(uint32_t) * n) != hipSuccess ||
@@ -510,23 +493,24 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
             hipMalloc(&L->feats, sizeof(float) * kIn * (size_t)n) != hipSuccess)
             return BVHCD_E_OOM;
         L->node_cap = n;
+        L->counted_version = -1;
     }
     hipLaunchKernelGGL(k_treelet_flags, dim3(grid_for(I)), dim3(kBlock), 0, s, I, d, L->flags);
     launch_exscan(d, L->flags, L->pos, I, L->total, s);
     hipLaunchKernelGGL(k_treelet_compact, dim3(grid_for(I)), dim3(kBlock), 0, s, I, L->flags, L->pos, L->roots);
+    if (L->counted_version != d.front_version) {  // per-leaf front counts of this front
+        hipMemsetAsync(L->acnt, 0, sizeof(uint32_t) * (size_t)n, s);
+        hipMemsetAsync(L->lcnt, 0, sizeof(uint32_t) * (size_t)n, s);
+        hipLaunchKernelGGL(k_count_front, dim3(4096), dim3(kBlock), 0, s, d, L->acnt, L->lcnt);
+        L->counted_version = d.front_version;
+    }
     uint32_t cnt = 0;
     if (hipMemcpyAsync(&cnt, L->total, sizeof(cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return BVHCD_E_DEVICE;
-    if (cnt > 0) {
-        fill_i32(L->tl, n, -1, s);
-        hipLaunchKernelGGL(k_treelet_of_leaf, dim3(cnt), dim3(kBlock), 0, s, L->roots, d, L->tl);
-        hipMemsetAsync(L->tcount, 0, sizeof(uint32_t) * cnt, s);
-        hipMemsetAsync(L->lcnt, 0, sizeof(uint32_t) * (size_t)n, s);
-        hipLaunchKernelGGL(k_count_front, dim3(2048), dim3(kBlock), 0, s, d, L->tl, L->tcount, L->lcnt);
+    if (cnt > 0)
         hipLaunchKernelGGL(k_features, dim3(cnt), dim3(kBlock), 0, s, L->roots, d, (int32_t)frame, 1.0f / 12.0f,
-                           L->tcount, L->lcnt, L->feats);
-    }
+                           L->acnt, L->lcnt, L->feats);
     *rows = cnt;
     return hipGetLastError() == hipSuccess ? BVHCD_OK : BVHCD_E_DEVICE;
 }

@@ -51,6 +51,7 @@ int restructure_subtrees(DevState& d, int32_t n, hipStream_t s, bool prepare_fro
     hipLaunchKernelGGL(k_rt_of_leaf, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.r_leaf_hi, d.r_leaf_lo, d.rt_of_leaf);
     // the front is classified while the old topology is still in place
     if (prepare_front) launch_front_restr_prepare(d, n, s);
+    d.front_version++;  // the topology under the front changes
     // fresh scene box and codes, stable sort inside each run
     launch_scene_box(d, n, s);
     hipLaunchKernelGGL(k_restr_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, d.r_keys, d.vals_tmp);
