@@ -78,7 +78,10 @@ def frame_digests(pkg, sc, faces, frames_dev, count, scheduler, weights):
 
 
 def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights=None, digests_upto=0,
-            autotune=True):
+            autotune=True, instrument=False):
+    """Warm-up frames, then `steps` timed frames. instrument=False: wall clock only (the rate that
+    is reported); instrument=True: per-stage hipEvents and counters on every frame (each event is
+    a stream barrier, so this pass is slower and is used only for the stage breakdown)."""
     ctx = make_ctx(pkg, sc, faces, scheduler, weights)
     nbytes = sc.num_verts * 12
     base = frames_dev.data_ptr()
@@ -90,8 +93,11 @@ def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights
             digests.append(ctx.digest(True))
     if autotune:  # launch shapes tuned on this workload before the timed frames (process-wide table)
         ctx.autotune()
+        chosen = [ln for ln in pkg.autotune_report().splitlines() if ln.endswith("*")]
+        log("tuned launch shapes: " + "; ".join(chosen))
     torch.cuda.synchronize()
-    ctx.set_timing(steps)
+    if instrument:
+        ctx.set_timing(steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
@@ -100,14 +106,34 @@ def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights
         ctx.step_scheduled()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ms, by, ops = ctx.timing_frames()
+    ms, by, ops = ctx.timing_frames() if instrument else (None, None, None)
     st = ctx.stats()
     last_digest = ctx.digest(True)
     ctx.close()
-    restructured = int((ops[:, 0] == pkg.BVH_RESTRUCTURE).sum())
-    updates = int((ops[:, 1] == pkg.FRONT_UPDATE).sum())
+    restructured = int((ops[:, 0] == pkg.BVH_RESTRUCTURE).sum()) if instrument else None
+    updates = int((ops[:, 1] == pkg.FRONT_UPDATE).sum()) if instrument else None
     return dict(wall_s=t1 - t0, ms=ms, bytes=by, ops=ops, stats=st, digests=digests, last_digest=last_digest,
                 restructured=restructured, updates=updates)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_cloth1024.md")
+
+
+def profiled_traffic(kernel, workload, schedule):
+    """HBM bytes per launch of `kernel` on this workload and schedule, from the committed counter
+    summary (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter reads for
+    a 1 GiB copy); None when the summary holds no row for it."""
+    if workload != "cloth1024" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as fh:
+        for line in fh:
+            cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+            if len(cells) == 6 and cells[0] == kernel and cells[1] == schedule:
+                try:
+                    return int(cells[5].replace(",", ""))
+                except ValueError:
+                    return None
+    return None
 
 
 def roofline(res):
@@ -136,6 +162,23 @@ def roofline(res):
     return {"kernel": name, "stage": STAGES[s], "launches": n, "avg_ms": t / n,
             "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None}, stage_summary
+
+
+def copy_ceiling(torch):
+    """Measured stream-copy ceiling: read + write bytes per second of a 1 GiB device-to-device
+    copy (torch's copy kernel, timed with events on torch's stream)."""
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    del src, dst
+    return round(2 * (1 << 30) / (ms * 1e-3) / 1e9, 1)
 
 
 def cpu_baseline(pkg, sc, faces, budget_s, threads, gpu_digests):
@@ -256,8 +299,7 @@ def main():
         weights = (z["params"], float(z["threshold"]))
         learned = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED, torch, weights=weights,
                           autotune=not args.no_autotune)
-        log(f"learned: {steps / learned['wall_s']:.1f} frames/s "
-            f"({learned['restructured']} restructure frames, {learned['updates']} UPDATE frames)")
+        log(f"learned: {steps / learned['wall_s']:.1f} frames/s")
         if not args.no_verify:  # untimed replay: pair sets of every frame, learned vs plain
             dp = frame_digests(pkg, sc, faces, frames_dev, warmup + steps, pkg.SCHED_PLAIN, None)
             dl = frame_digests(pkg, sc, faces, frames_dev, warmup + steps, pkg.SCHED_LEARNED, weights)
@@ -266,7 +308,19 @@ def main():
 
     rep = learned if learned is not None else plain
     value = steps / rep["wall_s"]
-    roof, stages = roofline(rep)
+    # stage breakdown: the same frames again with per-stage events (launch shapes already tuned)
+    inst = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED if learned is not None
+                   else pkg.SCHED_PLAIN, torch, weights=weights, autotune=False, instrument=True)
+    log(f"instrumented pass: {steps / inst['wall_s']:.1f} frames/s ({inst['restructured']} restructure frames, "
+        f"{inst['updates']} UPDATE frames)")
+    roof, stages = roofline(inst)
+    roof["copy_ceiling_GBps"] = copy_ceiling(torch)
+    traffic = profiled_traffic(roof["kernel"], args.workload, "learned" if learned is not None else "plain")
+    if traffic is not None:  # measured HBM bytes per launch; its rate over the same event time
+        roof["traffic"] = traffic
+        roof["traffic_GBps"] = round(traffic / (roof["avg_ms"] * 1e-3) / 1e9, 1)
+        roof["traffic_frac"] = round(roof["traffic_GBps"] / PEAK_HBM_GBS, 4)
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT)
 
     host_fps = None
     if args.host_upload:  # same frames, vertices handed over from host memory every frame
@@ -317,7 +371,8 @@ def main():
         "learned_vs_plain": round((steps / learned["wall_s"]) / plain_fps, 4) if learned is not None else None,
         "learned_pairs_identical_all_frames": identical,
         "host_upload_fps": host_fps,
-        "learned_restructure_frames": learned["restructured"] if learned is not None else None,
+        "restructure_frames": inst["restructured"],
+        "instrumented_fps": round(steps / inst["wall_s"], 3),
         "pairs_last_frame": rep["last_digest"][1],
         "roofline": roof,
         "stages": stages,

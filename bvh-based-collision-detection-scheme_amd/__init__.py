@@ -49,7 +49,7 @@ class Config(C.Structure):
         ("int_front_cap", C.c_int64), ("ext_front_cap", C.c_int64), ("pair_cap", C.c_int64),
         ("narrow_phase", C.c_int32), ("scheduler", C.c_int32), ("enable_restructure", C.c_int32),
         ("bvh_cycle", C.c_int32), ("front_cycle", C.c_int32), ("device", C.c_int32),
-        ("stream", C.c_void_p),
+        ("stream", C.c_void_p), ("skip_adjacent", C.c_int32),
     ]
 
 
@@ -59,7 +59,7 @@ class Stats(C.Structure):
         ("int_front", C.c_int64), ("ext_front", C.c_int64), ("bp_pairs", C.c_int64), ("np_pairs", C.c_int64),
         ("required_int_front", C.c_int64), ("required_ext_front", C.c_int64), ("required_pairs", C.c_int64),
         ("node_visits", C.c_int64), ("restructured_subtrees", C.c_int64), ("restructured_leaves", C.c_int64),
-        ("stage_ms", C.c_double * 8), ("stage_bytes", C.c_double * 8),
+        ("stage_ms", C.c_double * 8), ("stage_bytes", C.c_double * 8), ("implicit_pairs", C.c_int64),
     ]
 
 
@@ -122,6 +122,8 @@ def lib():
         "bvhcd_autotune": (C.c_int, [P]),
         "bvhcd_batch_autotune": (C.c_int, [P]),
         "bvhcd_autotune_report": (C.c_int, [P, I64]),
+        "bvhcd_autotune_set": (C.c_int, [C.c_int, I64, C.c_uint32, C.c_uint32]),
+        "bvhcd_update_boxes": (C.c_int, [P, P, I64, C.c_int]),
         "bvhcd_scene_create": (C.c_int, [C.c_int, C.POINTER(SceneParams), C.POINTER(P)]),
         "bvhcd_scene_destroy": (None, [P]),
         "bvhcd_scene_num_tris": (I64, [P]),
@@ -238,11 +240,11 @@ class CollisionContext:
 
     def __init__(self, max_tris, max_verts, *, int_front_cap=0, ext_front_cap=0, pair_cap=0,
                  narrow_phase=True, scheduler=SCHED_PLAIN, enable_restructure=True, bvh_cycle=12,
-                 front_cycle=6, device=0, stream=None):
+                 front_cycle=6, device=0, stream=None, skip_adjacent=False):
         L = lib()
         cfg = Config(int(max_tris), int(max_verts), int(int_front_cap), int(ext_front_cap), int(pair_cap),
                      1 if narrow_phase else 0, int(scheduler), 1 if enable_restructure else 0,
-                     int(bvh_cycle), int(front_cycle), int(device), stream)
+                     int(bvh_cycle), int(front_cycle), int(device), stream, 1 if skip_adjacent else 0)
         h = C.c_void_p()
         st = L.bvhcd_create(C.byref(cfg), C.byref(h))
         if st != OK:
@@ -286,6 +288,15 @@ class CollisionContext:
         else:
             self._chk(lib().bvhcd_update_vertices(self._h, C.c_void_p(int(xyz)), self.V, MEM_DEVICE),
                       "update_vertices")
+
+    def update_boxes(self, boxes, where=MEM_HOST):
+        """Box mode: (T, 6) float32 per-triangle boxes (min xyz, max xyz) replace the vertices
+        until the next update_vertices; the narrow phase must be off."""
+        if where == MEM_HOST:
+            b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+            self._chk(lib().bvhcd_update_boxes(self._h, _ptr(b), b.shape[0], MEM_HOST), "update_boxes")
+        else:
+            self._chk(lib().bvhcd_update_boxes(self._h, C.c_void_p(int(boxes)), self.T, MEM_DEVICE), "update_boxes")
 
     def step(self, bvh_op, front_op):
         self._chk(lib().bvhcd_step(self._h, int(bvh_op), int(front_op)), "step")
@@ -412,6 +423,16 @@ class CollisionContext:
     def autotune(self):
         """Tune the grids of KEEP and the narrow phase on the current data (needs a step first)."""
         self._chk(lib().bvhcd_autotune(self._h), "autotune")
+
+
+TUNE_KEEP, TUNE_NARROW, TUNE_BATCH_CD = 0, 1, 2
+
+
+def autotune_set(kernel: int, n: int, grid: int, unroll: int = 1) -> None:
+    """Fix one kernel's launch shape for the size class of n (bvhcd_autotune_set)."""
+    st = lib().bvhcd_autotune_set(int(kernel), int(n), int(grid), int(unroll))
+    if st != OK:
+        raise BvhcdError(st, "autotune_set")
 
 
 def autotune_report() -> str:

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "bvhcd_internal.hpp"
 #include "kernels.hpp"
@@ -51,6 +52,7 @@ struct bvhcd_ctx {
     std::vector<int64_t> trestr;
     std::vector<void*> allocs;
     LearnedState* learned = nullptr;
+    float* boxes = nullptr;      // box mode input (6 floats per triangle), allocated on first use
 };
 
 namespace {
@@ -103,16 +105,22 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.int_mark, T);
     ok &= dalloc(c, d.int_rx, T);
     ok &= dalloc(c, d.int_ry, T);
-    ok &= dalloc(c, d.int_flag, T);
     ok &= dalloc(c, d.itopo, T);
     ok &= dalloc(c, d.tri_finite, T);
     ok &= dalloc(c, d.lfaces, 3 * T);
     ok &= dalloc(c, d.sa_ref, T);
     ok &= dalloc(c, d.sa_ref_leaf, T);
     ok &= dalloc(c, d.built_frame, T);
-    ok &= dalloc(c, d.gpart, 12 * T);
-    ok &= dalloc(c, d.lroot, T);
-    ok &= dalloc(c, d.lbase, T);
+    ok &= dalloc(c, d.rp_cstart, (size_t)refit_chunks((int32_t)T) + 1);
+    ok &= dalloc(c, d.rp_cinfo, (size_t)refit_chunks((int32_t)T));
+    ok &= dalloc(c, d.rp_ent, T);
+    ok &= dalloc(c, d.rp_tent, T);
+    ok &= dalloc(c, d.rp_tlist, T);
+    ok &= dalloc(c, d.rp_tpos, T);
+    ok &= dalloc(c, d.rp_tlev, 1024);
+    ok &= dalloc(c, d.rp_tinfo, 2);
+    ok &= dalloc(c, d.rp_tcount, 1);
+    ok &= dalloc(c, d.rp_cbad, (size_t)refit_chunks((int32_t)T));
     ok &= dalloc(c, d.leaf_par, T);
     ok &= dalloc(c, d.leaf_mark, T);
     ok &= dalloc(c, d.leaf_lca, T + 1);
@@ -193,6 +201,8 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.pair_tmp, d.pair_cap);
     ok &= dalloc(c, d.pcnt, 2 * kPairSegs * kCntStride);
     ok &= dalloc(c, d.fcnt, 2 * kPairSegs * kCntStride);
+    ok &= dalloc(c, d.adj_cnt, (size_t)(d.front_cap[1] / kAdjTile + 2));
+    ok &= dalloc(c, d.adj_off, (size_t)(d.front_cap[1] / kAdjTile + 2));
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
     HIPCHK(c, hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(d.pcnt, 0, 2 * kPairSegs * kCntStride * sizeof(uint32_t), c->stream));
@@ -219,6 +229,8 @@ __global__ void k_log_frame(DevState d, int cur_int, int cur_ext, uint64_t* out)
     out[3] = d.fsize[cur_int];
     out[4] = d.fsize[2 + cur_ext];
     out[5] = d.ctr[C_OVF];
+    out[6] = d.ctr[C_IMPLICIT];
+    out[7] = d.ctr[C_IMPL_NP];
 }
 
 void log_frame(bvhcd_ctx* c, int bvh_op, int front_op) {
@@ -242,9 +254,14 @@ void log_frame(bvhcd_ctx* c, int bvh_op, int front_op) {
 // event brackets include the node-id sort of GENERATE / UPDATE / REORDER (16 bytes per entry
 // and 16 per node over the passes' reads and writes), so its bytes count there; the
 // order+quality stage only times the quality snapshot (16 bytes per node).
+// Implicit shared-vertex pairs (impl of bp, impl_np of np) are neither read as front entries nor
+// written or re-read as pairs, so they carry no bytes.
 void frame_bytes(double T, double V, int bvh_op, int front_op, double visits, double fi0, double fe0, double fi1,
-                 double fe1, double bp, double np, double restr_leaves, bool ordered, double* out) {
+                 double fe1, double bp, double np, double restr_leaves, bool ordered, double impl, double impl_np,
+                 double* out) {
     const double I = T > 1 ? T - 1 : 0;
+    bp -= impl;
+    np -= impl_np;
     for (int i = 0; i < 8; ++i) out[i] = 0;
     if (bvh_op == BVHCD_BVH_BUILD)
         out[S_BUILD] = 24 * V + 16 * T + (4 * T + 16.0 * 4 * T) + (4 * T + 24 * I + 12 * T + 24 * (T + I));
@@ -252,7 +269,7 @@ void frame_bytes(double T, double V, int bvh_op, int front_op, double visits, do
     if (bvh_op == BVHCD_BVH_RESTRUCTURE) out[S_RESTR] = 12 * V + 40 * T + 88 * I + 16.0 * 7 * restr_leaves;
     if (front_op == BVHCD_FRONT_RESTRUCTURE) out[S_RESTR] += 16 * (fi0 + fe0);  // front classification
     const bool whole = front_op == BVHCD_FRONT_PURE_BVH_CD || front_op == BVHCD_FRONT_GENERATE;
-    const double nin = whole ? T : fi0 + fe0;
+    const double nin = whole ? T : fi0 + fe0 - impl;
     const double nout = (front_op == BVHCD_FRONT_KEEP || front_op == BVHCD_FRONT_PURE_BVH_CD) ? 0 : fi1 + fe1;
     out[S_FRONT] = 32 * nin + 36 * visits + 12 * nout + 8 * bp;
     if (ordered && (front_op == BVHCD_FRONT_GENERATE || front_op == BVHCD_FRONT_UPDATE))
@@ -276,7 +293,6 @@ int do_build(bvhcd_ctx* c) {
     launch_prim_map(d, n, s);
     if (n >= 2) {
         launch_emit_full(d, n, s);
-        launch_refit_prep(d, n, s);
     } else {
         // single leaf: no internal nodes; lca = {leaf 0, end}
         const int32_t lca[2] = {1, -1};
@@ -285,6 +301,7 @@ int do_build(bvhcd_ctx* c) {
         HIPCHK(c, hipMemsetAsync(d.leaf_par, 0xff, sizeof(int32_t), s));
         fill_u32(d.leaf_mark, 1, 3u, s);
     }
+    launch_refit_prep(d, n, s);
     return BVHCD_OK;
 }
 
@@ -466,13 +483,15 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     begin(c, S_NARROW);
     if (c->cfg.narrow_phase) launch_narrow(d, s);
     else clear_np(d, s);
-    launch_pair_totals(d, s);
+    launch_pair_totals(d, s, c->cfg.narrow_phase != 0);
     end(c, S_NARROW);
     return BVHCD_OK;
 }
 
 int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     if (c->T <= 0) return fail(c, BVHCD_E_INVALID, "set_topology first");
+    if (c->d.pbox && c->cfg.narrow_phase)
+        return fail(c, BVHCD_E_INVALID, "box mode: the narrow phase needs vertices (disable it or update_vertices)");
     if (bvh_op < 0 || bvh_op > BVHCD_BVH_NONE || front_op < 0 || front_op > BVHCD_FRONT_KEEP)
         return fail(c, BVHCD_E_INVALID, "bad op");
     begin(c, S_TOTAL);
@@ -545,6 +564,7 @@ int bvhcd_create(const bvhcd_config* cfg, bvhcd_ctx** out) {
     bvhcd_ctx* c = new (std::nothrow) bvhcd_ctx();
     if (!c) return BVHCD_E_OOM;
     c->cfg = *cfg;
+    c->d.skip_adj = cfg->skip_adjacent ? 1 : 0;
     if (c->cfg.bvh_cycle <= 0) c->cfg.bvh_cycle = 12;
     if (c->cfg.front_cycle <= 0) c->cfg.front_cycle = 6;
     if (hipSetDevice(cfg->device) != hipSuccess) {
@@ -607,6 +627,24 @@ int bvhcd_update_vertices(bvhcd_ctx* c, const float* xyz, int64_t V, int where) 
     HIPCHK(c, hipMemcpyAsync(c->d.verts, xyz, sizeof(float) * 3 * V,
                              where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
     if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->d.pbox = nullptr;  // vertex mode
+    return BVHCD_OK;
+}
+
+int bvhcd_update_boxes(bvhcd_ctx* c, const float* boxes, int64_t T, int where) {
+    if (!c) return BVHCD_E_INVALID;
+    if (!boxes || T != c->T || T < 1) return fail(c, BVHCD_E_INVALID, "box count differs from set_topology");
+    if (!c->boxes) {
+        void* p = nullptr;
+        if (hipMalloc(&p, sizeof(float) * 6 * (size_t)c->d.maxT) != hipSuccess)
+            return fail(c, BVHCD_E_OOM, "box buffer");
+        c->allocs.push_back(p);
+        c->boxes = static_cast<float*>(p);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->boxes, boxes, sizeof(float) * 6 * T,
+                             where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->d.pbox = c->boxes;
     return BVHCD_OK;
 }
 
@@ -635,12 +673,15 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
         if (c->frame % bc == 0 || (c->cfg.enable_restructure && c->restr_pending)) c->learned_frame = true;
         if (c->frame % fc == 0 && c->last_front == BVHCD_FRONT_KEEP) {
             uint64_t visits = 0;
-            uint32_t sz[4] = {0, 0, 0, 0};
+            uint32_t sz[4] = {0, 0, 0, 0}, implicit = 0;
             HIPCHK(c, hipMemcpyAsync(&visits, ctr64(c->d.ctr) + C64_VISITS, sizeof(visits), hipMemcpyDeviceToHost,
                                      c->stream));
             HIPCHK(c, hipMemcpyAsync(sz, c->d.fsize, sizeof(sz), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(&implicit, c->d.ctr + C_IMPLICIT, sizeof(implicit), hipMemcpyDeviceToHost,
+                                     c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
-            const double entries = (double)sz[c->d.fcur[0]] + (double)sz[2 + c->d.fcur[1]];
+            // entries KEEP tested (the implicit shared-vertex head takes no node test)
+            const double entries = (double)sz[c->d.fcur[0]] + (double)sz[2 + c->d.fcur[1]] - (double)implicit;
             if ((double)visits > kLearnedUpdateRatio * entries + 1024.0) front_op = BVHCD_FRONT_UPDATE;
         }
         return do_step(c, bvh_op, front_op);
@@ -659,13 +700,20 @@ int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int w
     if (out && cap > 0) {
         const int64_t k = std::min<int64_t>(cap, cnt);
         if (k > 0) {
-            // segments gathered (broad phase: leaf ids mapped to triangle ids) straight into a
-            // device buffer, or into the read-out copy for the host
+            // segments, then the implicit shared-vertex pairs, gathered (leaf ids mapped to
+            // triangle ids) straight into a device buffer, or through the read-out buffer to the
+            // host in pieces of its size (implicit pairs are not stored, so the total can exceed it)
             if (where == BVHCD_MEM_DEVICE) {
-                launch_pair_gather(c->d, filtered ? 1 : 0, reinterpret_cast<int2*>(out), k, c->stream);
+                launch_pair_gather(c->d, filtered ? 1 : 0, reinterpret_cast<int2*>(out), 0, k, c->stream);
             } else {
-                launch_pair_gather(c->d, filtered ? 1 : 0, c->d.pair_tmp, k, c->stream);
-                HIPCHK(c, hipMemcpyAsync(out, c->d.pair_tmp, sizeof(int2) * k, hipMemcpyDeviceToHost, c->stream));
+                const int64_t piece = std::max<int64_t>(1, c->d.pair_cap);
+                for (int64_t first = 0; first < k; first += piece) {
+                    const int64_t m = std::min<int64_t>(piece, k - first);
+                    launch_pair_gather(c->d, filtered ? 1 : 0, c->d.pair_tmp, first, m, c->stream);
+                    HIPCHK(c, hipMemcpyAsync(out + 2 * first, c->d.pair_tmp, sizeof(int2) * m, hipMemcpyDeviceToHost,
+                                             c->stream));
+                    HIPCHK(c, hipStreamSynchronize(c->stream));
+                }
             }
         }
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -725,7 +773,7 @@ int bvhcd_timing_frames(bvhcd_ctx* c, int64_t cap, double* ms, double* bytes, in
         if (bytes)
             frame_bytes((double)c->T, (double)c->V, bo, fo, (double)L[0], fi0, fe0, (double)L[3], (double)L[4],
                         (double)L[1], (double)L[2], (double)c->trestr[f], c->cfg.scheduler != BVHCD_SCHED_LEARNED,
-                        bytes + 8 * f);
+                        (double)L[6], (double)L[7], bytes + 8 * f);
         if (ops) { ops[2 * f] = bo; ops[2 * f + 1] = fo; }
         fi0 = (double)L[3];
         fe0 = (double)L[4];
@@ -755,6 +803,7 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     out->required_pairs = std::max<int64_t>((int64_t)ctr[C_BP],
                                             (int64_t)std::max(ctr[C_BP_SEGMAX], ctr[C_NP_SEGMAX]) * kPairSegs);
     out->node_visits = (int64_t)c64[C64_VISITS];
+    out->implicit_pairs = ctr[C_IMPLICIT];
     out->restructured_subtrees = c->restr_sub;
     out->restructured_leaves = c->restr_leaf;
     if (c->tcap > 0 && c->tn > 0) {
@@ -768,7 +817,8 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     }
     frame_bytes((double)c->T, (double)c->V, c->last_bvh, c->last_front, (double)c64[C64_VISITS], 0, 0,
                 (double)out->int_front, (double)out->ext_front, (double)out->bp_pairs, (double)out->np_pairs,
-                (double)c->restr_leaf, c->cfg.scheduler != BVHCD_SCHED_LEARNED, out->stage_bytes);
+                (double)c->restr_leaf, c->cfg.scheduler != BVHCD_SCHED_LEARNED, (double)ctr[C_IMPLICIT],
+                (double)ctr[C_IMPL_NP], out->stage_bytes);
     if (ctr[C_OVF]) {
         if (ctr[C_OVF] & 16u) return fail(c, BVHCD_E_DEVICE, "traversal step bound exceeded (corrupted tree)");
         out->required_int_front = std::max<int64_t>(out->int_front, c->d.front_cap[0] + 1);
@@ -901,12 +951,15 @@ int bvhcd_autotune(bvhcd_ctx* c) {
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    const unsigned cand[] = {512u, 1024u, 2048u, 4096u, 8192u, 16384u};
+    std::vector<unsigned> cand[2];
+    for (unsigned g : {1024u, 2048u, 4096u, 8192u})
+        for (unsigned u : {1u, 2u, 4u}) cand[TK_KEEP].push_back(make_shape(g, u));
+    for (unsigned g : {1024u, 2048u, 4096u, 8192u, 16384u}) cand[TK_NARROW].push_back(g);
     for (int k : {(int)TK_KEEP, (int)TK_NARROW}) {
         float best_ms = 1e30f;
-        unsigned best = tuned_grid(k, k == TK_KEEP ? c->T : d.maxT);
-        for (unsigned g : cand) {
-            set_tuned_grid(k, k == TK_KEEP ? c->T : d.maxT, g);
+        unsigned best = tuned_shape(k, k == TK_KEEP ? c->T : d.maxT);
+        for (unsigned g : cand[k]) {
+            set_tuned_shape(k, k == TK_KEEP ? c->T : d.maxT, g);
             auto run = [&]() {
                 if (k == TK_KEEP) launch_keep(d, n, c->stream);  // rewrites the same pair list
                 else launch_narrow(d, c->stream);
@@ -928,7 +981,7 @@ int bvhcd_autotune(bvhcd_ctx* c) {
                 best = g;
             }
         }
-        set_tuned_grid(k, k == TK_KEEP ? c->T : d.maxT, best);
+        set_tuned_shape(k, k == TK_KEEP ? c->T : d.maxT, best);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -936,8 +989,16 @@ int bvhcd_autotune(bvhcd_ctx* c) {
     launch_keep(d, n, c->stream);
     if (c->cfg.narrow_phase) launch_narrow(d, c->stream);
     else clear_np(d, c->stream);
-    launch_pair_totals(d, c->stream);
+    launch_pair_totals(d, c->stream, c->cfg.narrow_phase != 0);
     return check_overflow(c);
+}
+
+int bvhcd_autotune_set(int kernel, int64_t n, uint32_t grid, uint32_t unroll) {
+    if (kernel < 0 || kernel >= TK_COUNT || n < 1 || grid < 1 || grid >= (1u << 24)) return BVHCD_E_INVALID;
+    const bool ok = unroll == 1 || ((unroll == 2 || unroll == 4) && kernel == TK_KEEP);
+    if (!ok) return BVHCD_E_INVALID;
+    set_tuned_shape(kernel, n, make_shape(grid, unroll));
+    return BVHCD_OK;
 }
 
 int bvhcd_autotune_report(char* buf, int64_t cap) {

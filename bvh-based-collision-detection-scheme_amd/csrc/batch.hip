@@ -253,7 +253,7 @@ void record_frame(bvhcd_batch* b, int bvh_op, unsigned grid) {
     launch_refit(d, n, s);
     launch_batch_cd(d, n, b->keys, b->scene_last, b->queue, grid, s);
     launch_narrow(d, s);
-    launch_pair_totals(d, s);
+    launch_pair_totals(d, s, true);
     fill_u32(b->scnt, b->S + 1, 0u, s);
     fill_u32(b->scur, b->S + 1, 0u, s);
     hipLaunchKernelGGL(k_pair_scene_count, dim3(1024), dim3(kBlock), 0, s, d, b->tri_scene, b->scnt);
@@ -262,7 +262,7 @@ void record_frame(bvhcd_batch* b, int bvh_op, unsigned grid) {
                        b->scur, b->grouped);
 }
 
-unsigned batch_grid(bvhcd_batch* b) { return tuned_grid(TK_BATCH_CD, b->T); }
+unsigned batch_grid(bvhcd_batch* b) { return shape_grid(tuned_shape(TK_BATCH_CD, b->T)); }
 
 int check_batch_overflow(bvhcd_batch* b) {
     uint32_t ovf = 0;
@@ -340,13 +340,19 @@ int bvhcd_batch_create(const bvhcd_config* cfg, int32_t S, const int64_t* tps, c
     ok &= balloc(b, d.int_mark, T);
     ok &= balloc(b, d.int_rx, T);
     ok &= balloc(b, d.int_ry, T);
-    ok &= balloc(b, d.int_flag, T);
     ok &= balloc(b, d.itopo, T);
     ok &= balloc(b, d.tri_finite, T);
     ok &= balloc(b, d.lfaces, 3 * T);
-    ok &= balloc(b, d.gpart, 12 * T);
-    ok &= balloc(b, d.lroot, T);
-    ok &= balloc(b, d.lbase, T);
+    ok &= balloc(b, d.rp_cstart, (size_t)refit_chunks((int32_t)T) + 1);
+    ok &= balloc(b, d.rp_cinfo, (size_t)refit_chunks((int32_t)T));
+    ok &= balloc(b, d.rp_ent, T);
+    ok &= balloc(b, d.rp_tent, T);
+    ok &= balloc(b, d.rp_tlist, T);
+    ok &= balloc(b, d.rp_tpos, T);
+    ok &= balloc(b, d.rp_tlev, 1024);
+    ok &= balloc(b, d.rp_tinfo, 2);
+    ok &= balloc(b, d.rp_tcount, 1);
+    ok &= balloc(b, d.rp_cbad, (size_t)refit_chunks((int32_t)T));
     ok &= balloc(b, d.leaf_par, T);
     ok &= balloc(b, d.leaf_mark, T);
     ok &= balloc(b, d.leaf_lca, T + 1);
@@ -542,7 +548,7 @@ int bvhcd_batch_autotune(bvhcd_batch* b) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    set_tuned_grid(TK_BATCH_CD, b->T, best);
+    set_tuned_shape(TK_BATCH_CD, b->T, best);
     drop_graphs(b);
     // leave the batch with a complete frame result (pairs of the current vertices)
     record_frame(b, BVHCD_BVH_REFIT, best);

@@ -155,17 +155,26 @@ struct DevState {
     uint32_t* int_mark = nullptr;
     int32_t* int_rx = nullptr;
     int32_t* int_ry = nullptr;
-    uint32_t* int_flag = nullptr;   // T-1 refit arrival counters
-    int4* itopo = nullptr;          // T-1 packed (lc, rc, par, mark) for the refit climb
+    int4* itopo = nullptr;          // T-1 packed (lc, rc, par, mark)
+    // refit schedule, rebuilt after every topology change (treelet chunks + top tree)
+    int32_t* rp_cstart = nullptr;   // chunks + 1: first leaf of each chunk, then T
+    int2* rp_cinfo = nullptr;       // chunks: (internal nodes, deepest level)
+    int4* rp_ent = nullptr;         // T: per chunk (node, LDS slot, child slots, depth), deepest first
+    int4* rp_tent = nullptr;        // T: top-tree (node, child, child, depth), unordered
+    int4* rp_tlist = nullptr;       // T: the same, deepest first, top children as list positions
+    int32_t* rp_tpos = nullptr;     // T-1: list position of each top node
+    int32_t* rp_tlev = nullptr;     // 1024: first entry of each top-tree depth
+    int32_t* rp_tinfo = nullptr;    // 2: (top nodes, deepest level or -1)
+    uint32_t* rp_tcount = nullptr;  // 1: append counter
+    uint32_t* rp_cbad = nullptr;    // chunks: some leaf of the chunk has a non-finite coordinate
     float* sa_ref = nullptr;        // T-1 internal-node surface area at its last rebuild
     float* sa_ref_leaf = nullptr;   // T leaf-box surface area at its last rebuild
     int32_t* built_frame = nullptr; // T frame at which each leaf's subtree was last rebuilt
     uint8_t* tri_finite = nullptr;  // T (by leaf): all 9 |coordinates| <= 1e8 (set by refit)
     int32_t* lfaces = nullptr;      // 3T: the faces in leaf (sorted) order, rebuilt with prim_idx
-    float* gpart = nullptr;         // 12 floats per internal node: child boxes handed over
-                                    // between workgroups at chunk-spanning nodes (left, right)
-    int32_t* lroot = nullptr;       // T maximal chunk-local subtree root of each leaf (-1 none)
-    int32_t* lbase = nullptr;       // T LDS slot base of that subtree
+    const float* pbox = nullptr;    // box mode: 6T per-triangle boxes (min xyz, max xyz) given by the
+                                    // caller, used instead of the vertices; null otherwise
+    int skip_adj = 0;               // never report pairs of triangles sharing a vertex
     int32_t* leaf_par = nullptr;    // T
     uint32_t* leaf_mark = nullptr;  // T
     int32_t* leaf_lca = nullptr;    // T+1
@@ -193,6 +202,9 @@ struct DevState {
     int64_t front_cap[2] = {0, 0};  // entries per buffer, per kind
     int fcur[2] = {0, 0};           // current buffer per kind
     int64_t front_version = 0;      // host-side: bumped whenever a step changes the front's contents
+    int64_t adj_version = -1;       // host-side: front version of the last shared-vertex partition
+    uint32_t* adj_cnt = nullptr;    // partition scratch: shared-vertex entries per tile, then offsets
+    uint32_t* adj_off = nullptr;
     uint32_t* fsize = nullptr;      // [kind*2 + buf] sizes (4 words)
     uint32_t* fcnt = nullptr;       // [kind][kPairSegs * kCntStride] segment counters of a front
                                     // being produced (front_cap is kPairSegs equal segments)
@@ -235,9 +247,12 @@ struct DevState {
 };
 
 enum Counter : int {
-    C_BP = 0,          // broad-phase pair count
-    C_NP = 1,          // narrow-phase hit count
+    C_BP = 0,          // broad-phase pair count (listed + implicit)
+    C_NP = 1,          // narrow-phase hit count (listed + implicit)
     C_OVF = 2,         // overflow bits: 1 int front, 2 ext front, 4 bp pairs, 8 np pairs
+    C_IMPLICIT = 3,    // broad-phase pairs of the last step held implicitly: the head of the
+                       // external front (shared-vertex entries, every coordinate finite)
+    C_IMPL_NP = 4,     // of those, counted as narrow-phase hits (0 without a narrow phase)
     C_RT_SUB = 5,      // restructure: subtree count
     C_RT_INT = 6,      // restructure: internal node count
     C_RT_LEAF = 7,     // restructure: leaf count
@@ -248,17 +263,43 @@ enum Counter : int {
     C_NP_SEGMAX = 12,
     C_FREQ_I = 13,     // capacity the last produced internal / external front needed
     C_FREQ_E = 14,
-    C_NUM = 16         // words 16..31 hold 64-bit counters (see Counter64)
+    // words 16..23 hold 64-bit counters (see Counter64)
+    C_ADJ = 24,        // shared-vertex entries heading the current external front (last partition)
+    C_NO_IMPLICIT = 25,// the last refit cannot vouch for shared-vertex pairs: a vertex coordinate
+                       // above 1e8 in magnitude (or NaN / inf), or boxes given by the caller
+    C_NUM = 32
 };
 enum Counter64 : int {
     C64_VISITS = 0,    // node tests of the last step
     C64_DIGEST = 1,    // pair-set digest
-    C64_NUM = 8
+    C64_NUM = 4
 };
 __host__ __device__ inline uint64_t* ctr64(uint32_t* ctr) { return reinterpret_cast<uint64_t*>(ctr + 16); }
 
+__device__ inline Box given_box(const float* __restrict__ b, int64_t t) {
+    const float* p = b + 6 * t;
+    return {p[0], p[1], p[2], p[3], p[4], p[5]};
+}
+// Box of triangle t: the caller's (box mode) or the vertices'.
+__device__ inline Box prim_box(const DevState& d, int32_t t) {
+    return d.pbox ? given_box(d.pbox, t) : tri_box(d.faces, d.verts, t);
+}
+// Do leaves a and b (faces in leaf order) share a vertex (the reference's covertex,
+// utility/CudaDeviceUtils.cu:153-157).
+__device__ inline bool faces_share_vertex(const int32_t* __restrict__ lfaces, int32_t a, int32_t b) {
+    const int32_t* fa = lfaces + 3 * (int64_t)a;
+    const int32_t* fb = lfaces + 3 * (int64_t)b;
+    const int32_t a0 = fa[0], a1 = fa[1], a2 = fa[2], b0 = fb[0], b1 = fb[1], b2 = fb[2];
+    return a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 || a2 == b1 ||
+           a2 == b2;
+}
+__device__ inline bool leaves_share_vertex(const DevState& d, int32_t a, int32_t b) {
+    return faces_share_vertex(d.lfaces, a, b);
+}
+
 // Launch helpers
 constexpr int kBlock = 256;
+constexpr uint32_t kAdjTile = 16 * kBlock;  // external-front entries per shared-vertex partition block
 inline unsigned grid_for(int64_t n, int block = kBlock) {
     int64_t g = (n + block - 1) / block;
     if (g < 1) g = 1;

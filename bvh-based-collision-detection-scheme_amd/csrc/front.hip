@@ -187,9 +187,12 @@ struct Out {
     volatile int2* sb[3];
     volatile uint32_t* sf[3];
     uint32_t scap;  // staging entries per wave and list
+    const int32_t* lfaces;
+    int skip_adj;   // drop pairs of triangles sharing a vertex
     // Broad-phase pairs are kept as leaf (sorted-slot) ids, query first (qi < j): the narrow
     // phase then reads leaf-ordered faces with locality, and maps to triangle ids itself.
     __device__ void pair(int32_t qi, int32_t j) const {
+        if (skip_adj && faces_share_vertex(lfaces, qi, j)) return;
         stage_push(bp, ovf, sb[2], sf[2], scap, true, make_int2(qi, j));
     }
     __device__ void int_entry(int32_t qi, int32_t node) const {
@@ -235,6 +238,8 @@ __device__ inline Out make_out(StagesBP& st, const DevState& d) {
     o.sb[2] = st.buf[w];
     o.sf[2] = st.fill[w];
     o.scap = kStage;
+    o.lfaces = d.lfaces;
+    o.skip_adj = d.skip_adj;
     return o;
 }
 
@@ -250,6 +255,8 @@ __device__ inline Out make_out(Stages& st, const FrontSink& fi, const FrontSink&
         o.sf[k] = st.fill[k][w];
     }
     o.scap = kStage3;
+    o.lfaces = d.lfaces;
+    o.skip_adj = d.skip_adj;
     return o;
 }
 
@@ -395,12 +402,17 @@ static FrontIn cur_front(DevState& d) {
 }
 __host__ __device__ inline FrontSink null_sink() { return FrontSink{nullptr, nullptr, nullptr, 0u, 0u}; }
 
-// Pair totals, overflow bits, visits and the broad-phase segment counters (fill kernels, no
-// memset nodes: the batched frame graph records this too).
+// One 64-lane launch (a kernel, not memsets: the batched frame graph records it): the per-step
+// pair counters, the visit counter and the broad-phase segment counters.
+__global__ void k_reset_step(DevState d) {
+    const int g = threadIdx.x;
+    d.pcnt[g * kCntStride] = 0u;
+    if (g <= C_IMPL_NP) d.ctr[g] = 0u;  // C_BP, C_NP, C_OVF, C_IMPLICIT, C_IMPL_NP
+    if (g == 8) ctr64(d.ctr)[C64_VISITS] = 0ull;
+}
 static void reset_step_counters(DevState& d, hipStream_t s) {
-    fill_u32(d.ctr, 3, 0u, s);
-    fill_u32(reinterpret_cast<uint32_t*>(ctr64(d.ctr) + C64_VISITS), 2, 0u, s);
-    fill_u32(d.pcnt, kPairSegs * kCntStride, 0u, s);
+    static_assert(kPairSegs == 64, "one lane per segment counter");
+    hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(kPairSegs), 0, s, d);
 }
 
 static unsigned stream_grid() { return 2048; }
@@ -428,21 +440,21 @@ int size_class(int64_t n) {
 }
 }  // namespace
 
-unsigned tuned_grid(int kernel, int64_t n) {
+unsigned tuned_shape(int kernel, int64_t n) {
     const unsigned g = g_grid[kernel][size_class(n)];
     return g ? g : kDefaultGrid[kernel];
 }
-void set_tuned_grid(int kernel, int64_t n, unsigned grid) { g_grid[kernel][size_class(n)] = grid; }
-void record_tuning(int kernel, int64_t n, unsigned grid, float ms) {
-    g_trials.push_back({kernel, size_class(n), grid, ms});
+void set_tuned_shape(int kernel, int64_t n, unsigned sh) { g_grid[kernel][size_class(n)] = sh; }
+void record_tuning(int kernel, int64_t n, unsigned sh, float ms) {
+    g_trials.push_back({kernel, size_class(n), sh, ms});
 }
 std::string tuning_report() {
     static const char* names[TK_COUNT] = {"k_keep", "k_narrow", "k_batch_cd"};
-    std::string r = "kernel size_class grid ms_per_launch chosen\n";
+    std::string r = "kernel size_class grid unroll ms_per_launch chosen\n";
     char line[160];
     for (const TuneTrial& t : g_trials) {
-        snprintf(line, sizeof(line), "%s 2^%d %u %.4f %s\n", names[t.kernel], t.cls, t.grid, t.ms,
-                 g_grid[t.kernel][t.cls] == t.grid ? "*" : "");
+        snprintf(line, sizeof(line), "%s 2^%d %u %u %.4f %s\n", names[t.kernel], t.cls, shape_grid(t.grid),
+                 shape_unroll(t.grid), t.ms, g_grid[t.kernel][t.cls] == t.grid ? "*" : "");
         r += line;
     }
     return r;
@@ -683,40 +695,165 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool orde
 }
 
 // ---------------------------------------------------------------- KEEP
-__global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in) {
+// KEEP over the current front, U entries per thread per loop iteration with all their loads
+// issued together: the kernel is bound by the latency of dependent loads (entry, then the two
+// node records), so more loads in flight per wave is what speeds it up.
+// implicit: the current front's shared-vertex head (C_ADJ entries after the last partition) is
+// not tested when the last refit found every coordinate finite: its pairs are broad-phase hits
+// whatever the positions (both boxes hold the shared vertex) and are counted, not listed.
+template <int U>
+__global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int implicit) {
     __shared__ StagesBP st;
     stages_init(st);
     const Out out = make_out(st, d);
-    const uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]), total = ni + ne;
+    uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]);
+    // with skip_adj the head's pairs are filtered out anyway: skipped and not counted
+    const uint32_t head = implicit ? min(d.ctr[C_ADJ], ne) : 0u;
+    const uint32_t skip = d.skip_adj || d.ctr[C_NO_IMPLICIT] == 0u ? head : 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.ctr[C_IMPLICIT] = d.skip_adj ? 0u : skip;
+    ne -= skip;
+    const uint32_t total = ni + ne;
+    const uint32_t* ext_node = in.node[1] + skip;
+    const int32_t* ext_query = in.query[1] + skip;
+    const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t visits = 0;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
-        if (k < ni) {
-            const int32_t q = in.query[0][k];
-            const int32_t node = (int32_t)in.node[0][k];
-            const float4 qlo = d.leaf[q].lo, qhi = d.leaf[q].hi;
-            const Box qb = rec_box(qlo, qhi);
-            const float4 lo = d.inode[node].lo, hi = d.inode[node].hi;
-            ++visits;
-            if (box_overlap(qb, lo, hi)) {
-                const int32_t start = (d.int_lc[node] << 1) | (int32_t)(d.int_mark[node] & 1u);
-                visits += stackless<false>(d, out, q, qb, start, f2i(hi.w));
+    for (uint32_t k0 = blockIdx.x * blockDim.x + threadIdx.x; k0 < total; k0 += U * stride) {
+        int32_t q[U], nd[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = k0 + u * stride;
+            q[u] = -1;
+            nd[u] = 0;
+            if (k < ni) {
+                q[u] = in.query[0][k];
+                nd[u] = (int32_t)in.node[0][k];
+            } else if (k < total) {
+                q[u] = ext_query[k - ni];
+                nd[u] = (int32_t)ext_node[k - ni];
             }
-        } else {
-            const int32_t q = in.query[1][k - ni];
-            const int32_t j = (int32_t)in.node[1][k - ni];
-            const float4 alo = d.leaf[q].lo, ahi = d.leaf[q].hi;
-            const float4 blo = d.leaf[j].lo, bhi = d.leaf[j].hi;
-            ++visits;
-            if (box_overlap(rec_box(alo, ahi), blo, bhi)) out.pair(q, j);
+        }
+        float4 qlo[U], qhi[U], lo[U], hi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            qlo[u] = qhi[u] = lo[u] = hi[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (q[u] >= 0) {
+                const NodeRec* r = k0 + u * stride < ni ? &d.inode[nd[u]] : &d.leaf[nd[u]];
+                qlo[u] = d.leaf[q[u]].lo;
+                qhi[u] = d.leaf[q[u]].hi;
+                lo[u] = r->lo;
+                hi[u] = r->hi;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool internal = k0 + u * stride < ni;
+            const Box qb = rec_box(qlo[u], qhi[u]);
+            const bool hit = q[u] >= 0 && box_overlap(qb, lo[u], hi[u]);
+            visits += q[u] >= 0 ? 1u : 0u;
+            if (internal && hit) {  // the node now overlaps: descend below it
+                const int32_t start = (d.int_lc[nd[u]] << 1) | (int32_t)(d.int_mark[nd[u]] & 1u);
+                visits += stackless<false>(d, out, q[u], qb, start, f2i(hi[u].w));
+            }
+            bool emit = hit && !internal;
+            if (out.skip_adj && emit) emit = !faces_share_vertex(out.lfaces, q[u], nd[u]);
+            stage_push(out.bp, out.ovf, out.sb[2], out.sf[2], out.scap, emit, make_int2(q[u], nd[u]));
         }
     }
     stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
     add_visits(d, visits);
 }
 
+// ---------------------------------------------------------------- shared-vertex partition
+// Reorders the current external front, stably, into [entries whose two triangles share a vertex |
+// the rest] and records the head length in C_ADJ. Two passes over tiles of kAdjTile entries
+// (count, exclusive scan, scatter into the spare buffer), run by KEEP once per front version.
+
+__global__ void __launch_bounds__(kBlock) k_adj_count(DevState d, FrontIn in, uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t wsum[kWaves];
+    const uint32_t ne = min(*in.size[1], in.cap[1]);
+    const uint32_t b0 = blockIdx.x * kAdjTile;
+    uint32_t c = 0;
+    for (uint32_t i = b0 + threadIdx.x; i < min(b0 + kAdjTile, ne); i += kBlock)
+        c += leaves_share_vertex(d, in.query[1][i], (int32_t)in.node[1][i]) ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+    if (lane_id() == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kWaves; ++w) t += wsum[w];
+        cnt[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_scatter(DevState d, FrontIn in, const uint32_t* __restrict__ off,
+                                                        uint32_t* __restrict__ dst_node, int32_t* __restrict__ dst_query,
+                                                        uint32_t* dst_size) {
+    __shared__ uint32_t wa[kWaves], wv[kWaves];
+    const uint32_t ne = min(*in.size[1], in.cap[1]);
+    const uint32_t total_adj = d.ctr[C_ADJ];
+    const uint32_t b0 = blockIdx.x * kAdjTile;
+    const int w = threadIdx.x >> 6;
+    uint32_t run_a = off[blockIdx.x], run_n = total_adj + (b0 - off[blockIdx.x]);
+    for (uint32_t t0 = b0; t0 < min(b0 + kAdjTile, ne); t0 += kBlock) {
+        const uint32_t i = t0 + threadIdx.x;
+        const bool valid = i < ne;
+        uint32_t node = 0;
+        int32_t query = 0;
+        if (valid) {
+            node = in.node[1][i];
+            query = in.query[1][i];
+        }
+        const bool adj = valid && leaves_share_vertex(d, query, (int32_t)node);
+        const uint64_t ma = __ballot(adj), mv = __ballot(valid);
+        if (lane_id() == 0) {
+            wa[w] = (uint32_t)__popcll(ma);
+            wv[w] = (uint32_t)__popcll(mv);
+        }
+        __syncthreads();
+        uint32_t before_a = 0, before_v = 0, tile_a = 0, tile_v = 0;
+        for (int k = 0; k < kWaves; ++k) {
+            if (k < w) {
+                before_a += wa[k];
+                before_v += wv[k];
+            }
+            tile_a += wa[k];
+            tile_v += wv[k];
+        }
+        if (valid) {
+            const uint64_t below = lanes_below();
+            const uint32_t o = adj ? run_a + before_a + (uint32_t)__popcll(ma & below)
+                                   : run_n + (before_v - before_a) + (uint32_t)__popcll(mv & ~ma & below);
+            dst_node[o] = node;
+            dst_query[o] = query;
+        }
+        run_a += tile_a;
+        run_n += tile_v - tile_a;
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst_size = ne;
+}
+
+void launch_adj_partition(DevState& d, int32_t n, hipStream_t s) {
+    const FrontIn in = cur_front(d);
+    const unsigned tiles = (unsigned)std::max<int64_t>(1, (d.front_cap[1] + kAdjTile - 1) / kAdjTile);
+    hipLaunchKernelGGL(k_adj_count, dim3(tiles), dim3(kBlock), 0, s, d, in, d.adj_cnt);
+    launch_exscan(d, d.adj_cnt, d.adj_off, tiles, d.ctr + C_ADJ, s);
+    const int nb = d.fcur[1] ^ 1;
+    hipLaunchKernelGGL(k_adj_scatter, dim3(tiles), dim3(kBlock), 0, s, d, in, d.adj_off, d.fnode[1][nb],
+                       d.fquery[1][nb], d.fsize + 2 + nb);
+    d.fcur[1] = nb;
+    d.adj_version = d.front_version;
+}
+
 void launch_keep(DevState& d, int32_t n, hipStream_t s) {
+    if (d.adj_version != d.front_version) launch_adj_partition(d, n, s);
     reset_step_counters(d, s);
-    hipLaunchKernelGGL(k_keep, dim3(tuned_grid(TK_KEEP, n)), dim3(kBlock), 0, s, d, cur_front(d));
+    const unsigned sh = tuned_shape(TK_KEEP, n), g = shape_grid(sh);
+    switch (shape_unroll(sh)) {
+        case 4: hipLaunchKernelGGL(k_keep<4>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), 1); break;
+        case 2: hipLaunchKernelGGL(k_keep<2>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), 1); break;
+        default: hipLaunchKernelGGL(k_keep<1>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), 1); break;
+    }
 }
 
 // ---------------------------------------------------------------- incremental front restructure
@@ -817,7 +954,7 @@ void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream
     FrontIn kept = cur_front(d);
     kept.size[0] = d.ctr + C_KEPT_I;
     kept.size[1] = d.ctr + C_KEPT_E;
-    hipLaunchKernelGGL(k_keep, dim3(stream_grid()), dim3(kBlock), 0, s, d, kept);
+    hipLaunchKernelGGL(k_keep<1>, dim3(stream_grid()), dim3(kBlock), 0, s, d, kept, 0);
     // the walks write segments into the spare buffers, then append after the kept entries
     front_begin(d, s);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
@@ -979,12 +1116,13 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
 void launch_narrow(DevState& d, hipStream_t s) {
     // kernels, not memsets: the batched frame graph records this
     fill_u32(d.pcnt + kPairSegs * kCntStride, kPairSegs * kCntStride, 0u, s);
-    hipLaunchKernelGGL(k_narrow, dim3(tuned_grid(TK_NARROW, d.maxT)), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_narrow, dim3(shape_grid(tuned_shape(TK_NARROW, d.maxT))), dim3(kBlock), 0, s, d);
 }
 
 // Totals of both segmented pair lists (requested counts, so an overflowing list reports what it
 // needed) and their largest segment: ctr[C_BP], ctr[C_NP], ctr[C_BP_SEGMAX], ctr[C_NP_SEGMAX].
-__global__ void k_pair_totals(DevState d) {
+// Implicit (shared-vertex) pairs are added to both totals; with a narrow phase they are hits.
+__global__ void k_pair_totals(DevState d, int narrow) {
     const int kind = threadIdx.x >> 6, g = lane_id();
     uint32_t v = d.pcnt[(kind * kPairSegs + g) * kCntStride], m = v;
     for (int off = 32; off > 0; off >>= 1) {
@@ -992,35 +1130,47 @@ __global__ void k_pair_totals(DevState d) {
         m = max(m, __shfl_down(m, off));
     }
     if (g == 0) {
-        d.ctr[kind ? C_NP : C_BP] = v;
+        const uint32_t impl = d.ctr[C_IMPLICIT];
+        if (kind == 1) d.ctr[C_IMPL_NP] = narrow ? impl : 0u;
+        d.ctr[kind ? C_NP : C_BP] = v + (kind == 0 || narrow ? impl : 0u);
         d.ctr[kind ? C_NP_SEGMAX : C_BP_SEGMAX] = m;
     }
 }
-void launch_pair_totals(DevState& d, hipStream_t s) {
-    hipLaunchKernelGGL(k_pair_totals, dim3(1), dim3(128), 0, s, d);
+void launch_pair_totals(DevState& d, hipStream_t s, bool narrow) {
+    hipLaunchKernelGGL(k_pair_totals, dim3(1), dim3(128), 0, s, d, narrow ? 1 : 0);
 }
 void clear_np(DevState& d, hipStream_t s) {
     fill_u32(d.pcnt + kPairSegs * kCntStride, kPairSegs * kCntStride, 0u, s);
 }
 
-// Contiguous copy of a segmented pair list (at most cap pairs) into dst; the broad-phase list
-// (leaf ids) is mapped to triangle ids, lower first.
-__global__ void __launch_bounds__(kBlock) k_pair_gather(DevState d, int kind, int2* __restrict__ dst, uint64_t cap) {
-    __shared__ uint32_t pre[kPairSegs + 1];
-    const uint32_t n = seg_prefix(d.pcnt + kind * kPairSegs * kCntStride, (uint32_t)d.pair_seg_cap, pre);
-    const uint32_t m = (uint32_t)min((uint64_t)n, cap);
-    const int2* src = kind ? d.np : d.bp;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
-        int2 p = src[seg_slot(pre, k, d.pair_seg_cap)];
-        if (kind == 0) {
-            const int32_t a = d.prim_idx[p.x], b = d.prim_idx[p.y];
-            p = a < b ? make_int2(a, b) : make_int2(b, a);
-        }
-        dst[k] = p;
+// Pair k of a list, in the virtual order [listed segments | implicit head of the external front];
+// leaf-id pairs (the broad-phase list, the implicit ones) mapped to triangle ids, lower first.
+__device__ inline int2 list_pair(const DevState& d, int kind, const uint32_t* pre, uint32_t listed, uint32_t k) {
+    int2 p;
+    if (k < listed) {
+        p = (kind ? d.np : d.bp)[seg_slot(pre, k, d.pair_seg_cap)];
+        if (kind == 1) return p;
+    } else {
+        const int c = d.fcur[1];
+        p = make_int2(d.fquery[1][c][k - listed], (int32_t)d.fnode[1][c][k - listed]);
     }
+    const int32_t a = d.prim_idx[p.x], b = d.prim_idx[p.y];
+    return a < b ? make_int2(a, b) : make_int2(b, a);
 }
-void launch_pair_gather(DevState& d, int kind, int2* dst, int64_t cap, hipStream_t s) {
-    hipLaunchKernelGGL(k_pair_gather, dim3(2048), dim3(kBlock), 0, s, d, kind, dst, (uint64_t)cap);
+
+// Contiguous copy of pairs [first, first + count) of a list into dst.
+__global__ void __launch_bounds__(kBlock) k_pair_gather(DevState d, int kind, int2* __restrict__ dst, uint64_t first,
+                                                        uint64_t count) {
+    __shared__ uint32_t pre[kPairSegs + 1];
+    const uint32_t listed = seg_prefix(d.pcnt + kind * kPairSegs * kCntStride, (uint32_t)d.pair_seg_cap, pre);
+    const uint64_t total = (uint64_t)listed + d.ctr[kind ? C_IMPL_NP : C_IMPLICIT];
+    const uint64_t end = min(first + count, total);
+    for (uint64_t k = first + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += gridDim.x * blockDim.x)
+        dst[k - first] = list_pair(d, kind, pre, listed, (uint32_t)k);
+}
+void launch_pair_gather(DevState& d, int kind, int2* dst, int64_t first, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_pair_gather, dim3(2048), dim3(kBlock), 0, s, d, kind, dst, (uint64_t)first,
+                       (uint64_t)count);
 }
 
 // ---------------------------------------------------------------- pair-set digest
@@ -1031,15 +1181,11 @@ __device__ inline uint64_t mix64(uint64_t z) {
 }
 __global__ void __launch_bounds__(kBlock) k_digest(DevState d, int kind, uint64_t* out) {
     __shared__ uint32_t pre[kPairSegs + 1];
-    const uint32_t n = seg_prefix(d.pcnt + kind * kPairSegs * kCntStride, (uint32_t)d.pair_seg_cap, pre);
-    const int2* src = kind ? d.np : d.bp;
+    const uint32_t listed = seg_prefix(d.pcnt + kind * kPairSegs * kCntStride, (uint32_t)d.pair_seg_cap, pre);
+    const uint32_t n = listed + d.ctr[kind ? C_IMPL_NP : C_IMPLICIT];
     uint64_t acc = 0;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int2 p = src[seg_slot(pre, k, d.pair_seg_cap)];
-        if (kind == 0) {
-            const int32_t a = d.prim_idx[p.x], b = d.prim_idx[p.y];
-            p = a < b ? make_int2(a, b) : make_int2(b, a);
-        }
+        const int2 p = list_pair(d, kind, pre, listed, k);
         acc += mix64(((uint64_t)(uint32_t)p.x << 32) | (uint32_t)p.y);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);

@@ -24,9 +24,13 @@ void launch_emit_full(DevState& d, int32_t n, hipStream_t s);
 // Launch-shape tuning: grid sizes (workgroups) of the grid-stride / persistent kernels per size
 // class ceil(log2 n), process-wide; 0 entries fall back to the defaults.
 enum TunedKernel { TK_KEEP = 0, TK_NARROW = 1, TK_BATCH_CD = 2, TK_COUNT = 3 };
-unsigned tuned_grid(int kernel, int64_t n);
-void set_tuned_grid(int kernel, int64_t n, unsigned grid);
-void record_tuning(int kernel, int64_t n, unsigned grid, float ms);
+// A launch shape: grid size (low 24 bits) and items per thread per loop iteration (1, 2, 4).
+inline unsigned make_shape(unsigned grid, unsigned unroll) { return grid | (unroll << 24); }
+inline unsigned shape_grid(unsigned sh) { return sh & 0xFFFFFFu; }
+inline unsigned shape_unroll(unsigned sh) { return (sh >> 24) ? (sh >> 24) : 1u; }
+unsigned tuned_shape(int kernel, int64_t n);
+void set_tuned_shape(int kernel, int64_t n, unsigned sh);
+void record_tuning(int kernel, int64_t n, unsigned sh, float ms);
 std::string tuning_report();
 // Emission from sorted 64-bit keys (batched scenes: scene id above the Morton bits).
 void launch_emit_full64(DevState& d, const uint64_t* keys, int32_t n, hipStream_t s);
@@ -42,6 +46,7 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
                       hipStream_t s);
 void launch_refit(DevState& d, int32_t n, hipStream_t s);
 void launch_refit_prep(DevState& d, int32_t n, hipStream_t s);  // after every topology change
+int32_t refit_chunks(int32_t n);  // refit workgroups (treelet chunks) for n leaves
 void launch_prim_map(DevState& d, int32_t n, hipStream_t s);
 
 // front.hip
@@ -63,9 +68,9 @@ void launch_narrow(DevState& d, hipStream_t s);
 // Segmented pair lists: totals (and largest segments) into ctr[C_BP / C_NP / *_SEGMAX]; clearing
 // the narrow-phase list; a contiguous copy of list `kind` (0 broad phase, mapped from leaf to
 // triangle ids; 1 narrow phase) of at most cap pairs into dst.
-void launch_pair_totals(DevState& d, hipStream_t s);
+void launch_pair_totals(DevState& d, hipStream_t s, bool narrow);
 void clear_np(DevState& d, hipStream_t s);
-void launch_pair_gather(DevState& d, int kind, int2* dst, int64_t cap, hipStream_t s);
+void launch_pair_gather(DevState& d, int kind, int2* dst, int64_t first, int64_t count, hipStream_t s);
 void launch_digest(DevState& d, int filtered, hipStream_t s);
 void launch_restr_decision(DevState& d, int32_t n, hipStream_t s);    // counts into ctr[C_RT_*]
 

@@ -18,8 +18,9 @@
 //  * Sorting is an LSD radix sort with 8-bit digits over an LDS-staged tile of 4096 keys:
 //    per-tile digit counts, a device-wide exclusive scan of the digit-major count table,
 //    then a stable wave64 multisplit ranking in LDS and coalesced scatter.
-//  * Refit is two-level: chunk-local subtrees climb through LDS inside one workgroup; only
-//    the few nodes spanning chunk boundaries use agent-scope acquire/release counters.
+//  * Refit is level-synchronous over treelet-aligned chunks: each workgroup merges its
+//    treelets' nodes one depth level per barrier in LDS (no atomics), then one workgroup
+//    merges the small top tree above the treelets.
 #include "bvhcd_internal.hpp"
 #include "kernels.hpp"
 
@@ -42,11 +43,13 @@ __global__ void k_scene_box_init(uint32_t* box_u) {
     else if (threadIdx.x < 6) box_u[threadIdx.x] = 0u;           // max slots
 }
 
+// pbox: per-triangle boxes given by the caller (box mode), else null (boxes from the vertices).
 __global__ void __launch_bounds__(kBlock) k_scene_box(int32_t T, const int32_t* __restrict__ faces,
-                                                      const float* __restrict__ verts, uint32_t* box_u) {
+                                                      const float* __restrict__ verts, const float* __restrict__ pbox,
+                                                      uint32_t* box_u) {
     Box acc = empty_box();
     for (int32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
-        acc = box_merge(acc, tri_box(faces, verts, t));
+        acc = box_merge(acc, pbox ? given_box(pbox, t) : tri_box(faces, verts, t));
     // wave reduction
     float v[6] = {acc.mnx, acc.mny, acc.mnz, acc.mxx, acc.mxy, acc.mxz};
     for (int off = 32; off > 0; off >>= 1)
@@ -73,11 +76,12 @@ __global__ void k_scene_box_final(const uint32_t* box_u, float* box) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_codes(int32_t T, const int32_t* __restrict__ faces,
-                                                  const float* __restrict__ verts, const float* __restrict__ scene,
-                                                  uint32_t* __restrict__ codes, int32_t* __restrict__ vals) {
+                                                  const float* __restrict__ verts, const float* __restrict__ pbox,
+                                                  const float* __restrict__ scene, uint32_t* __restrict__ codes,
+                                                  int32_t* __restrict__ vals) {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
-    codes[t] = box_code(tri_box(faces, verts, t), scene);
+    codes[t] = box_code(pbox ? given_box(pbox, t) : tri_box(faces, verts, t), scene);
     vals[t] = t;
 }
 
@@ -85,12 +89,13 @@ void launch_scene_box(DevState& d, int32_t T, hipStream_t s) {
     hipLaunchKernelGGL(k_scene_box_init, dim3(1), dim3(64), 0, s, d.scene_box_u);
     unsigned g = grid_for(T);
     if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(k_scene_box, dim3(g), dim3(kBlock), 0, s, T, d.faces, d.verts, d.scene_box_u);
+    hipLaunchKernelGGL(k_scene_box, dim3(g), dim3(kBlock), 0, s, T, d.faces, d.verts, d.pbox, d.scene_box_u);
     hipLaunchKernelGGL(k_scene_box_final, dim3(1), dim3(64), 0, s, d.scene_box_u, d.scene_box);
 }
 
 void launch_codes(DevState& d, int32_t T, uint32_t* codes, int32_t* vals, hipStream_t s) {
-    hipLaunchKernelGGL(k_codes, dim3(grid_for(T)), dim3(kBlock), 0, s, T, d.faces, d.verts, d.scene_box, codes, vals);
+    hipLaunchKernelGGL(k_codes, dim3(grid_for(T)), dim3(kBlock), 0, s, T, d.faces, d.verts, d.pbox, d.scene_box, codes,
+                       vals);
 }
 
 // ======================================================================= exclusive scan (u32)
@@ -674,41 +679,190 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
 }
 
 // ======================================================================= refit
-// Two-level bottom-up refit. Leaves are cut into chunks of kRefitChunk consecutive slots; a
-// node whose leaf range lies inside one chunk is "local" and is refit inside that chunk's
-// workgroup through LDS (boxes, arrival counters with workgroup-scope ordering). A local
-// node's LDS slot is (id - r) + (rx(r) - chunk start), r = its maximal local subtree root:
-// preorder makes every local subtree a contiguous id range next to a contiguous leaf range,
-// so the slots of one chunk never collide. Only nodes spanning two chunks (a few per chunk
-// boundary) climb through global memory with agent-scope acquire/release counters.
-constexpr int kRefitChunk = 1024;
+// Treelet-aligned, level-synchronous refit: no atomics, no cross-workgroup hand-off.
+//
+// Topology preparation (after every topology change):
+//  * a treelet is a maximal subtree with at most kTreelet leaves (a leaf whose parent holds
+//    more is a treelet of its own); the internal nodes holding more form the top tree;
+//  * chunk c is the run of treelets whose first leaf lies in [c*kTreelet, (c+1)*kTreelet).
+//    The treelet holding leaf c*kTreelet either starts there or ends before (c+1)*kTreelet - 1,
+//    so every window but the tail of the last treelet starts one, the chunks tile the leaves
+//    and each spans fewer than kChunkCap = 2*kTreelet leaves;
+//  * preorder numbering gives the internal nodes of a treelet with root r and leaves [a, b]
+//    the ids [r, r + b - a): node p takes slot p - r + a, slot b stays empty, and the slots of
+//    a chunk are its own leaf range;
+//  * each chunk lists its internal nodes deepest first (depth below the treelet root) with
+//    the LDS slots of the node and its two children (leaf x: x - chunk start; internal node:
+//    slot - chunk start + kChunkCap); the top tree is listed deepest first (depth below the
+//    root), children as internal ids or ~leaf ids.
+// Per refit: one workgroup per chunk writes the leaf boxes into LDS, merges one depth level
+// per barrier in LDS and writes its node boxes at the end; then one workgroup merges the top
+// tree level by level through the node records it writes itself (workgroup-coherent).
+// Depth bins: the tree is a radix tree over at most 96 key bits (code + slot tie-break), so no
+// depth reaches kDepthCap; within a chunk the depth is below the chunk size anyway.
+constexpr int kTreelet = 512;
+constexpr int kChunkCap = 2 * kTreelet;
+constexpr int kRefitPer = kChunkCap / kBlock;  // entries (leaves, internal nodes) per thread
+constexpr int kDepthCap = 1024;
+constexpr int kTopBlock = 1024;
+static_assert(kChunkCap % kBlock == 0, "chunk capacity is a whole number of block passes");
+static_assert(2 * kChunkCap <= 65536, "slots are packed in 16 bits");
 
-// Per leaf: maximal local root (internal id, or -1 when the parent already spans two chunks)
-// and the LDS slot base of that local subtree. Recomputed after every topology change.
-__global__ void __launch_bounds__(kBlock) k_refit_prep(int32_t n, const int32_t* __restrict__ leaf_par,
-                                                       const int32_t* __restrict__ int_par,
-                                                       const int32_t* __restrict__ int_rx,
-                                                       const int32_t* __restrict__ int_ry, int32_t* __restrict__ lroot,
-                                                       int32_t* __restrict__ lbase) {
+__device__ inline bool rp_is_top(const DevState& d, int32_t p) { return d.int_ry[p] - d.int_rx[p] >= kTreelet; }
+
+// Treelet starts (atomic min per window) and the unordered top-tree list.
+__global__ void __launch_bounds__(kBlock) k_rp_starts(int32_t n, DevState d) {
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
-    const int32_t s = (x / kRefitChunk) * kRefitChunk;
-    const int32_t e = min(s + kRefitChunk, n);
-    int32_t r = -1;
-    int32_t p = leaf_par[x];
-    for (int32_t guard = 0; p >= 0 && guard < kRefitChunk; ++guard) {
-        if (int_rx[p] < s || int_ry[p] >= e) break;
-        r = p;
-        p = int_par[p];
+    const int32_t lp = d.leaf_par[x];
+    if (lp < 0 || rp_is_top(d, lp)) atomicMin(&d.rp_cstart[x / kTreelet], x);
+    bool top = false;
+    int4 e = make_int4(0, 0, 0, 0);
+    if (x < n - 1) {
+        const int32_t rx = d.int_rx[x];
+        top = d.int_ry[x] - rx >= kTreelet;
+        const int32_t par = d.int_par[x];
+        if (!top && (par < 0 || rp_is_top(d, par))) atomicMin(&d.rp_cstart[rx / kTreelet], rx);
+        if (top) {
+            int32_t depth = 0;
+            for (int32_t q = par; q >= 0 && depth < kDepthCap - 1; q = d.int_par[q]) ++depth;
+            const int4 tp = d.itopo[x];
+            e = make_int4(x, tp.x, tp.y, depth | ((tp.w & 1) << 16) | ((tp.w & 2) << 17));
+        }
     }
-    lroot[x] = r;
-    lbase[x] = r >= 0 ? int_rx[r] - s - r : 0;
+    const uint32_t slot = wave_append(d.rp_tcount, top);
+    if (top) d.rp_tent[slot] = e;
 }
 
+// Per leaf: the treelet root and depth, then the entries of the internal nodes whose leftmost
+// leaf it is (each node written once; slot b of every treelet written empty by leaf b).
+__global__ void __launch_bounds__(kBlock) k_rp_walk(int32_t n, DevState d) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int4 empty = make_int4(-1, 0, 0, -1);
+    const int32_t p = d.leaf_par[x];
+    int32_t len = 0, r = -1;
+    for (int32_t q = p; q >= 0 && len < kTreelet && !rp_is_top(d, q); q = d.int_par[q]) {
+        r = q;
+        ++len;
+    }
+    if (len == 0) {  // a treelet of its own
+        d.rp_ent[x] = empty;
+        return;
+    }
+    const int32_t a = d.int_rx[r];
+    if (x == d.int_ry[r]) d.rp_ent[x] = empty;
+    const int32_t cs = d.rp_cstart[a / kTreelet];
+    const int32_t base = a - r - cs + kChunkCap;  // LDS slot of internal node u: u + base
+    int32_t q = p;
+    for (int32_t i = 0; i < len && d.int_rx[q] == x; ++i) {
+        const int4 tp = d.itopo[q];
+        const int32_t ls = (tp.w & 1) ? tp.x - cs : tp.x + base;
+        const int32_t rs = (tp.w & 2) ? tp.y - cs : tp.y + base;
+        d.rp_ent[q - r + a] = make_int4(q, q + base, ls | (rs << 16), len - 1 - i);
+        q = tp.z;
+    }
+}
+
+// Per chunk: entries sorted in place by depth, deepest first; (count, deepest depth).
+__global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
+    __shared__ uint32_t hist[kDepthCap];
+    __shared__ uint32_t dmax;
+    const int32_t c = blockIdx.x;
+    const int32_t s = d.rp_cstart[c], e = d.rp_cstart[c + 1];
+    for (int i = threadIdx.x; i < kDepthCap; i += kBlock) hist[i] = 0;
+    if (threadIdx.x == 0) dmax = 0;
+    __syncthreads();
+    int4 v[kRefitPer];
+#pragma unroll
+    for (int k = 0; k < kRefitPer; ++k) {
+        const int32_t i = s + threadIdx.x + k * kBlock;
+        v[k] = i < e ? d.rp_ent[i] : make_int4(-1, 0, 0, -1);
+        if (v[k].w >= 0) {
+            atomicAdd(&hist[v[k].w], 1u);
+            atomicMax(&dmax, (uint32_t)v[k].w);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int t = (int)dmax; t >= 0; --t) {
+            const uint32_t h = hist[t];
+            hist[t] = acc;
+            acc += h;
+        }
+        d.rp_cinfo[c] = make_int2((int32_t)acc, (int32_t)dmax);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRefitPer; ++k)
+        if (v[k].w >= 0) d.rp_ent[s + atomicAdd(&hist[v[k].w], 1u)] = v[k];
+}
+
+// Top tree sorted deepest first; level t occupies [rp_tlev[t], t ? rp_tlev[t-1] : count).
+// Entry (node, left, right, depth | left kind << 16 | right kind << 18), kind 0: internal node
+// record, 1: leaf record, 2: list position of a top node (set here once positions are known;
+// the workgroup's own global writes are ordered for it by the barrier).
+__device__ inline int32_t top_depth(const int4& e) { return e.w & 0xFFFF; }
+__device__ inline int top_kind(const int4& e, int side) { return (e.w >> (16 + 2 * side)) & 3; }
+
+__global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
+    __shared__ uint32_t hist[kDepthCap];
+    __shared__ uint32_t dmax;
+    const int32_t m = (int32_t)*d.rp_tcount;
+    for (int i = threadIdx.x; i < kDepthCap; i += kTopBlock) hist[i] = 0;
+    if (threadIdx.x == 0) dmax = 0;
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
+        const uint32_t t = (uint32_t)top_depth(d.rp_tent[i]);
+        atomicAdd(&hist[t], 1u);
+        atomicMax(&dmax, t);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int t = (int)dmax; t >= 0; --t) {
+            const uint32_t h = hist[t];
+            hist[t] = acc;
+            d.rp_tlev[t] = (int32_t)acc;
+            acc += h;
+        }
+        d.rp_tinfo[0] = m;
+        d.rp_tinfo[1] = m > 0 ? (int32_t)dmax : -1;
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
+        const int4 e = d.rp_tent[i];
+        const uint32_t pos = atomicAdd(&hist[top_depth(e)], 1u);
+        d.rp_tlist[pos] = e;
+        d.rp_tpos[e.x] = (int32_t)pos;
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
+        int4 e = d.rp_tlist[i];
+        if (top_kind(e, 0) == 0 && rp_is_top(d, e.y)) {
+            e.y = d.rp_tpos[e.y];
+            e.w |= 2 << 16;
+        }
+        if (top_kind(e, 1) == 0 && rp_is_top(d, e.z)) {
+            e.z = d.rp_tpos[e.z];
+            e.w |= 2 << 18;
+        }
+        d.rp_tlist[i] = e;
+    }
+}
+
+int32_t refit_chunks(int32_t n) { return (n + kTreelet - 1) / kTreelet; }
+
 void launch_refit_prep(DevState& d, int32_t n, hipStream_t s) {
-    if (n >= 2)
-        hipLaunchKernelGGL(k_refit_prep, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d.leaf_par, d.int_par, d.int_rx,
-                           d.int_ry, d.lroot, d.lbase);
+    if (n < 1) return;
+    const int32_t nc = refit_chunks(n);
+    fill_i32(d.rp_cstart, nc + 1, n, s);
+    fill_u32(d.rp_tcount, 1, 0u, s);
+    hipLaunchKernelGGL(k_rp_starts, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d);
+    hipLaunchKernelGGL(k_rp_walk, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d);
+    hipLaunchKernelGGL(k_rp_chunk_sort, dim3(nc), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_rp_top_sort, dim3(1), dim3(kTopBlock), 0, s, d);
 }
 
 __device__ inline void store_box(NodeRec* rec, const Box& b) {
@@ -717,111 +871,156 @@ __device__ inline void store_box(NodeRec* rec, const Box& b) {
     p[1].x = b.mxx; p[1].y = b.mxy; p[1].z = b.mxz;
 }
 
-// Agent-scope (device-coherent) relaxed store / load of one float: used only to hand a
-// child box to the workgroup that finishes a chunk-spanning node. Ordering comes from
-// waiting for the stores before the arrival counter moves (s_waitcnt) and from issuing the
-// loads only after the counter's return value is known, so no L2 write-back or
-// invalidate (an agent-scope release/acquire fence) is needed on this path.
-__device__ inline void st_agent(float* p, float v) {
-    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __builtin_bit_cast(uint32_t, v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline float ld_agent(const float* p) {
-    return __builtin_bit_cast(float, __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT));
-}
-
-// Continue the climb above the chunk from `self` (leaf or internal) holding box `b`.
-__device__ void refit_global_climb(int32_t n, int32_t self, bool self_leaf, Box b, const DevState& d) {
-    int32_t p = self_leaf ? d.leaf_par[self] : d.int_par[self];
-    for (int32_t guard = 0; p >= 0 && guard < n; ++guard) {
-        const int4 tp = d.itopo[p];  // lc, rc, par, mark
-        const uint32_t m = (uint32_t)tp.w;
-        const bool self_is_left = (((m & 1u) != 0) == self_leaf) && tp.x == self;
-        float* mine = d.gpart + 12 * (int64_t)p + (self_is_left ? 0 : 6);
-        st_agent(mine + 0, b.mnx); st_agent(mine + 1, b.mny); st_agent(mine + 2, b.mnz);
-        st_agent(mine + 3, b.mxx); st_agent(mine + 4, b.mxy); st_agent(mine + 5, b.mxz);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);  // the six stores are complete before the counter moves
-        const uint32_t old = __hip_atomic_fetch_add(&d.int_flag[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (old == 0u) return;  // the sibling's thread finishes p
-        const float* other = d.gpart + 12 * (int64_t)p + (self_is_left ? 6 : 0);
-        const Box o{ld_agent(other + 0), ld_agent(other + 1), ld_agent(other + 2),
-                    ld_agent(other + 3), ld_agent(other + 4), ld_agent(other + 5)};
-        b = box_merge(b, o);
-        store_box(&d.inode[p], b);  // read by later kernels only
-        self = p;
-        self_leaf = false;
-        p = tp.z;
-    }
-}
-
 __global__ void __launch_bounds__(kBlock) k_refit(int32_t n, DevState d) {
-    __shared__ Box leafbox[kRefitChunk];
-    __shared__ Box nodebox[kRefitChunk];
-    __shared__ uint32_t flag[kRefitChunk];
-    const int32_t s = blockIdx.x * kRefitChunk;
-    const int32_t e = min(s + kRefitChunk, n);
-    for (int i = threadIdx.x; i < kRefitChunk; i += kBlock) flag[i] = 0;
+    __shared__ Box sb[2 * kChunkCap];
+    const int32_t c = blockIdx.x;
+    const int32_t s = d.rp_cstart[c], e = d.rp_cstart[c + 1];
+    const int2 info = d.rp_cinfo[c];  // (internal nodes, deepest level)
     // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase
-    for (int32_t x = s + threadIdx.x; x < e; x += kBlock) {
-        const int32_t* f = d.lfaces + 3 * (int64_t)x;
-        bool fin = true;
-        Box b = empty_box();
-        for (int k = 0; k < 3; ++k) {
-            const float* v = d.verts + 3 * (int64_t)f[k];
-            const float vx = v[0], vy = v[1], vz = v[2];
-            // |x| <= 1e8 keeps every SAT product finite (dots of edge x normal crosses stay
-            // below 1536 * 1e32): no NaN can drop the shared vertex's projection.
-            fin = fin && fabsf(vx) <= 1e8f && fabsf(vy) <= 1e8f && fabsf(vz) <= 1e8f;
-            box_add(b, vx, vy, vz);
-        }
-        d.tri_finite[x] = fin ? 1 : 0;
-        leafbox[x - s] = b;
-        store_box(&d.leaf[x], b);
-    }
-    __syncthreads();
-    if (n < 2) return;
-    for (int32_t x = s + threadIdx.x; x < e; x += kBlock) {
-        Box b = leafbox[x - s];
-        const int32_t root = d.lroot[x];
-        if (root < 0) {  // parent spans two chunks
-            refit_global_climb(n, x, true, b, d);
-            continue;
-        }
-        const int32_t base = d.lbase[x];
-        int32_t self = x;
-        bool self_leaf = true;
-        int32_t p = d.leaf_par[x];
-        for (int32_t guard = 0; guard < kRefitChunk; ++guard) {
-            const int4 tp = d.itopo[p];  // issued before the arrival counter: off the critical path
-            const int32_t slot = p + base;
-            const uint32_t old =
-                __hip_atomic_fetch_add(&flag[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0u) break;  // sibling not done yet: it continues
-            const uint32_t m = (uint32_t)tp.w;
-            const bool self_is_left = (((m & 1u) != 0) == self_leaf) && tp.x == self;
-            const int32_t sib = self_is_left ? tp.y : tp.x;
-            const bool sib_leaf = self_is_left ? (m & 2u) : (m & 1u);
-            b = box_merge(b, sib_leaf ? leafbox[sib - s] : nodebox[sib + base]);
-            nodebox[slot] = b;
-            store_box(&d.inode[p], b);
-            if (p == root) {
-                refit_global_climb(n, p, false, b, d);
-                break;
+    int bad = 0;
+#pragma unroll
+    for (int k = 0; k < kRefitPer; ++k) {
+        const int32_t x = s + threadIdx.x + k * kBlock;
+        if (x < e) {
+            bool fin = true;
+            Box b = empty_box();
+            if (d.pbox) {  // box mode: the caller's box; a shared vertex need not lie in both boxes
+                b = given_box(d.pbox, d.prim_idx[x]);
+                fin = false;
+            } else {
+                const int32_t* f = d.lfaces + 3 * (int64_t)x;
+                for (int j = 0; j < 3; ++j) {
+                    const float* v = d.verts + 3 * (int64_t)f[j];
+                    const float vx = v[0], vy = v[1], vz = v[2];
+                    // |x| <= 1e8 keeps every SAT product finite (dots of edge x normal crosses
+                    // stay below 1536 * 1e32): no NaN can drop the shared vertex's projection.
+                    fin = fin && fabsf(vx) <= 1e8f && fabsf(vy) <= 1e8f && fabsf(vz) <= 1e8f;
+                    box_add(b, vx, vy, vz);
+                }
             }
-            self = p;
-            self_leaf = false;
-            p = tp.z;
+            d.tri_finite[x] = fin ? 1 : 0;
+            bad |= fin ? 0 : 1;
+            sb[x - s] = b;
+            store_box(&d.leaf[x], b);
         }
+    }
+    int4 v[kRefitPer];
+#pragma unroll
+    for (int k = 0; k < kRefitPer; ++k) {
+        const int32_t i = threadIdx.x + k * kBlock;
+        v[k] = i < info.x ? d.rp_ent[s + i] : make_int4(-1, 0, 0, -1);
+    }
+    Box out[kRefitPer];
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) d.rp_cbad[c] = (uint32_t)bad;
+    for (int32_t lev = info.y; lev >= 0; --lev) {
+#pragma unroll
+        for (int k = 0; k < kRefitPer; ++k) {
+            if (v[k].w == lev) {
+                out[k] = box_merge(sb[v[k].z & 0xFFFF], sb[v[k].z >> 16]);
+                sb[v[k].y] = out[k];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kRefitPer; ++k)
+        if (v[k].w >= 0) store_box(&d.inode[v[k].x], out[k]);
+}
+
+__device__ inline Box record_box(const DevState& d, int32_t v, int kind) {
+    const NodeRec* r = kind == 1 ? &d.leaf[v] : &d.inode[v];
+    return rec_box(r->lo, r->hi);
+}
+
+// One workgroup. Top trees of up to kTopLds nodes stay in LDS, slot = list position: a node
+// whose children are both below the top tree (treelet roots) is merged at once, a node with
+// one such child starts its slot with that child's box and merges the other at its level.
+// Larger top trees go level by level through the node records the workgroup writes itself
+// (the barrier orders them at workgroup scope).
+constexpr int kTopLds = 6144;
+constexpr int kTopPer = kTopLds / kTopBlock;
+
+__global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chunks) {
+    __shared__ Box tb[kTopLds];
+    // frame-wide: may KEEP count shared-vertex pairs without a test (every leaf box built from
+    // finite vertices); or over the chunks
+    int bad = 0;
+    for (int32_t c = threadIdx.x; c < chunks; c += kTopBlock) bad |= (int)d.rp_cbad[c];
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
+    const int32_t m = d.rp_tinfo[0], top = d.rp_tinfo[1];
+    if (m <= kTopLds) {
+        int4 e[kTopPer];
+#pragma unroll
+        for (int k = 0; k < kTopPer; ++k) {
+            const int32_t i = threadIdx.x + k * kTopBlock;
+            e[k] = i < m ? d.rp_tlist[i] : make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
+        }
+        // the record loads of a group of entries all in flight before its merges (children below
+        // the top tree); two groups keep the registers within the 1024-thread budget
+        constexpr int kGroup = kTopPer / 2;
+#pragma unroll
+        for (int g = 0; g < kTopPer; g += kGroup) {
+            Box cl[kGroup], cr[kGroup];
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
+                const int lk = top_kind(e[g + k], 0), rk = top_kind(e[g + k], 1);
+                cl[k] = lk != 2 ? record_box(d, e[g + k].y, lk) : Box{};
+                cr[k] = rk != 2 ? record_box(d, e[g + k].z, rk) : Box{};
+            }
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
+                const int32_t i = threadIdx.x + (g + k) * kTopBlock;
+                const int lk = top_kind(e[g + k], 0), rk = top_kind(e[g + k], 1);
+                if (lk != 2 && rk != 2) {
+                    tb[i] = box_merge(cl[k], cr[k]);
+                    e[g + k].w |= 0xFFFF;  // done
+                } else if (lk != 2) {
+                    tb[i] = cl[k];
+                } else if (rk != 2) {
+                    tb[i] = cr[k];
+                }
+            }
+        }
+        __syncthreads();
+        for (int32_t lev = top; lev >= 0; --lev) {
+#pragma unroll
+            for (int k = 0; k < kTopPer; ++k) {
+                if (top_depth(e[k]) == lev) {
+                    const int32_t i = threadIdx.x + k * kTopBlock;
+                    const int lk = top_kind(e[k], 0), rk = top_kind(e[k], 1);
+                    const Box l = tb[lk == 2 ? e[k].y : i];
+                    const Box r = tb[rk == 2 ? e[k].z : i];
+                    tb[i] = box_merge(l, r);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int k = 0; k < kTopPer; ++k) {
+            const int32_t i = threadIdx.x + k * kTopBlock;
+            if (i < m) store_box(&d.inode[e[k].x], tb[i]);
+        }
+        return;
+    }
+    for (int32_t lev = top; lev >= 0; --lev) {
+        const int32_t b0 = d.rp_tlev[lev], b1 = lev > 0 ? d.rp_tlev[lev - 1] : m;
+        for (int32_t i = b0 + (int32_t)threadIdx.x; i < b1; i += kTopBlock) {
+            const int4 e = d.rp_tlist[i];
+            const int lk = top_kind(e, 0), rk = top_kind(e, 1);
+            const Box l = record_box(d, lk == 2 ? d.rp_tlist[e.y].x : e.y, lk == 2 ? 0 : lk);
+            const Box r = record_box(d, rk == 2 ? d.rp_tlist[e.z].x : e.z, rk == 2 ? 0 : rk);
+            store_box(&d.inode[e.x], box_merge(l, r));
+        }
+        __syncthreads();
     }
 }
 
 void launch_refit(DevState& d, int32_t n, hipStream_t s) {
-    if (n >= 2) fill_u32(d.int_flag, n - 1, 0u, s);
-    const unsigned chunks = (unsigned)((n + kRefitChunk - 1) / kRefitChunk);
-    hipLaunchKernelGGL(k_refit, dim3(chunks), dim3(kBlock), 0, s, n, d);
+    if (n < 1) return;
+    hipLaunchKernelGGL(k_refit, dim3(refit_chunks(n)), dim3(kBlock), 0, s, n, d);
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(kTopBlock), 0, s, d, refit_chunks(n));
 }
 
 // ======================================================================= prim map / misc

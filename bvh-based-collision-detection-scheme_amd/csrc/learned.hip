@@ -406,9 +406,22 @@ __global__ void k_treelet_compact(int32_t I, const uint32_t* __restrict__ flags,
     if (i < I && flags[i]) roots[pos[i]] = i;
 }
 
+// Adds, for every distinct key among the lanes with pred set, the number of such lanes to
+// cnt[key] with one atomic (neighbouring front entries share nodes, so keys repeat within a
+// wave and per-lane atomics would serialise on the same address).
+__device__ inline void wave_count(uint32_t* __restrict__ cnt, int32_t key, bool pred) {
+    uint64_t todo = __ballot(pred);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int32_t k = __shfl(key, leader);
+        const uint64_t same = __ballot(pred && key == k);
+        if (lane_id() == leader) atomicAdd(&cnt[k], (uint32_t)__popcll(same));
+        todo &= ~same;
+    }
+}
+
 // Front entries per leaf: every entry counts for its node's leftmost leaf (acnt), external
-// entries also for their leaf (lcnt). Leaves of neighbouring entries are mostly distinct, so
-// plain per-lane atomics over 2 T addresses; recomputed only when the front changed.
+// entries also for their leaf (lcnt); recomputed only when the front changed.
 __global__ void __launch_bounds__(kBlock) k_count_front(DevState d, uint32_t* __restrict__ acnt,
                                                         uint32_t* __restrict__ lcnt) {
     const int c0 = d.fcur[0], c1 = d.fcur[1];
@@ -418,8 +431,8 @@ __global__ void __launch_bounds__(kBlock) k_count_front(DevState d, uint32_t* __
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
         const bool ext = k >= ni;
         const int32_t leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
-        atomicAdd(&acnt[leaf], 1u);
-        if (ext) atomicAdd(&lcnt[leaf], 1u);
+        wave_count(acnt, leaf, true);
+        wave_count(lcnt, leaf, ext);
     }
 }
 

@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(kBlock) k_restr_keys(int32_t n, DevState d, ui
     const int32_t r = d.rt_of_leaf[x];
     const int32_t tri = d.prim_idx[x];
     uint64_t key;
-    if (r >= 0) key = ((uint64_t)(uint32_t)d.int_rx[r] << 30) | box_code(tri_box(d.faces, d.verts, tri), d.scene_box);
+    if (r >= 0) key = ((uint64_t)(uint32_t)d.int_rx[r] << 30) | box_code(prim_box(d, tri), d.scene_box);
     else key = ((uint64_t)(uint32_t)x << 30) | d.codes[x];
     keys[x] = key;
     vals[x] = tri;

@@ -89,6 +89,10 @@ typedef struct bvhcd_config {
     int32_t front_cycle;     /* front cycle, 0 = 6 (BvttFrontSettings.h:105) */
     int32_t device;          /* HIP device ordinal */
     void*   stream;          /* hipStream_t to run on; NULL = the library creates one */
+    int32_t skip_adjacent;   /* 1 = never report pairs of triangles that share a vertex (the simulator
+                                build's covertex filter at every pair emission, e.g.
+                                BvttFrontKeepKernels.cu:33-35,52-55); 0 = report them (default: the
+                                triangle test reports them as intersecting) */
 } bvhcd_config;
 
 typedef struct bvhcd_stats {
@@ -108,6 +112,9 @@ typedef struct bvhcd_stats {
     double  stage_ms[8];       /* last step: 0 build, 1 refit, 2 restructure, 3 front/broad,
                                   4 ordering + quality, 5 narrow, 6 scheduler/MLP, 7 total */
     double  stage_bytes[8];    /* algorithmic HBM bytes of those stages in the last step */
+    int64_t implicit_pairs;    /* of bp_pairs (and of np_pairs with a narrow phase): KEEP's pairs of
+                                  triangles sharing a vertex, counted from the front's head without
+                                  a test (every coordinate finite: both phases report them anyway) */
 } bvhcd_stats;
 
 typedef struct bvhcd_ctx bvhcd_ctx;
@@ -120,6 +127,14 @@ const char* bvhcd_last_error(const bvhcd_ctx* ctx);
 int  bvhcd_set_topology(bvhcd_ctx* ctx, const int32_t* faces, int64_t num_tris, int64_t num_verts, int where);
 /* xyz: V x float[3]. Copied into the context (host: H2D, device: D2D) on the context stream. */
 int  bvhcd_update_vertices(bvhcd_ctx* ctx, const float* xyz, int64_t num_verts, int where);
+/* Box mode, the simulator-facing input of the reference (ARCSimSceneData, world/Scene.h:39-46;
+   LBvhFixedDeformable::maintain, LBvhFixedDeformable.cu:89-112; calcMaxBVARCSim / calcMCsARCSim,
+   LBvhBuildKernels.cu:10-33; refitExtNodeARCSim, LBvhUpdateKernels.cu:10-33): per-triangle boxes
+   (num_tris x 6 floats: min x y z, max x y z, in triangle order) replace the vertices for the
+   scene box, the Morton codes, builds, restructures and refits of the following steps, until the
+   next bvhcd_update_vertices. Boxes with min > max on an axis overlap nothing on it. The narrow
+   phase needs vertices: steps in box mode return BVHCD_E_INVALID when it is enabled. */
+int  bvhcd_update_boxes(bvhcd_ctx* ctx, const float* boxes, int64_t num_tris, int where);
 
 /* One frame with explicit ops (LBvh::maintain then BvttFront::maintain). Stream-ordered. */
 int  bvhcd_step(bvhcd_ctx* ctx, int bvh_op, int front_op);
@@ -207,12 +222,21 @@ int  bvhcd_batch_pair_counts(bvhcd_batch* b, int64_t* counts /* num_scenes */);
 const char* bvhcd_batch_last_error(const bvhcd_batch* b);
 
 /* ---- autotuner (replaces CudaDevice::launchConfig / evalOptimalBlockSize, CudaDevice.cu:60-94) ----
-   Block sizes are fixed at 256 (wave64 x 4); the grids of the grid-stride and persistent kernels
-   are tuned by timing candidates on the current data, per kernel and size class ceil(log2 n),
-   in a process-wide table used by every later launch. */
+   Block sizes are fixed at 256 (wave64 x 4); the launch shapes of the grid-stride and persistent
+   kernels (grid size, and for KEEP the entries per thread per loop iteration) are tuned by timing
+   candidates on the current data, per kernel and size class ceil(log2 n), in a process-wide
+   table used by every later launch. */
+#define BVHCD_TUNE_KEEP      0
+#define BVHCD_TUNE_NARROW    1
+#define BVHCD_TUNE_BATCH_CD  2
 /* KEEP and the narrow phase of a context with a built BVH and front; leaves the current front's
    pairs in place. */
 int  bvhcd_autotune(bvhcd_ctx* ctx);
+/* Sets the shape of one kernel for the size class of n (n = the scene's triangles for KEEP and
+   the batched traversal, the context's triangle capacity for the narrow phase): grid in [1, 2^24),
+   unroll 1, 2 or 4 for KEEP, 1 elsewhere. Stands where CudaDevice::launchConfig
+   (system/CudaDevice/CudaDevice.cu:85-94) fixes a kernel's configuration by name. */
+int  bvhcd_autotune_set(int kernel, int64_t n, uint32_t grid, uint32_t unroll);
 /* The persistent batched traversal; drops the captured frame graphs (re-captured on next use). */
 int  bvhcd_batch_autotune(bvhcd_batch* b);
 /* Text table of every timed candidate (kernel, size class, grid, ms, chosen); returns its

@@ -127,3 +127,30 @@ def test_autotune_keeps_results(pkg, oracle):
     rep = pkg.autotune_report()
     assert "k_keep" in rep and "k_narrow" in rep and "k_batch_cd" in rep and "*" in rep
     ctx.close()
+
+
+@pytest.mark.parametrize("grid,unroll", [(64, 1), (1024, 2), (3000, 4), (1, 4)])
+def test_keep_launch_shapes(pkg, oracle, grid, unroll):
+    """KEEP with every unroll factor and the narrow phase with the same grids (down to one
+    workgroup, and sizes that leave partial iterations) give the brute-force pair sets; the
+    default shapes are restored afterwards."""
+    print(f"[gpu] keep/narrow shape grid {grid} unroll {unroll}", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=96)
+    ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts)
+    ctx.set_topology(sc.faces(), sc.num_verts)
+    ctx.update_vertices(sc.frame(100))
+    ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
+    try:
+        pkg.autotune_set(pkg.TUNE_KEEP, sc.num_tris, grid, unroll)
+        pkg.autotune_set(pkg.TUNE_NARROW, sc.num_tris, grid, 1)
+        for fr in (101, 102):
+            ctx.update_vertices(sc.frame(fr))
+            ctx.step(pkg.BVH_REFIT, pkg.FRONT_KEEP)
+            ref = oracle.pair_set(oracle.bruteforce_pairs(sc.faces(), sc.frame(fr), True))
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), ref, err_msg=f"frame {fr}")
+            ref_bp = oracle.pair_set(oracle.bruteforce_pairs(sc.faces(), sc.frame(fr), False))
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), ref_bp, err_msg=f"bp frame {fr}")
+    finally:
+        pkg.autotune_set(pkg.TUNE_KEEP, sc.num_tris, 2048, 1)
+        pkg.autotune_set(pkg.TUNE_NARROW, sc.num_tris, 4096, 1)
+        ctx.close()

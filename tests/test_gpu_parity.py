@@ -116,6 +116,32 @@ def test_explicit_ops_parity(pkg, oracle):
     ctx.close()
 
 
+@pytest.mark.parametrize("name,frames", [
+    ("cloth1024", (0, 37)),   # 2,093,058 triangles: ~4K top-tree nodes, merged in LDS
+    ("layers4", (0, 9)),      # 8,372,232 triangles: ~16K top-tree nodes, merged through the records
+])
+def test_refit_large_trees(pkg, oracle, name, frames):
+    """Treelet-chunk refit and top-tree merge at full size, after a build and after a later
+    refit: leaf boxes equal the per-triangle min/max, internal boxes equal the oracle's refit
+    over the exported topology."""
+    print(f"[gpu] refit {name} frames {frames}", flush=True)
+    sc = pkg.Scene.preset(name)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc, narrow_phase=False)
+    for k, fr in enumerate(frames):
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        ctx.step(pkg.BVH_BUILD if k == 0 else pkg.BVH_REFIT, pkg.FRONT_PURE_BVH_CD)
+        b = ctx.export_bvh()
+        tv = v[f[b["prim_idx"]]]
+        lb = np.concatenate([tv.min(1), tv.max(1)], 1)
+        assert np.array_equal(b["leaf_box"], lb), f"{name} frame {fr} leaf boxes"
+        ref = oracle.refit_boxes({key: b[key] for key in ("int_lc", "int_rc", "int_mark")}, lb)
+        assert np.array_equal(b["int_box"], ref), f"{name} frame {fr} internal boxes"
+        del tv, lb, ref, b
+    ctx.close()
+
+
 @pytest.mark.parametrize("T", [1, 2, 3, 7])
 def test_tiny_scenes(pkg, oracle, T):
     print(f"[gpu] tiny T={T}", flush=True)
@@ -175,6 +201,131 @@ def test_huge_and_shared_vertices(pkg, oracle):
     orc.step(0, 1)
     np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), oracle.pair_set(orc.pairs(False)))
     np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), oracle.pair_set(orc.pairs(True)))
+    ctx.close()
+
+
+def test_keep_shared_vertex_pairs(pkg, oracle):
+    """KEEP counts the front's shared-vertex entries instead of testing them while every
+    coordinate is finite: pair sets equal brute force on finite frames, on a frame with a vertex
+    beyond 1e8 (where the shortcut is off and the full tests decide) and after it; the host
+    read-out still returns every pair when listed + implicit pairs exceed the pair buffer."""
+    print("[gpu] keep shared-vertex pairs", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=96)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc)
+    v = sc.frame(100)
+    ctx.update_vertices(v)
+    ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
+    for fr, huge in ((101, False), (102, True), (103, False)):
+        v = sc.frame(fr)
+        if huge:
+            v[5] = np.float32(2e8)
+        ctx.update_vertices(v)
+        ctx.step(pkg.BVH_REFIT, pkg.FRONT_KEEP)
+        st = ctx.stats()
+        if huge:
+            assert st["implicit_pairs"] == 0
+        else:
+            assert 0 < st["implicit_pairs"] <= st["np_pairs"] <= st["bp_pairs"]
+        for narrow in (False, True):
+            ref = oracle.pair_set(oracle.bruteforce_pairs(f, v, narrow))
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(narrow)), ref, err_msg=f"frame {fr} {narrow}")
+    st = ctx.stats()
+    listed = st["bp_pairs"] - st["implicit_pairs"]
+    ctx.close()
+    # a pair buffer that holds the listed pairs but not listed + implicit
+    cap = max(2 * listed, 64 * 256)
+    assert cap < st["bp_pairs"], (cap, st["bp_pairs"])
+    ctx = _ctx(pkg, sc, pair_cap=cap)
+    ctx.update_vertices(sc.frame(100))
+    ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
+    ctx.update_vertices(sc.frame(103))
+    ctx.step(pkg.BVH_REFIT, pkg.FRONT_KEEP)
+    for narrow in (False, True):
+        ref = oracle.pair_set(oracle.bruteforce_pairs(f, sc.frame(103), narrow))
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(narrow)), ref, err_msg=f"small buffer {narrow}")
+    ctx.close()
+
+
+def _share_vertex(faces, pairs):
+    """Rows of pairs (triangle ids) whose triangles share a vertex (numpy covertex)."""
+    if len(pairs) == 0:
+        return np.zeros(0, bool)
+    a, b = faces[pairs[:, 0]], faces[pairs[:, 1]]
+    return (a[:, :, None] == b[:, None, :]).any(axis=(1, 2))
+
+
+@pytest.mark.parametrize("ops", ["plain", "explicit"])
+def test_skip_adjacent_filter(pkg, oracle, ops):
+    """skip_adjacent drops exactly the pairs of triangles sharing a vertex, at every emission
+    point (traversal, UPDATE sprouts, KEEP, REORDER), from both lists: pair sets equal brute force
+    minus the vertex-sharing pairs."""
+    print(f"[gpu] skip adjacent {ops}", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=64, frames=40)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc, skip_adjacent=True)
+    seq = [(pkg.BVH_BUILD, pkg.FRONT_PURE_BVH_CD), (pkg.BVH_BUILD, pkg.FRONT_GENERATE),
+           (pkg.BVH_REFIT, pkg.FRONT_KEEP), (pkg.BVH_REFIT, pkg.FRONT_UPDATE), (pkg.BVH_REFIT, pkg.FRONT_KEEP),
+           (pkg.BVH_REFIT, pkg.FRONT_REORDER), (pkg.BVH_REFIT, pkg.FRONT_KEEP)]
+    for k in range(len(seq) if ops == "explicit" else 30):
+        fr = 5 + k
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        if ops == "explicit":
+            ctx.step(*seq[k])
+        else:
+            ctx.step_scheduled()
+        st = ctx.stats()
+        assert st["implicit_pairs"] == 0
+        for narrow in (False, True):
+            ref = oracle.pair_set(oracle.bruteforce_pairs(f, v, narrow))
+            ref = ref[~_share_vertex(f, ref)]
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(narrow)), ref, err_msg=f"frame {fr} {narrow}")
+    ctx.close()
+
+
+def test_box_mode(pkg, oracle):
+    """Per-triangle boxes given directly (the simulator-facing input): broad-phase pairs equal a
+    brute-force overlap of those boxes after a build, refits, KEEP / UPDATE fronts and a plain
+    schedule; the narrow phase is refused in box mode; vertices switch back to vertex mode."""
+    print("[gpu] box mode", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=32, frames=40)
+    f = sc.faces()
+    T = sc.num_tris
+    rng = np.random.default_rng(21)
+
+    def boxes_of(fr):
+        tv = sc.frame(fr)[f]
+        lo, hi = tv.min(1), tv.max(1)
+        pad = (rng.random((T, 1)) * 2e-3).astype(np.float32)  # swept-box style inflation
+        return np.ascontiguousarray(np.concatenate([lo - pad, hi + pad], 1).astype(np.float32))
+
+    def brute(b):
+        lo, hi = b[:, :3], b[:, 3:]
+        i, j = np.triu_indices(T, 1)
+        ok = np.all((lo[i] <= hi[j]) & (lo[j] <= hi[i]), axis=1)
+        return oracle.pair_set(np.stack([i[ok], j[ok]], 1).astype(np.int32))
+
+    ctx = _ctx(pkg, sc, narrow_phase=False)
+    ops = [(pkg.BVH_BUILD, pkg.FRONT_GENERATE), (pkg.BVH_REFIT, pkg.FRONT_KEEP), (pkg.BVH_REFIT, pkg.FRONT_UPDATE),
+           (pkg.BVH_REFIT, pkg.FRONT_KEEP), (pkg.BVH_BUILD, pkg.FRONT_PURE_BVH_CD)]
+    for k, op in enumerate(ops):
+        b = boxes_of(3 + k)
+        ctx.update_boxes(b)
+        ctx.step(*op)
+        assert ctx.stats()["implicit_pairs"] == 0
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), brute(b), err_msg=f"op {k}")
+    ctx.close()
+    ctx = _ctx(pkg, sc)  # narrow phase on: refused in box mode, fine with vertices again
+    ctx.update_boxes(boxes_of(0))
+    with pytest.raises(pkg.BvhcdError) as e:
+        ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
+    assert e.value.code == pkg.E_INVALID
+    v = sc.frame(0)
+    ctx.update_vertices(v)
+    ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
+    np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)),
+                                  oracle.pair_set(oracle.bruteforce_pairs(f, v, True)))
     ctx.close()
 
 

@@ -129,3 +129,16 @@ def test_bench_rejects_multi_gpu():
                        timeout=120)
     assert r.returncode != 0
     assert "one GPU" in r.stdout
+
+
+def test_autotune_set_validates_shapes():
+    """Launch-shape override (host-side table only): grid in [1, 2^24), unroll 1/2/4 for KEEP
+    and 1 for the other kernels."""
+    pkg = load_package()
+    for args in [(pkg.TUNE_KEEP, 1000, 0, 1), (pkg.TUNE_KEEP, 1000, 1 << 24, 1), (pkg.TUNE_KEEP, 1000, 64, 3),
+                 (pkg.TUNE_NARROW, 1000, 64, 2), (pkg.TUNE_BATCH_CD, 1000, 64, 2), (7, 1000, 64, 1),
+                 (pkg.TUNE_KEEP, 0, 64, 1)]:
+        with pytest.raises(pkg.BvhcdError):
+            pkg.autotune_set(*args)
+    pkg.autotune_set(pkg.TUNE_KEEP, 1000, 512, 4)
+    pkg.autotune_set(pkg.TUNE_KEEP, 1000, 2048, 1)
