@@ -120,6 +120,7 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.rp_tlev, 1024);
     ok &= dalloc(c, d.rp_tinfo, 2);
     ok &= dalloc(c, d.rp_tcount, 1);
+    ok &= dalloc(c, d.rp_tcb, 6 * 2 * 6144);
     ok &= dalloc(c, d.rp_cbad, (size_t)refit_chunks((int32_t)T));
     ok &= dalloc(c, d.leaf_par, T);
     ok &= dalloc(c, d.leaf_mark, T);
@@ -474,6 +475,7 @@ int run_front(bvhcd_ctx* c, int& front_op) {
         case BVHCD_FRONT_RESTRUCTURE:
             begin(c, S_FRONT);
             launch_front_restr_apply(d, n, c->repair_tasks, s);
+            if (!order) launch_query_order(d, n, s);
             end(c, S_FRONT);
             c->repair_pending = false;
             break;

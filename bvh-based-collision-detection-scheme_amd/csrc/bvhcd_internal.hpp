@@ -167,6 +167,8 @@ struct DevState {
     int32_t* rp_tinfo = nullptr;    // 2: (top nodes, deepest level or -1)
     uint32_t* rp_tcount = nullptr;  // 1: append counter
     uint32_t* rp_cbad = nullptr;    // chunks: some leaf of the chunk has a non-finite coordinate
+    float* rp_tcb = nullptr;        // 2 x 6144 boxes: treelet-root children of the top tree by
+                                    // (list position, side), written by the chunk refit
     float* sa_ref = nullptr;        // T-1 internal-node surface area at its last rebuild
     float* sa_ref_leaf = nullptr;   // T leaf-box surface area at its last rebuild
     int32_t* built_frame = nullptr; // T frame at which each leaf's subtree was last rebuilt
@@ -304,6 +306,23 @@ inline unsigned grid_for(int64_t n, int block = kBlock) {
     int64_t g = (n + block - 1) / block;
     if (g < 1) g = 1;
     return (unsigned)g;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2. xcd_loop gives
+// workgroup b the XCD b % 8 and the rank b / 8 inside it, so that every XCD sweeps one
+// contiguous eighth of [0, total) with a grid-stride loop over its own workgroups: on a
+// spatially ordered list, an XCD's L2 then holds the nodes of one region instead of every L2
+// fetching every node. Grids that are not a multiple of 8 fall back to the plain loop.
+constexpr uint32_t kXcds = 8;
+struct XcdLoop {
+    uint32_t begin, end, stride;
+};
+__device__ inline XcdLoop xcd_loop(uint32_t total) {
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    if (g % kXcds != 0) return {b * blockDim.x + threadIdx.x, total, g * blockDim.x};
+    const uint32_t per = total / kXcds + 1;
+    const uint32_t x = b % kXcds, lo = min(x * per, total), hi = min(lo + per, total);
+    return {lo + (b / kXcds) * blockDim.x + threadIdx.x, hi, (g / kXcds) * blockDim.x};
 }
 
 }  // namespace bvhcd

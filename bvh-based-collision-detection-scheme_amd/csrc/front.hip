@@ -87,9 +87,14 @@ __device__ void stage_flush(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, 
     }
     if (over) atomicOr(ovf, sk.ovf_bit);
 }
-// Segmented lists: successive flushes of a wave rotate over the segments (the rotor starts at
-// the wave's global index), so the appends spread over kPairSegs counters and every segment
-// fills evenly whichever workgroups produce the entries.
+// Segmented lists: successive flushes of a wave rotate over the segments, so the appends
+// spread over kPairSegs counters and every segment fills evenly whichever workgroups produce
+// the entries. The rotor start spreads the waves of one XCD (workgroups b with equal b % 8,
+// which a per-XCD split makes neighbours in the data) over all segments too.
+__device__ inline uint32_t rotor_start() {
+    const uint32_t b = blockIdx.x, w = threadIdx.x >> 6;
+    return (b / kXcds) * kWaves + w + (b % kXcds) * kXcds;
+}
 __device__ inline uint32_t seg_reserve(uint32_t* cnt, volatile uint32_t* sfill, uint32_t f, uint64_t active,
                                        uint32_t& g) {
     const int leader = __ffsll((long long)active) - 1;
@@ -175,7 +180,7 @@ __device__ inline void stages_init(Stages& st) {
     const int w = threadIdx.x >> 6;
     if (lane_id() < 3) {
         st.fill[lane_id()][w][0] = 0;
-        st.fill[lane_id()][w][1] = blockIdx.x * kWaves + w;
+        st.fill[lane_id()][w][1] = rotor_start();
     }
 }
 
@@ -223,7 +228,7 @@ __device__ inline void stages_init(StagesBP& st) {
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) {
         st.fill[w][0] = 0;
-        st.fill[w][1] = blockIdx.x * kWaves + w;
+        st.fill[w][1] = rotor_start();
     }
 }
 __device__ inline Out make_out(StagesBP& st, const DevState& d) {
@@ -560,6 +565,27 @@ static void sort_front(DevState& d, int kind, int64_t max_id, hipStream_t s) {
     }
 }
 
+// Fronts of the learned schedule are not node-ordered; they are sorted by query leaf instead
+// (stable, so each query keeps its emission order): neighbouring queries read neighbouring
+// nodes, which KEEP's per-XCD split then keeps inside one L2.
+static void sort_front_by_query(DevState& d, int kind, int32_t n, hipStream_t s) {
+    const int c = d.fcur[kind], o = c ^ 1;
+    const bool in_tmp = radix_sort_u32_dev(d, reinterpret_cast<uint32_t*>(d.fquery[kind][c]),
+                                           reinterpret_cast<int32_t*>(d.fnode[kind][c]),
+                                           reinterpret_cast<uint32_t*>(d.fquery[kind][o]),
+                                           reinterpret_cast<int32_t*>(d.fnode[kind][o]), d.front_cap[kind],
+                                           d.fsize + 2 * kind + c, bits_for(n - 1), s);
+    if (in_tmp) {
+        hipMemcpyAsync(d.fsize + 2 * kind + o, d.fsize + 2 * kind + c, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        d.fcur[kind] = o;
+    }
+}
+void launch_query_order(DevState& d, int32_t n, hipStream_t s) {
+    if (n < 2) return;
+    sort_front_by_query(d, 0, n, s);
+    sort_front_by_query(d, 1, n, s);
+}
+
 // The reference's FlOrderLog::prepare + counting-sort scatter: here a radix sort per kind,
 // then intbeg[0..n-1] (internal ids) and extbeg[0..n] (leaves) from the sorted node ids.
 void launch_order_fronts(DevState& d, int32_t n, hipStream_t s) {
@@ -579,6 +605,7 @@ void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order) {
     front_finish(d, 0, false, s);
     front_finish(d, 1, false, s);
     if (order && n >= 2) launch_order_fronts(d, n, s);
+    if (!order) launch_query_order(d, n, s);
 }
 
 // ---------------------------------------------------------------- UPDATE (prune / sprout)
@@ -692,6 +719,7 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool orde
         hipLaunchKernelGGL(k_quality, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d);
     }
     if (order && n >= 2) launch_order_fronts(d, n, s);
+    if (!order) launch_query_order(d, n, s);
 }
 
 // ---------------------------------------------------------------- KEEP
@@ -715,19 +743,21 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int imp
     const uint32_t total = ni + ne;
     const uint32_t* ext_node = in.node[1] + skip;
     const int32_t* ext_query = in.query[1] + skip;
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const XcdLoop it = xcd_loop(total);
+    const uint32_t stride = it.stride;
     uint32_t visits = 0;
-    for (uint32_t k0 = blockIdx.x * blockDim.x + threadIdx.x; k0 < total; k0 += U * stride) {
+    for (uint32_t k0 = it.begin; k0 < it.end; k0 += U * stride) {
         int32_t q[U], nd[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t k = k0 + u * stride;
             q[u] = -1;
             nd[u] = 0;
+            if (k >= it.end) continue;
             if (k < ni) {
                 q[u] = in.query[0][k];
                 nd[u] = (int32_t)in.node[0][k];
-            } else if (k < total) {
+            } else {
                 q[u] = ext_query[k - ni];
                 nd[u] = (int32_t)ext_node[k - ni];
             }
@@ -1080,7 +1110,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) {
         sfill[w][0] = 0;
-        sfill[w][1] = blockIdx.x * kWaves + w;
+        sfill[w][1] = rotor_start();
     }
     const SegPairSink sink = pair_segments(d, 1, 8u);
     const uint32_t npairs = seg_prefix(d.pcnt, (uint32_t)d.pair_seg_cap, pre);

@@ -705,6 +705,8 @@ constexpr int kChunkCap = 2 * kTreelet;
 constexpr int kRefitPer = kChunkCap / kBlock;  // entries (leaves, internal nodes) per thread
 constexpr int kDepthCap = 1024;
 constexpr int kTopBlock = 1024;
+constexpr int kTopLds = 6144;  // top trees up to this size are merged in LDS
+constexpr int kTopPer = kTopLds / kTopBlock;
 static_assert(kChunkCap % kBlock == 0, "chunk capacity is a whole number of block passes");
 static_assert(2 * kChunkCap <= 65536, "slots are packed in 16 bits");
 
@@ -926,6 +928,19 @@ __global__ void __launch_bounds__(kBlock) k_refit(int32_t n, DevState d) {
 #pragma unroll
     for (int k = 0; k < kRefitPer; ++k)
         if (v[k].w >= 0) store_box(&d.inode[v[k].x], out[k]);
+    // treelet roots under an LDS-sized top tree also hand their box to the top kernel, in the
+    // slot of their parent's list position and side: it loads them without a dependent load
+    if (d.rp_tinfo[0] <= kTopLds) {
+#pragma unroll
+        for (int k = 0; k < kRefitPer; ++k) {
+            if (v[k].w != 0) continue;
+            const int4 tp = d.itopo[v[k].x];  // (lc, rc, par, mark); mark bit 2: right child
+            if (tp.z < 0) continue;
+            float* dst = d.rp_tcb + 6 * (2 * (int64_t)d.rp_tpos[tp.z] + ((tp.w & 4) ? 1 : 0));
+            dst[0] = out[k].mnx; dst[1] = out[k].mny; dst[2] = out[k].mnz;
+            dst[3] = out[k].mxx; dst[4] = out[k].mxy; dst[5] = out[k].mxz;
+        }
+    }
 }
 
 __device__ inline Box record_box(const DevState& d, int32_t v, int kind) {
@@ -938,8 +953,6 @@ __device__ inline Box record_box(const DevState& d, int32_t v, int kind) {
 // one such child starts its slot with that child's box and merges the other at its level.
 // Larger top trees go level by level through the node records the workgroup writes itself
 // (the barrier orders them at workgroup scope).
-constexpr int kTopLds = 6144;
-constexpr int kTopPer = kTopLds / kTopBlock;
 
 __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chunks) {
     __shared__ Box tb[kTopLds];
@@ -957,17 +970,24 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
             const int32_t i = threadIdx.x + k * kTopBlock;
             e[k] = i < m ? d.rp_tlist[i] : make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
         }
-        // the record loads of a group of entries all in flight before its merges (children below
-        // the top tree); two groups keep the registers within the 1024-thread budget
+        // children below the top tree: treelet roots were handed over by k_refit in the entry's
+        // own slots (loaded without waiting for the entry), leaves come from their records; a
+        // group's loads all in flight before its merges, two groups keep the register budget
         constexpr int kGroup = kTopPer / 2;
 #pragma unroll
         for (int g = 0; g < kTopPer; g += kGroup) {
             Box cl[kGroup], cr[kGroup];
 #pragma unroll
             for (int k = 0; k < kGroup; ++k) {
+                const int32_t i = threadIdx.x + (g + k) * kTopBlock;
+                cl[k] = i < m ? given_box(d.rp_tcb, 2 * (int64_t)i) : Box{};
+                cr[k] = i < m ? given_box(d.rp_tcb, 2 * (int64_t)i + 1) : Box{};
+            }
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
                 const int lk = top_kind(e[g + k], 0), rk = top_kind(e[g + k], 1);
-                cl[k] = lk != 2 ? record_box(d, e[g + k].y, lk) : Box{};
-                cr[k] = rk != 2 ? record_box(d, e[g + k].z, rk) : Box{};
+                if (lk == 1) cl[k] = record_box(d, e[g + k].y, 1);
+                if (rk == 1) cr[k] = record_box(d, e[g + k].z, 1);
             }
 #pragma unroll
             for (int k = 0; k < kGroup; ++k) {
