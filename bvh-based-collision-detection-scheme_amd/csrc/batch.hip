@@ -252,7 +252,7 @@ void record_frame(bvhcd_batch* b, int bvh_op, unsigned grid) {
     }
     launch_refit(d, n, s);
     launch_batch_cd(d, n, b->keys, b->scene_last, b->queue, grid, s);
-    launch_narrow(d, s);
+    launch_narrow(d, s, true);  // launch_batch_cd reset the counters
     launch_pair_totals(d, s, true);
     fill_u32(b->scnt, b->S + 1, 0u, s);
     fill_u32(b->scur, b->S + 1, 0u, s);

@@ -408,10 +408,11 @@ static FrontIn cur_front(DevState& d) {
 __host__ __device__ inline FrontSink null_sink() { return FrontSink{nullptr, nullptr, nullptr, 0u, 0u}; }
 
 // One 64-lane launch (a kernel, not memsets: the batched frame graph records it): the per-step
-// pair counters, the visit counter and the broad-phase segment counters.
+// pair counters, the visit counter and the segment counters of both pair lists.
 __global__ void k_reset_step(DevState d) {
     const int g = threadIdx.x;
-    d.pcnt[g * kCntStride] = 0u;
+    d.pcnt[g * kCntStride] = 0u;                // broad-phase segment counters
+    d.pcnt[(kPairSegs + g) * kCntStride] = 0u;  // narrow-phase segment counters
     if (g <= C_IMPL_NP) d.ctr[g] = 0u;  // C_BP, C_NP, C_OVF, C_IMPLICIT, C_IMPL_NP
     if (g == 8) ctr64(d.ctr)[C64_VISITS] = 0ull;
 }
@@ -1143,9 +1144,9 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
     stage_final(sink, &d.ctr[C_OVF], sbuf[w], sfill[w]);
 }
 
-void launch_narrow(DevState& d, hipStream_t s) {
+void launch_narrow(DevState& d, hipStream_t s, bool counters_zeroed) {
     // kernels, not memsets: the batched frame graph records this
-    fill_u32(d.pcnt + kPairSegs * kCntStride, kPairSegs * kCntStride, 0u, s);
+    if (!counters_zeroed) fill_u32(d.pcnt + kPairSegs * kCntStride, kPairSegs * kCntStride, 0u, s);
     hipLaunchKernelGGL(k_narrow, dim3(shape_grid(tuned_shape(TK_NARROW, d.maxT))), dim3(kBlock), 0, s, d);
 }
 

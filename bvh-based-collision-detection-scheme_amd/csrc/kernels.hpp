@@ -65,7 +65,7 @@ void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream
 void launch_order_fronts(DevState& d, int32_t n, hipStream_t s);      // radix sort by node + bucket starts
 void launch_query_order(DevState& d, int32_t n, hipStream_t s);       // stable radix sort by query leaf
 void launch_snapshot(DevState& d, int32_t n, hipStream_t s);
-void launch_narrow(DevState& d, hipStream_t s);
+void launch_narrow(DevState& d, hipStream_t s, bool counters_zeroed = false);  // true: right after a step's reset
 // Segmented pair lists: totals (and largest segments) into ctr[C_BP / C_NP / *_SEGMAX]; clearing
 // the narrow-phase list; a contiguous copy of list `kind` (0 broad phase, mapped from leaf to
 // triangle ids; 1 narrow phase) of at most cap pairs into dst.

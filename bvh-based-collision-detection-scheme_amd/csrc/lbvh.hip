@@ -691,10 +691,10 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
 //  * preorder numbering gives the internal nodes of a treelet with root r and leaves [a, b]
 //    the ids [r, r + b - a): node p takes slot p - r + a, slot b stays empty, and the slots of
 //    a chunk are its own leaf range;
-//  * each chunk lists its internal nodes deepest first (depth below the treelet root) with
-//    the LDS slots of the node and its two children (leaf x: x - chunk start; internal node:
-//    slot - chunk start + kChunkCap); the top tree is listed deepest first (depth below the
-//    root), children as internal ids or ~leaf ids.
+//  * each chunk lists its internal nodes deepest first (depth below the treelet root) as
+//    (node, LDS slot | depth << 16, child slots, top-kernel slot or -1), slots being leaf x:
+//    x - chunk start, internal node: slot - chunk start + kChunkCap; the top tree is listed
+//    deepest first (depth below the root) with its children's kinds.
 // Per refit: one workgroup per chunk writes the leaf boxes into LDS, merges one depth level
 // per barrier in LDS and writes its node boxes at the end; then one workgroup merges the top
 // tree level by level through the node records it writes itself (workgroup-coherent).
@@ -705,6 +705,8 @@ constexpr int kChunkCap = 2 * kTreelet;
 constexpr int kRefitPer = kChunkCap / kBlock;  // entries (leaves, internal nodes) per thread
 constexpr int kDepthCap = 1024;
 constexpr int kTopBlock = 1024;
+constexpr int kRefitBlock = 512;                       // per-frame chunk refit: 8 waves per chunk
+constexpr int kRefitPerT = kChunkCap / kRefitBlock;     // its leaves / nodes per thread
 constexpr int kTopLds = 6144;  // top trees up to this size are merged in LDS
 constexpr int kTopPer = kTopLds / kTopBlock;
 static_assert(kChunkCap % kBlock == 0, "chunk capacity is a whole number of block passes");
@@ -741,7 +743,7 @@ __global__ void __launch_bounds__(kBlock) k_rp_starts(int32_t n, DevState d) {
 __global__ void __launch_bounds__(kBlock) k_rp_walk(int32_t n, DevState d) {
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
-    const int4 empty = make_int4(-1, 0, 0, -1);
+    const int4 empty = make_int4(-1, -1, 0, -1);
     const int32_t p = d.leaf_par[x];
     int32_t len = 0, r = -1;
     for (int32_t q = p; q >= 0 && len < kTreelet && !rp_is_top(d, q); q = d.int_par[q]) {
@@ -756,12 +758,17 @@ __global__ void __launch_bounds__(kBlock) k_rp_walk(int32_t n, DevState d) {
     if (x == d.int_ry[r]) d.rp_ent[x] = empty;
     const int32_t cs = d.rp_cstart[a / kTreelet];
     const int32_t base = a - r - cs + kChunkCap;  // LDS slot of internal node u: u + base
+    const bool lds_top = d.rp_tinfo[0] <= kTopLds;
     int32_t q = p;
     for (int32_t i = 0; i < len && d.int_rx[q] == x; ++i) {
         const int4 tp = d.itopo[q];
         const int32_t ls = (tp.w & 1) ? tp.x - cs : tp.x + base;
         const int32_t rs = (tp.w & 2) ? tp.y - cs : tp.y + base;
-        d.rp_ent[q - r + a] = make_int4(q, q + base, ls | (rs << 16), len - 1 - i);
+        // the treelet root hands its box to the top kernel: slot of its parent's list position
+        // and side (mark bit 2: right child)
+        int32_t dest = -1;
+        if (q == r && tp.z >= 0 && lds_top) dest = 2 * d.rp_tpos[tp.z] + ((tp.w & 4) ? 1 : 0);
+        d.rp_ent[q - r + a] = make_int4(q, (q + base) | ((len - 1 - i) << 16), ls | (rs << 16), dest);
         q = tp.z;
     }
 }
@@ -779,10 +786,10 @@ __global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
 #pragma unroll
     for (int k = 0; k < kRefitPer; ++k) {
         const int32_t i = s + threadIdx.x + k * kBlock;
-        v[k] = i < e ? d.rp_ent[i] : make_int4(-1, 0, 0, -1);
-        if (v[k].w >= 0) {
-            atomicAdd(&hist[v[k].w], 1u);
-            atomicMax(&dmax, (uint32_t)v[k].w);
+        v[k] = i < e ? d.rp_ent[i] : make_int4(-1, -1, 0, -1);
+        if (v[k].x >= 0) {
+            atomicAdd(&hist[v[k].y >> 16], 1u);
+            atomicMax(&dmax, (uint32_t)(v[k].y >> 16));
         }
     }
     __syncthreads();
@@ -798,7 +805,7 @@ __global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kRefitPer; ++k)
-        if (v[k].w >= 0) d.rp_ent[s + atomicAdd(&hist[v[k].w], 1u)] = v[k];
+        if (v[k].x >= 0) d.rp_ent[s + atomicAdd(&hist[v[k].y >> 16], 1u)] = v[k];
 }
 
 // Top tree sorted deepest first; level t occupies [rp_tlev[t], t ? rp_tlev[t-1] : count).
@@ -862,9 +869,9 @@ void launch_refit_prep(DevState& d, int32_t n, hipStream_t s) {
     fill_i32(d.rp_cstart, nc + 1, n, s);
     fill_u32(d.rp_tcount, 1, 0u, s);
     hipLaunchKernelGGL(k_rp_starts, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d);
+    hipLaunchKernelGGL(k_rp_top_sort, dim3(1), dim3(kTopBlock), 0, s, d);  // positions, read by the walk
     hipLaunchKernelGGL(k_rp_walk, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d);
     hipLaunchKernelGGL(k_rp_chunk_sort, dim3(nc), dim3(kBlock), 0, s, d);
-    hipLaunchKernelGGL(k_rp_top_sort, dim3(1), dim3(kTopBlock), 0, s, d);
 }
 
 __device__ inline void store_box(NodeRec* rec, const Box& b) {
@@ -873,7 +880,7 @@ __device__ inline void store_box(NodeRec* rec, const Box& b) {
     p[1].x = b.mxx; p[1].y = b.mxy; p[1].z = b.mxz;
 }
 
-__global__ void __launch_bounds__(kBlock) k_refit(int32_t n, DevState d) {
+__global__ void __launch_bounds__(kRefitBlock) k_refit(int32_t n, DevState d) {
     __shared__ Box sb[2 * kChunkCap];
     const int32_t c = blockIdx.x;
     const int32_t s = d.rp_cstart[c], e = d.rp_cstart[c + 1];
@@ -881,8 +888,8 @@ __global__ void __launch_bounds__(kBlock) k_refit(int32_t n, DevState d) {
     // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase
     int bad = 0;
 #pragma unroll
-    for (int k = 0; k < kRefitPer; ++k) {
-        const int32_t x = s + threadIdx.x + k * kBlock;
+    for (int k = 0; k < kRefitPerT; ++k) {
+        const int32_t x = s + threadIdx.x + k * kRefitBlock;
         if (x < e) {
             bool fin = true;
             Box b = empty_box();
@@ -906,37 +913,33 @@ __global__ void __launch_bounds__(kBlock) k_refit(int32_t n, DevState d) {
             store_box(&d.leaf[x], b);
         }
     }
-    int4 v[kRefitPer];
+    int4 v[kRefitPerT];
 #pragma unroll
-    for (int k = 0; k < kRefitPer; ++k) {
-        const int32_t i = threadIdx.x + k * kBlock;
-        v[k] = i < info.x ? d.rp_ent[s + i] : make_int4(-1, 0, 0, -1);
+    for (int k = 0; k < kRefitPerT; ++k) {
+        const int32_t i = threadIdx.x + k * kRefitBlock;
+        v[k] = i < info.x ? d.rp_ent[s + i] : make_int4(-1, -1, 0, -1);
     }
-    Box out[kRefitPer];
+    Box out[kRefitPerT];
     bad = __syncthreads_or(bad);
     if (threadIdx.x == 0) d.rp_cbad[c] = (uint32_t)bad;
     for (int32_t lev = info.y; lev >= 0; --lev) {
 #pragma unroll
-        for (int k = 0; k < kRefitPer; ++k) {
-            if (v[k].w == lev) {
+        for (int k = 0; k < kRefitPerT; ++k) {
+            if ((v[k].y >> 16) == lev) {
                 out[k] = box_merge(sb[v[k].z & 0xFFFF], sb[v[k].z >> 16]);
-                sb[v[k].y] = out[k];
+                sb[v[k].y & 0xFFFF] = out[k];
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < kRefitPer; ++k)
-        if (v[k].w >= 0) store_box(&d.inode[v[k].x], out[k]);
-    // treelet roots under an LDS-sized top tree also hand their box to the top kernel, in the
-    // slot of their parent's list position and side: it loads them without a dependent load
-    if (d.rp_tinfo[0] <= kTopLds) {
-#pragma unroll
-        for (int k = 0; k < kRefitPer; ++k) {
-            if (v[k].w != 0) continue;
-            const int4 tp = d.itopo[v[k].x];  // (lc, rc, par, mark); mark bit 2: right child
-            if (tp.z < 0) continue;
-            float* dst = d.rp_tcb + 6 * (2 * (int64_t)d.rp_tpos[tp.z] + ((tp.w & 4) ? 1 : 0));
+    for (int k = 0; k < kRefitPerT; ++k) {
+        if (v[k].x < 0) continue;
+        store_box(&d.inode[v[k].x], out[k]);
+        // a treelet root under an LDS-sized top tree also hands its box to the top kernel, which
+        // then loads it without a dependent load
+        if (v[k].w >= 0) {
+            float* dst = d.rp_tcb + 6 * (int64_t)v[k].w;
             dst[0] = out[k].mnx; dst[1] = out[k].mny; dst[2] = out[k].mnz;
             dst[3] = out[k].mxx; dst[4] = out[k].mxy; dst[5] = out[k].mxz;
         }
@@ -1039,7 +1042,7 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
 
 void launch_refit(DevState& d, int32_t n, hipStream_t s) {
     if (n < 1) return;
-    hipLaunchKernelGGL(k_refit, dim3(refit_chunks(n)), dim3(kBlock), 0, s, n, d);
+    hipLaunchKernelGGL(k_refit, dim3(refit_chunks(n)), dim3(kRefitBlock), 0, s, n, d);
     hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(kTopBlock), 0, s, d, refit_chunks(n));
 }
 
