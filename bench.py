@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark: collision-detection frames/sec on the 1024x1024 folding cloth (2,093,058 triangles).
 
-One step = one frame of the hot path: vertices of frame f (already in HBM) -> BVH maintenance
+One step = one frame of the hot path: vertices of frame f (already in HBM, read in place as a
+simulator hands its device buffer over: BVHCD_MEM_DEVICE_BORROW) -> BVH maintenance
 (build / refit / restructure) -> BVTT front maintenance -> narrow phase -> colliding pairs.
 The run replays the seeded 400-frame sequence from frame 0 (warm-up frames first, then the
 timed frames), with the plain scheduler (reference STATIC_MANDATORY schedule + restructure
@@ -70,7 +71,7 @@ def frame_digests(pkg, sc, faces, frames_dev, count, scheduler, weights):
     nbytes = sc.num_verts * 12
     out = []
     for f in range(count):
-        ctx.update_vertices(frames_dev.data_ptr() + f * nbytes, pkg.MEM_DEVICE)
+        ctx.update_vertices(frames_dev.data_ptr() + f * nbytes, pkg.MEM_DEVICE_BORROW)
         ctx.step_scheduled()
         out.append(tuple(ctx.digest(True)))
     ctx.close()
@@ -87,7 +88,7 @@ def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights
     base = frames_dev.data_ptr()
     digests = []
     for f in range(warmup):
-        ctx.update_vertices(base + f * nbytes, pkg.MEM_DEVICE)
+        ctx.update_vertices(base + f * nbytes, pkg.MEM_DEVICE_BORROW)
         ctx.step_scheduled()
         if f < digests_upto:
             digests.append(ctx.digest(True))
@@ -102,7 +103,7 @@ def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights
     t0 = time.perf_counter()
     for k in range(steps):
         f = warmup + k
-        ctx.update_vertices(base + f * nbytes, pkg.MEM_DEVICE)
+        ctx.update_vertices(base + f * nbytes, pkg.MEM_DEVICE_BORROW)
         ctx.step_scheduled()
     torch.cuda.synchronize()
     t1 = time.perf_counter()

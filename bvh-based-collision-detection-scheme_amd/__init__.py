@@ -26,7 +26,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbvhcd.so")
 
 OK, E_INVALID, E_OOM, E_CAPACITY, E_DEVICE, E_NOT_BUILT = 0, -1, -2, -3, -4, -5
-MEM_HOST, MEM_DEVICE = 0, 1
+MEM_HOST, MEM_DEVICE, MEM_DEVICE_BORROW = 0, 1, 2
 BVH_BUILD, BVH_REFIT, BVH_RESTRUCTURE, BVH_NONE = 0, 1, 2, 3
 FRONT_PURE_BVH_CD, FRONT_GENERATE, FRONT_UPDATE, FRONT_REORDER, FRONT_KEEP, FRONT_RESTRUCTURE = 0, 1, 2, 3, 4, 5
 SCHED_PLAIN, SCHED_LEARNED, SCHED_GENERATE = 0, 1, 2
@@ -282,11 +282,13 @@ class CollisionContext:
         self.V = int(num_verts)
 
     def update_vertices(self, xyz, where=MEM_HOST):
+        """Host array (copied), or a device pointer: MEM_DEVICE copies it, MEM_DEVICE_BORROW has
+        the following steps read it in place (keep it alive and unchanged meanwhile)."""
         if where == MEM_HOST:
             v = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
             self._chk(lib().bvhcd_update_vertices(self._h, _ptr(v), v.shape[0], MEM_HOST), "update_vertices")
         else:
-            self._chk(lib().bvhcd_update_vertices(self._h, C.c_void_p(int(xyz)), self.V, MEM_DEVICE),
+            self._chk(lib().bvhcd_update_vertices(self._h, C.c_void_p(int(xyz)), self.V, int(where)),
                       "update_vertices")
 
     def update_boxes(self, boxes, where=MEM_HOST):

@@ -53,6 +53,8 @@ struct bvhcd_ctx {
     std::vector<void*> allocs;
     LearnedState* learned = nullptr;
     float* boxes = nullptr;      // box mode input (6 floats per triangle), allocated on first use
+    float* own_verts = nullptr;  // the context's vertex buffer (d.verts may borrow the caller's)
+    bool totals_pending = false; // pair totals of the last step not summed yet (done on demand)
 };
 
 namespace {
@@ -91,6 +93,7 @@ int allocate(bvhcd_ctx* c) {
     bool ok = true;
     ok &= dalloc(c, d.faces, 3 * T);
     ok &= dalloc(c, d.verts, 3 * V);
+    c->own_verts = d.verts;
     ok &= dalloc(c, d.codes, T);
     ok &= dalloc(c, d.keys_tmp, T);
     ok &= dalloc(c, d.keys_tmp2, T);
@@ -234,8 +237,17 @@ __global__ void k_log_frame(DevState d, int cur_int, int cur_ext, uint64_t* out)
     out[7] = d.ctr[C_IMPL_NP];
 }
 
+// Pair totals (C_BP, C_NP and the implicit narrow-phase count) are summed from the segment
+// counters only when something reads them, not on every step.
+void ensure_totals(bvhcd_ctx* c) {
+    if (!c->totals_pending) return;
+    launch_pair_totals(c->d, c->stream, c->cfg.narrow_phase != 0);
+    c->totals_pending = false;
+}
+
 void log_frame(bvhcd_ctx* c, int bvh_op, int front_op) {
     if (!timing_on(c)) return;
+    ensure_totals(c);
     hipLaunchKernelGGL(k_log_frame, dim3(1), dim3(64), 0, c->stream, c->d, c->d.fcur[0], c->d.fcur[1],
                        c->dlog + kLog * c->tn);
     c->tops.push_back(bvh_op);
@@ -483,9 +495,10 @@ int run_front(bvhcd_ctx* c, int& front_op) {
             return fail(c, BVHCD_E_INVALID, "bad front_op");
     }
     begin(c, S_NARROW);
-    if (c->cfg.narrow_phase) launch_narrow(d, s);
-    else clear_np(d, s);
-    launch_pair_totals(d, s, c->cfg.narrow_phase != 0);
+    // every front op starts with the step-counter reset, which also zeroes the narrow-phase
+    // segment counters (without a narrow phase they stay zero)
+    if (c->cfg.narrow_phase) launch_narrow(d, s, true);
+    c->totals_pending = true;
     end(c, S_NARROW);
     return BVHCD_OK;
 }
@@ -626,10 +639,17 @@ int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V,
 int bvhcd_update_vertices(bvhcd_ctx* c, const float* xyz, int64_t V, int where) {
     if (!c) return BVHCD_E_INVALID;
     if (!xyz || V != c->V) return fail(c, BVHCD_E_INVALID, "vertex count differs from set_topology");
+    if (where != BVHCD_MEM_HOST && where != BVHCD_MEM_DEVICE && where != BVHCD_MEM_DEVICE_BORROW)
+        return fail(c, BVHCD_E_INVALID, "bad memory kind");
+    c->d.pbox = nullptr;  // vertex mode
+    if (where == BVHCD_MEM_DEVICE_BORROW) {  // read in place by the following steps
+        c->d.verts = const_cast<float*>(xyz);
+        return BVHCD_OK;
+    }
+    c->d.verts = c->own_verts;
     HIPCHK(c, hipMemcpyAsync(c->d.verts, xyz, sizeof(float) * 3 * V,
                              where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
     if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->d.pbox = nullptr;  // vertex mode
     return BVHCD_OK;
 }
 
@@ -694,6 +714,7 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
 
 int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int where, int filtered) {
     if (!c || !num) return BVHCD_E_INVALID;
+    ensure_totals(c);
     uint32_t cnt = 0;
     HIPCHK(c, hipMemcpyAsync(&cnt, c->d.ctr + (filtered ? C_NP : C_BP), sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
     int st = check_overflow(c);
@@ -725,6 +746,7 @@ int bvhcd_get_pairs(bvhcd_ctx* c, int32_t* out, int64_t cap, int64_t* num, int w
 
 int bvhcd_pair_digest(bvhcd_ctx* c, int filtered, uint64_t* digest, int64_t* num) {
     if (!c || !digest) return BVHCD_E_INVALID;
+    ensure_totals(c);
     launch_digest(c->d, filtered, c->stream);
     uint64_t dg = 0;
     uint32_t cnt = 0;
@@ -785,6 +807,7 @@ int bvhcd_timing_frames(bvhcd_ctx* c, int64_t cap, double* ms, double* bytes, in
 
 int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     if (!c || !out) return BVHCD_E_INVALID;
+    ensure_totals(c);
     std::memset(out, 0, sizeof(*out));
     uint32_t sz[4] = {0, 0, 0, 0}, ctr[16];
     uint64_t c64[2];
@@ -992,6 +1015,7 @@ int bvhcd_autotune(bvhcd_ctx* c) {
     if (c->cfg.narrow_phase) launch_narrow(d, c->stream);
     else clear_np(d, c->stream);
     launch_pair_totals(d, c->stream, c->cfg.narrow_phase != 0);
+    c->totals_pending = false;
     return check_overflow(c);
 }
 

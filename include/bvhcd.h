@@ -54,6 +54,10 @@ extern "C" {
 /* ---- memory location of caller buffers ---- */
 #define BVHCD_MEM_HOST   0
 #define BVHCD_MEM_DEVICE 1
+/* bvhcd_update_vertices only: the library reads the caller's device buffer in place (no copy)
+   until the next bvhcd_update_vertices; the buffer must stay allocated and unchanged while the
+   steps that read it run (they are ordered on the context stream). */
+#define BVHCD_MEM_DEVICE_BORROW 2
 
 /* ---- BVH maintenance op (LBvhFixedDeformableMaintenance, LBvhFixedDeformable.h:18-24) ---- */
 #define BVHCD_BVH_BUILD       0

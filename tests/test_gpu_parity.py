@@ -331,6 +331,37 @@ def test_box_mode(pkg, oracle):
     ctx.close()
 
 
+def test_borrowed_vertices(pkg, oracle):
+    """Vertices read in place from a caller's device buffer (MEM_DEVICE_BORROW) give the same
+    pair sets, schedule ops and BVH as copied ones; a later host update switches back to the
+    context's own buffer."""
+    import torch
+    print("[gpu] borrowed vertices", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=64, frames=20)
+    frames = torch.from_numpy(np.stack([sc.frame(f) for f in range(20)])).cuda()
+    a, b = _ctx(pkg, sc), _ctx(pkg, sc)
+    nbytes = sc.num_verts * 12
+    for f in range(20):
+        a.update_vertices(frames.data_ptr() + f * nbytes, pkg.MEM_DEVICE)
+        b.update_vertices(frames.data_ptr() + f * nbytes, pkg.MEM_DEVICE_BORROW)
+        a.step_scheduled()
+        b.step_scheduled()
+        sa, sb = a.stats(), b.stats()
+        assert (sa["last_bvh_op"], sa["last_front_op"]) == (sb["last_bvh_op"], sb["last_front_op"])
+        for narrow in (False, True):
+            np.testing.assert_array_equal(oracle.pair_set(a.pairs(narrow)), oracle.pair_set(b.pairs(narrow)),
+                                          err_msg=f"frame {f} {narrow}")
+    compare_bvh(b.export_bvh(), a.export_bvh(), "borrowed")
+    v = sc.frame(19)
+    b.update_vertices(v)
+    del frames
+    torch.cuda.synchronize()
+    b.step(pkg.BVH_REFIT, pkg.FRONT_KEEP)
+    np.testing.assert_array_equal(oracle.pair_set(b.pairs(True)), oracle.pair_set(oracle.bruteforce_pairs(sc.faces(), v, True)))
+    a.close()
+    b.close()
+
+
 def test_capacity_overflow_reported(pkg):
     print("[gpu] overflow", flush=True)
     sc = pkg.Scene.preset("spheres", subdiv=3, spacing=2.0)
