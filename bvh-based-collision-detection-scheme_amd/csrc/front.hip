@@ -747,8 +747,8 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int imp
     const XcdLoop it = xcd_loop(total);
     const uint32_t stride = it.stride;
     uint32_t visits = 0;
-    for (uint32_t k0 = it.begin; k0 < it.end; k0 += U * stride) {
-        int32_t q[U], nd[U];
+    // entries of iteration k0 into (q, nd); -1 past the end
+    auto load_entries = [&](uint32_t k0, int32_t* q, int32_t* nd) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t k = k0 + u * stride;
@@ -763,6 +763,18 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int imp
                 nd[u] = (int32_t)ext_node[k - ni];
             }
         }
+    };
+    // software pipeline: the next iteration's entries are loaded while this one's node records
+    // are in flight, so each iteration waits for one round trip instead of two
+    int32_t qn[U], ndn[U];
+    load_entries(it.begin, qn, ndn);
+    for (uint32_t k0 = it.begin; k0 < it.end; k0 += U * stride) {
+        int32_t q[U], nd[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            q[u] = qn[u];
+            nd[u] = ndn[u];
+        }
         float4 qlo[U], qhi[U], lo[U], hi[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -775,6 +787,7 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int imp
                 hi[u] = r->hi;
             }
         }
+        load_entries(k0 + U * stride, qn, ndn);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const bool internal = k0 + u * stride < ni;

@@ -960,13 +960,15 @@ __device__ inline Box record_box(const DevState& d, int32_t v, int kind) {
 __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chunks) {
     __shared__ Box tb[kTopLds];
     // frame-wide: may KEEP count shared-vertex pairs without a test (every leaf box built from
-    // finite vertices); or over the chunks
+    // finite vertices); or over the chunks, reduced at the first barrier so that these loads
+    // overlap the ones below
     int bad = 0;
     for (int32_t c = threadIdx.x; c < chunks; c += kTopBlock) bad |= (int)d.rp_cbad[c];
-    bad = __syncthreads_or(bad);
-    if (threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
     const int32_t m = d.rp_tinfo[0], top = d.rp_tinfo[1];
-    if (m <= kTopLds) {
+    if (m > kTopLds) {
+        bad = __syncthreads_or(bad);
+        if (threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
+    } else {
         int4 e[kTopPer];
 #pragma unroll
         for (int k = 0; k < kTopPer; ++k) {
@@ -1006,7 +1008,8 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
                 }
             }
         }
-        __syncthreads();
+        bad = __syncthreads_or(bad);
+        if (threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
         for (int32_t lev = top; lev >= 0; --lev) {
 #pragma unroll
             for (int k = 0; k < kTopPer; ++k) {
