@@ -142,3 +142,16 @@ def test_autotune_set_validates_shapes():
             pkg.autotune_set(*args)
     pkg.autotune_set(pkg.TUNE_KEEP, 1000, 512, 4)
     pkg.autotune_set(pkg.TUNE_KEEP, 1000, 2048, 1)
+
+
+def test_bench_reads_profiled_traffic():
+    """bench.py takes roofline.traffic from the committed counter summary: the learned KEEP row
+    exists and parses to a byte count; unknown kernels or workloads give None."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    t = bench.profiled_traffic("k_keep", "cloth1024", "learned")
+    assert isinstance(t, int) and t > 0
+    assert bench.profiled_traffic("k_nothing", "cloth1024", "learned") is None
+    assert bench.profiled_traffic("k_keep", "layers4", "learned") is None
