@@ -124,6 +124,7 @@ def lib():
         "bvhcd_autotune_report": (C.c_int, [P, I64]),
         "bvhcd_autotune_set": (C.c_int, [C.c_int, I64, C.c_uint32, C.c_uint32]),
         "bvhcd_update_boxes": (C.c_int, [P, P, I64, C.c_int]),
+        "bvhcd_mlp_set_repair_cost": (C.c_int, [P, C.c_float]),
         "bvhcd_scene_create": (C.c_int, [C.c_int, C.POINTER(SceneParams), C.POINTER(P)]),
         "bvhcd_scene_destroy": (None, [P]),
         "bvhcd_scene_num_tris": (I64, [P]),
@@ -415,6 +416,10 @@ class CollisionContext:
 
     def mlp_set_threshold(self, t):
         self._chk(lib().bvhcd_mlp_set_threshold(self._h, float(t)), "mlp_set_threshold")
+
+    def mlp_set_repair_cost(self, cost):
+        """Cost gate of the learned selection (0 = off: every selection above the threshold)."""
+        self._chk(lib().bvhcd_mlp_set_repair_cost(self._h, float(cost)), "mlp_set_repair_cost")
 
     def restructure(self, roots):
         """Restructure the subtrees under the given preorder internal-node ids now; the next

@@ -27,6 +27,8 @@ constexpr int kLog = 8;   // device log words per frame
 // Learned schedule: a KEEP frame whose node tests exceed this multiple of the front size
 // (each entry is one test unless its internal node overlaps and is descended) asks for UPDATE.
 constexpr double kLearnedUpdateRatio = 1.25;
+constexpr double kLearnedNodeOrderRatio = 1.05;
+constexpr int kLearnedMaxBackoff = 3;
 }  // namespace
 
 struct bvhcd_ctx {
@@ -38,6 +40,8 @@ struct bvhcd_ctx {
     int64_t T = 0, V = 0;
     bool built = false, front_valid = false, restr_pending = false;
     bool learned_frame = false;  // set by the learned scheduler for its decision frames
+    int learned_backoff = 0;     // decision interval = bvh_cycle << backoff (rejected selections)
+    bool learned_node_order = false;  // produce node-ordered fronts (KEEP descends often)
     bool repair_pending = false; // front classified for the incremental front restructure
     uint32_t repair_tasks = 0;   // its (query, subtree) walks
     int64_t frame = 0;
@@ -367,8 +371,15 @@ int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed, bool use_lear
     int64_t nsub = 0, nint = 0, nleaf = 0;
     const bool learned = use_learned && learned_ready(c->learned);
     if (learned) {
-        int st = learned_select(c->learned, d, n, c->frame, c->stream, &nsub, &nint, &nleaf, &c->err);
+        const int bc = c->cfg.bvh_cycle > 0 ? c->cfg.bvh_cycle : 12;
+        bool rejected = false;
+        int st = learned_select(c->learned, d, n, c->frame, bc << c->learned_backoff, c->stream, &nsub, &nint, &nleaf,
+                                &rejected, &c->err);
         if (st < 0) return st;
+        // a selection that cannot pay for its front repair makes the next decision come later
+        // (up to 8 cycles); an applied one resets the interval
+        if (rejected) c->learned_backoff = std::min(c->learned_backoff + 1, kLearnedMaxBackoff);
+        else if (nsub > 0) c->learned_backoff = 0;
     } else {
         launch_restr_decision(d, n, c->stream);
         uint32_t cnt[3];
@@ -447,11 +458,13 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     }
     if (front_op != BVHCD_FRONT_PURE_BVH_CD && !c->front_valid) front_op = BVHCD_FRONT_GENERATE;
     c->restr_pending = false;  // the quality flag lives for one frame
-    // The learned schedule neither reads the node-id ordering nor the front-quality heuristic:
-    // its fronts keep emission order (neighbouring queries together, which KEEP streams with
-    // better locality than node order).
-    const bool order = c->cfg.scheduler != BVHCD_SCHED_LEARNED;
-    const bool quality = c->cfg.enable_restructure != 0 && order;
+    // The learned schedule reads neither the node-id ordering's bucket offsets nor the
+    // front-quality heuristic: its fronts are sorted by query leaf (neighbouring queries
+    // together, the best locality for KEEP without descents), or by node id once KEEP has been
+    // seen descending often (lanes then descend the same subtrees).
+    const bool plain_order = c->cfg.scheduler != BVHCD_SCHED_LEARNED;
+    const bool quality = c->cfg.enable_restructure != 0 && plain_order;
+    const bool order = plain_order || c->learned_node_order;  // node order, else query order
     if (front_op != BVHCD_FRONT_KEEP && front_op != BVHCD_FRONT_PURE_BVH_CD) d.front_version++;
     switch (front_op) {
         case BVHCD_FRONT_PURE_BVH_CD:
@@ -487,7 +500,6 @@ int run_front(bvhcd_ctx* c, int& front_op) {
         case BVHCD_FRONT_RESTRUCTURE:
             begin(c, S_FRONT);
             launch_front_restr_apply(d, n, c->repair_tasks, s);
-            if (!order) launch_query_order(d, n, s);
             end(c, S_FRONT);
             c->repair_pending = false;
             break;
@@ -692,7 +704,8 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
         front_op = BVHCD_FRONT_KEEP;
         const int bc = c->cfg.bvh_cycle > 0 ? c->cfg.bvh_cycle : 12;
         const int fc = c->cfg.front_cycle > 0 ? c->cfg.front_cycle : 6;
-        if (c->frame % bc == 0 || (c->cfg.enable_restructure && c->restr_pending)) c->learned_frame = true;
+        if (c->frame % ((int64_t)bc << c->learned_backoff) == 0 || (c->cfg.enable_restructure && c->restr_pending))
+            c->learned_frame = true;
         if (c->frame % fc == 0 && c->last_front == BVHCD_FRONT_KEEP) {
             uint64_t visits = 0;
             uint32_t sz[4] = {0, 0, 0, 0}, implicit = 0;
@@ -705,6 +718,10 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
             // entries KEEP tested (the implicit shared-vertex head takes no node test)
             const double entries = (double)sz[c->d.fcur[0]] + (double)sz[2 + c->d.fcur[1]] - (double)implicit;
             if ((double)visits > kLearnedUpdateRatio * entries + 1024.0) front_op = BVHCD_FRONT_UPDATE;
+            // KEEP's descents below now-overlapping internal entries are cheaper on node-ordered
+            // fronts (lanes descend the same subtree); without them, query order keeps each
+            // wave's records local. The next produced front takes the order that fits.
+            c->learned_node_order = (double)visits > kLearnedNodeOrderRatio * entries + 1024.0;
         }
         return do_step(c, bvh_op, front_op);
     }
@@ -1035,6 +1052,15 @@ int bvhcd_autotune_report(char* buf, int64_t cap) {
         buf[k] = 0;
     }
     return (int)r.size();
+}
+
+int bvhcd_mlp_set_repair_cost(bvhcd_ctx* c, float cost) {
+    if (!c || !(cost >= 0.f)) return BVHCD_E_INVALID;
+    if (!c->learned) c->learned = learned_create(c->stream);
+    if (!c->learned) return fail(c, BVHCD_E_OOM, "learned state");
+    learned_set_repair_cost(c->learned, cost);
+    c->learned_backoff = 0;
+    return BVHCD_OK;
 }
 
 int bvhcd_mlp_set_threshold(bvhcd_ctx* c, float t) {

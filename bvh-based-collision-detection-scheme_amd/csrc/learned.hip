@@ -31,6 +31,10 @@
 namespace bvhcd {
 
 constexpr int kIn = 16, kH = 64;
+// Cost of the incremental front repair that follows a rebuild, in KEEP work per front entry:
+// classification, KEEP of the kept entries and compaction each read the whole front (measured
+// on the 4-layer cloth: ~32 ms against ~5-8 ms per KEEP frame of the same front).
+constexpr float kDefaultRepairCost = 4.0f;
 constexpr int kOffW1 = 0, kOffB1 = kH * kIn, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH, kOffW3 = kOffB2 + kH,
               kOffB3 = kOffW3 + kH, kNParam = kOffB3 + 1;
 static_assert(kNParam == BVHCD_MLP_NPARAM, "parameter packing");
@@ -71,6 +75,8 @@ struct LearnedState {
     int64_t node_cap = 0;
     float* feats = nullptr;
     float threshold = 0.0f;
+    float repair_cost = kDefaultRepairCost;  // cost gate (0 = off), front entries' KEEP work per entry
+    float* gate = nullptr;                   // device: predicted saving of the selection
 };
 
 static bool ensure(float*& p, int64_t& cap, int64_t need) {
@@ -101,7 +107,7 @@ void learned_destroy(LearnedState* L) {
     if (!L) return;
     for (void* p : {(void*)L->params, (void*)L->wbf, (void*)L->grad, (void*)L->m, (void*)L->v, (void*)L->loss,
                     (void*)L->x, (void*)L->y, (void*)L->t, (void*)L->roots, (void*)L->flags, (void*)L->pos,
-                    (void*)L->total, (void*)L->feats, (void*)L->acnt, (void*)L->lcnt})
+                    (void*)L->total, (void*)L->feats, (void*)L->acnt, (void*)L->lcnt, (void*)L->gate})
         if (p) (void)hipFree(p);
     delete L;
 }
@@ -572,23 +578,68 @@ static int read_counts(DevState& d, hipStream_t s, int64_t* nsub, int64_t* nint,
     return BVHCD_OK;
 }
 
-int learned_select(LearnedState* L, DevState& d, int32_t n, int64_t frame, hipStream_t s, int64_t* nsub, int64_t* nint,
-                   int64_t* nleaf, std::string* err) {
+// Predicted saving of a selection, in front entries per frame: sum over the selected
+// candidates of (predicted fraction of their entries saved) x (their entries = feature 1 x
+// leaves). One workgroup.
+__global__ void __launch_bounds__(kBlock) k_gate(int64_t rows, const int32_t* __restrict__ roots,
+                                                 const float* __restrict__ pred, float threshold,
+                                                 const float* __restrict__ feats, DevState d, float* out) {
+    __shared__ float red[kBlock];
+    float acc = 0.f;
+    for (int64_t k = threadIdx.x; k < rows; k += kBlock) {
+        const float y = pred[k];
+        if (!(y > threshold)) continue;
+        const int32_t r = roots[k];
+        const float leaves = (float)(d.int_ry[r] - d.int_rx[r] + 1);
+        acc += y * feats[k * kIn + 1] * leaves;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// The network ranks subtrees; the cost gate judges the whole selection: a rebuild is followed
+// by an incremental front repair whose passes read the whole front (classification, KEEP of
+// the kept entries, compaction), about repair_cost x (front entries) of KEEP work, so the
+// selection is applied only if its predicted saving over `horizon` frames pays for that.
+int learned_select(LearnedState* L, DevState& d, int32_t n, int64_t frame, int32_t horizon, hipStream_t s,
+                   int64_t* nsub, int64_t* nint, int64_t* nleaf, bool* rejected, std::string* err) {
+    *rejected = false;
     int64_t rows = 0;
     int st = compute_features(L, d, n, frame, s, &rows);
     if (st) {
         if (err) *err = "learned selection: features failed";
         return st;
     }
+    if (!L->gate && hipMalloc(&L->gate, sizeof(float) * 2) != hipSuccess) return BVHCD_E_OOM;
     (void)hipMemsetAsync(d.ctr + C_RT_SUB, 0, sizeof(uint32_t) * 3, s);
     (void)hipMemsetAsync(d.r_leaf_lo, 0, sizeof(int32_t) * ((size_t)n + 1), s);
+    (void)hipMemsetAsync(L->gate, 0, sizeof(float) * 2, s);
     if (rows > 0) {
         if (stage_rows(L, rows)) return BVHCD_E_OOM;
         launch_forward(L, L->feats, rows, L->y, s);
         hipLaunchKernelGGL(k_mark_roots, dim3(grid_for(rows)), dim3(kBlock), 0, s, rows, L->roots, L->y, L->threshold,
                            1, d);
+        hipLaunchKernelGGL(k_gate, dim3(1), dim3(kBlock), 0, s, rows, L->roots, L->y, L->threshold, L->feats, d,
+                           L->gate);
     }
-    return read_counts(d, s, nsub, nint, nleaf);
+    float saving = 0.f;
+    uint32_t sz[4] = {0, 0, 0, 0};
+    if (hipMemcpyAsync(&saving, L->gate, sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(sz, d.fsize, sizeof(sz), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return BVHCD_E_DEVICE;
+    st = read_counts(d, s, nsub, nint, nleaf);
+    if (st) return st;
+    const double entries = (double)sz[d.fcur[0]] + (double)sz[2 + d.fcur[1]];
+    if (*nsub > 0 && L->repair_cost > 0.f && (double)saving * horizon < (double)L->repair_cost * entries) {
+        *nsub = *nint = *nleaf = 0;  // not worth its front repair: refit instead
+        *rejected = true;
+    }
+    return BVHCD_OK;
 }
 
 // Explicit selection (host-given roots), validated: internal ids with disjoint leaf ranges.
@@ -628,6 +679,9 @@ int mark_given_roots(DevState& d, int32_t n, const int32_t* roots_host, int64_t 
 
 void learned_set_threshold(LearnedState* L, float t) {
     if (L) L->threshold = t;
+}
+void learned_set_repair_cost(LearnedState* L, float cost) {
+    if (L) L->repair_cost = cost;
 }
 
 }  // namespace bvhcd

@@ -21,8 +21,11 @@ int learned_grad(LearnedState* L, const float* feats, const float* targets, int6
 bool learned_ready(const LearnedState* L);
 // Selects the subtrees to restructure with the MLP (difference marks in d.r_leaf_lo, as the
 // heuristic selection leaves them); returns subtree / internal-node / leaf counts.
-int learned_select(LearnedState* L, DevState& d, int32_t n, int64_t frame, hipStream_t s, int64_t* nsub,
-                   int64_t* nint, int64_t* nleaf, std::string* err);
+// Scores the candidate subtrees and marks those above the threshold; the selection is dropped
+// (*rejected, counts 0) when its predicted saving over `horizon` frames does not pay for the
+// front repair (cost gate, off with repair cost 0).
+int learned_select(LearnedState* L, DevState& d, int32_t n, int64_t frame, int32_t horizon, hipStream_t s,
+                   int64_t* nsub, int64_t* nint, int64_t* nleaf, bool* rejected, std::string* err);
 // Surface-area references and rebuild frame (all nodes after a build; all = 0: only the
 // subtrees just restructured, from d.rt_of_leaf).
 void update_sa_ref(DevState& d, int32_t n, int all, int32_t frame, hipStream_t s);
@@ -30,4 +33,5 @@ void update_sa_ref(DevState& d, int32_t n, int all, int32_t frame, hipStream_t s
 int mark_given_roots(DevState& d, int32_t n, const int32_t* roots, int64_t k, hipStream_t s, int64_t* nsub,
                      int64_t* nint, int64_t* nleaf, std::string* err);
 void learned_set_threshold(LearnedState* L, float t);
+void learned_set_repair_cost(LearnedState* L, float cost);
 }  // namespace bvhcd

@@ -248,13 +248,15 @@ def test_front_restructure_matches_oracle(pkg, oracle, max_leaves, every):
     ctx.close()
 
 
-def _timed_run(pkg, sc, frames, weights=None, threshold=0.0):
+def _timed_run(pkg, sc, frames, weights=None, threshold=0.0, repair_cost=None):
     ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts,
                                scheduler=pkg.SCHED_LEARNED if weights is not None else pkg.SCHED_PLAIN)
     ctx.set_topology(sc.faces(), sc.num_verts)
     if weights is not None:
         ctx.mlp_set_weights(weights)
         ctx.mlp_set_threshold(threshold)
+        if repair_cost is not None:
+            ctx.mlp_set_repair_cost(repair_cost)
     ctx.set_timing(len(frames))
     dig = []
     for v in frames:
@@ -277,7 +279,7 @@ def test_trained_scheduler_gains_over_plain(pkg):
     frames = [sc.frame(f) for f in range(sc.frames)]
     t_plain, d_plain, _ = _timed_run(pkg, sc, frames)
     t_learn, d_learn, ops = _timed_run(pkg, sc, frames, params, thr)
-    t_all, d_all, _ = _timed_run(pkg, sc, frames, params, -1e30)
+    t_all, d_all, _ = _timed_run(pkg, sc, frames, params, -1e30, repair_cost=0.0)
     print(f"[gpu] plain {t_plain:.1f} ms, learned {t_learn:.1f} ms ({t_plain / t_learn:.3f}x), "
           f"rebuild-all {t_all:.1f} ms, learned restructure frames {(ops[:, 0] == pkg.BVH_RESTRUCTURE).sum()}",
           flush=True)
@@ -295,6 +297,7 @@ def test_learned_schedule_keeps_pairs(pkg, oracle):
     ctx = _ctx(pkg, sc, scheduler=pkg.SCHED_LEARNED)
     ctx.mlp_set_weights(mlp_ref.init_params(21))
     ctx.mlp_set_threshold(-1e30)   # every candidate
+    ctx.mlp_set_repair_cost(0.0)   # and no cost gate
     orc = oracle.OracleContext(f, sc.num_verts)
     restr = 0
     for fr in range(sc.frames):
@@ -310,3 +313,37 @@ def test_learned_schedule_keeps_pairs(pkg, oracle):
         restr += ctx.stats()["last_bvh_op"] == pkg.BVH_RESTRUCTURE
     print(f"[gpu] learned restructure frames: {restr}", flush=True)
     ctx.close()
+
+
+def test_cost_gate_rejects_unprofitable_selections(pkg, oracle):
+    """With every candidate selected, the cost gate off rebuilds on decision frames; a prohibitive
+    repair cost rejects every selection (no restructure, decisions spaced out); pair sets equal
+    the oracle's either way."""
+    print("[gpu] learned cost gate", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=64, frames=50)
+    f = sc.faces()
+    counts = {}
+    for cost in (0.0, 1e9):
+        ctx = _ctx(pkg, sc, scheduler=pkg.SCHED_LEARNED)
+        ctx.mlp_set_weights(mlp_ref.init_params(21))
+        ctx.mlp_set_threshold(-1e30)
+        ctx.mlp_set_repair_cost(cost)
+        restr = 0
+        for fr in range(sc.frames):
+            v = sc.frame(fr)
+            ctx.update_vertices(v)
+            ctx.step_scheduled()
+            restr += ctx.stats()["last_bvh_op"] == pkg.BVH_RESTRUCTURE
+            if fr % 7 == 3:
+                np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)),
+                                              oracle.pair_set(oracle.bruteforce_pairs(f, v, True)),
+                                              err_msg=f"cost {cost} frame {fr}")
+        counts[cost] = restr
+        ctx.close()
+    assert counts[0.0] > 0 and counts[1e9] == 0, counts
+    with pytest.raises(pkg.BvhcdError):
+        ctx = _ctx(pkg, sc, scheduler=pkg.SCHED_LEARNED)
+        try:
+            ctx.mlp_set_repair_cost(-1.0)
+        finally:
+            ctx.close()
