@@ -122,12 +122,12 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.rp_cinfo, (size_t)refit_chunks((int32_t)T));
     ok &= dalloc(c, d.rp_ent, T);
     ok &= dalloc(c, d.rp_tent, T);
-    ok &= dalloc(c, d.rp_tlist, T);
+    ok &= dalloc(c, d.rp_tlist, std::max<size_t>(T, kRefitTopLds));
     ok &= dalloc(c, d.rp_tpos, T);
     ok &= dalloc(c, d.rp_tlev, 1024);
     ok &= dalloc(c, d.rp_tinfo, 2);
     ok &= dalloc(c, d.rp_tcount, 1);
-    ok &= dalloc(c, d.rp_tcb, 6 * 2 * 6144);
+    ok &= dalloc(c, d.rp_tcb, 6 * 2 * (size_t)kRefitTopLds);
     ok &= dalloc(c, d.rp_cbad, (size_t)refit_chunks((int32_t)T));
     ok &= dalloc(c, d.leaf_par, T);
     ok &= dalloc(c, d.leaf_mark, T);
@@ -376,10 +376,11 @@ int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed, bool use_lear
         int st = learned_select(c->learned, d, n, c->frame, bc << c->learned_backoff, c->stream, &nsub, &nint, &nleaf,
                                 &rejected, &c->err);
         if (st < 0) return st;
-        // a selection that cannot pay for its front repair makes the next decision come later
-        // (up to 8 cycles); an applied one resets the interval
-        if (rejected) c->learned_backoff = std::min(c->learned_backoff + 1, kLearnedMaxBackoff);
-        else if (nsub > 0) c->learned_backoff = 0;
+        // a decision that changes nothing (no subtree worth it, or a selection that cannot pay
+        // for its front repair) makes the next one come later (up to 8 cycles); an applied
+        // selection resets the interval
+        if (rejected || nsub == 0) c->learned_backoff = std::min(c->learned_backoff + 1, kLearnedMaxBackoff);
+        else c->learned_backoff = 0;
     } else {
         launch_restr_decision(d, n, c->stream);
         uint32_t cnt[3];

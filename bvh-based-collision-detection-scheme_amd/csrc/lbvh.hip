@@ -689,12 +689,15 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
 //    so every window but the tail of the last treelet starts one, the chunks tile the leaves
 //    and each spans fewer than kChunkCap = 2*kTreelet leaves;
 //  * preorder numbering gives the internal nodes of a treelet with root r and leaves [a, b]
-//    the ids [r, r + b - a): node p takes slot p - r + a, slot b stays empty, and the slots of
-//    a chunk are its own leaf range;
+//    the ids [r, r + b - a): node p's entry sits at p - r + a, so a chunk's entries fill its
+//    own leaf range;
+//  * LDS slots are leaf positions in the chunk: leaf x holds slot x - chunk start and every
+//    internal node the slot of its leftmost leaf, i.e. it overwrites its left child's box.
+//    Nodes merged at one depth are disjoint subtrees, so each slot is written by at most one
+//    of them, and a right child's slot (its own leftmost leaf) is written only by deeper nodes;
 //  * each chunk lists its internal nodes deepest first (depth below the treelet root) as
-//    (node, LDS slot | depth << 16, child slots, top-kernel slot or -1), slots being leaf x:
-//    x - chunk start, internal node: slot - chunk start + kChunkCap; the top tree is listed
-//    deepest first (depth below the root) with its children's kinds.
+//    (node, LDS slot | depth << 16, child slots, top-kernel slot or -1); the top tree is
+//    listed deepest first (depth below the root) with its children's kinds.
 // Per refit: one workgroup per chunk writes the leaf boxes into LDS, merges one depth level
 // per barrier in LDS and writes its node boxes at the end; then one workgroup merges the top
 // tree level by level through the node records it writes itself (workgroup-coherent).
@@ -707,10 +710,10 @@ constexpr int kDepthCap = 1024;
 constexpr int kTopBlock = 1024;
 constexpr int kRefitBlock = 512;                       // per-frame chunk refit: 8 waves per chunk
 constexpr int kRefitPerT = kChunkCap / kRefitBlock;     // its leaves / nodes per thread
-constexpr int kTopLds = 6144;  // top trees up to this size are merged in LDS
+constexpr int kTopLds = kRefitTopLds;  // top trees up to this size are merged in LDS
 constexpr int kTopPer = kTopLds / kTopBlock;
 static_assert(kChunkCap % kBlock == 0, "chunk capacity is a whole number of block passes");
-static_assert(2 * kChunkCap <= 65536, "slots are packed in 16 bits");
+static_assert(kChunkCap <= 65536, "slots are packed in 16 bits");
 
 __device__ inline bool rp_is_top(const DevState& d, int32_t p) { return d.int_ry[p] - d.int_rx[p] >= kTreelet; }
 
@@ -757,18 +760,20 @@ __global__ void __launch_bounds__(kBlock) k_rp_walk(int32_t n, DevState d) {
     const int32_t a = d.int_rx[r];
     if (x == d.int_ry[r]) d.rp_ent[x] = empty;
     const int32_t cs = d.rp_cstart[a / kTreelet];
-    const int32_t base = a - r - cs + kChunkCap;  // LDS slot of internal node u: u + base
+    const int32_t own = x - cs;  // slot of every node whose leftmost leaf is x (and of x)
     const bool lds_top = d.rp_tinfo[0] <= kTopLds;
     int32_t q = p;
     for (int32_t i = 0; i < len && d.int_rx[q] == x; ++i) {
         const int4 tp = d.itopo[q];
-        const int32_t ls = (tp.w & 1) ? tp.x - cs : tp.x + base;
-        const int32_t rs = (tp.w & 2) ? tp.y - cs : tp.y + base;
-        // the treelet root hands its box to the top kernel: slot of its parent's list position
-        // and side (mark bit 2: right child)
+        const int32_t rs = ((tp.w & 2) ? tp.y : d.int_rx[tp.y]) - cs;
+        // the treelet root hands its box to the top kernel: dense offset from its parent's
+        // entry, side by mark bit 2 (right child)
         int32_t dest = -1;
-        if (q == r && tp.z >= 0 && lds_top) dest = 2 * d.rp_tpos[tp.z] + ((tp.w & 4) ? 1 : 0);
-        d.rp_ent[q - r + a] = make_int4(q, (q + base) | ((len - 1 - i) << 16), ls | (rs << 16), dest);
+        if (q == r && tp.z >= 0 && lds_top) {
+            const int4 pe = d.rp_tlist[d.rp_tpos[tp.z]];  // the parent's entry, offsets per side
+            dest = (tp.w & 4) ? pe.z : pe.y;
+        }
+        d.rp_ent[q - r + a] = make_int4(q, own | ((len - 1 - i) << 16), own | (rs << 16), dest);
         q = tp.z;
     }
 }
@@ -810,8 +815,8 @@ __global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
 
 // Top tree sorted deepest first; level t occupies [rp_tlev[t], t ? rp_tlev[t-1] : count).
 // Entry (node, left, right, depth | left kind << 16 | right kind << 18), kind 0: internal node
-// record, 1: leaf record, 2: list position of a top node (set here once positions are known;
-// the workgroup's own global writes are ordered for it by the barrier).
+// record, 1: leaf record, 2: list position of a top node (set here once positions are known,
+// for top trees merged in LDS only; larger ones keep the node ids).
 __device__ inline int32_t top_depth(const int4& e) { return e.w & 0xFFFF; }
 __device__ inline int top_kind(const int4& e, int side) { return (e.w >> (16 + 2 * side)) & 3; }
 
@@ -847,7 +852,17 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
         d.rp_tpos[e.x] = (int32_t)pos;
     }
     __syncthreads();
-    for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
+    // LDS-resident merge only (larger trees keep node ids): top children by list position, and
+    // treelet-root children by dense hand-over offsets in list order, so that k_refit writes
+    // exactly the boxes the top kernel loads. Thread t renumbers positions [t*kTopPer, ...).
+    if (m > kTopLds) return;
+    __shared__ uint32_t part[kTopBlock];
+    int4 ev[kTopPer];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kTopPer; ++k) {
+        const int32_t i = (int32_t)threadIdx.x * kTopPer + k;
+        if (i >= m) continue;
         int4 e = d.rp_tlist[i];
         if (top_kind(e, 0) == 0 && rp_is_top(d, e.y)) {
             e.y = d.rp_tpos[e.y];
@@ -857,6 +872,25 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
             e.z = d.rp_tpos[e.z];
             e.w |= 2 << 18;
         }
+        cnt += (top_kind(e, 0) == 0 ? 1u : 0u) + (top_kind(e, 1) == 0 ? 1u : 0u);
+        ev[k] = e;
+    }
+    part[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < kTopBlock; off <<= 1) {  // inclusive scan (prep only)
+        const uint32_t add = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int32_t o = (int32_t)(part[threadIdx.x] - cnt);
+#pragma unroll
+    for (int k = 0; k < kTopPer; ++k) {
+        const int32_t i = (int32_t)threadIdx.x * kTopPer + k;
+        if (i >= m) continue;
+        int4 e = ev[k];
+        if (top_kind(e, 0) == 0) e.y = o++;
+        if (top_kind(e, 1) == 0) e.z = o++;
         d.rp_tlist[i] = e;
     }
 }
@@ -881,11 +915,14 @@ __device__ inline void store_box(NodeRec* rec, const Box& b) {
 }
 
 __global__ void __launch_bounds__(kRefitBlock) k_refit(int32_t n, DevState d) {
-    __shared__ Box sb[2 * kChunkCap];
+    __shared__ Box sb[kChunkCap];
     const int32_t c = blockIdx.x;
     const int32_t s = d.rp_cstart[c], e = d.rp_cstart[c + 1];
     const int2 info = d.rp_cinfo[c];  // (internal nodes, deepest level)
-    // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase
+    // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase. The loads of a
+    // leaf run one after another: measured faster than issuing all of a thread's loads at once
+    // (k_refit 52 vs 63-69 us on the 2M-triangle cloth), the gathers of neighbouring waves
+    // then share lines.
     int bad = 0;
 #pragma unroll
     for (int k = 0; k < kRefitPerT; ++k) {
@@ -951,6 +988,15 @@ __device__ inline Box record_box(const DevState& d, int32_t v, int kind) {
     return rec_box(r->lo, r->hi);
 }
 
+// Box of a top node's child below the top tree: kind 0 a treelet root (dense hand-over box),
+// 1 a leaf (record), 2 none (a valid address is read and the value ignored). Branch-free, so
+// that every load of a thread is in flight before the first one is used.
+__device__ inline Box below_top_box(const DevState& d, int kind, int32_t ref) {
+    const float* p = kind == 1 ? reinterpret_cast<const float*>(&d.leaf[ref]) : d.rp_tcb + 6 * (int64_t)(kind == 0 ? ref : 0);
+    const float* q = p + (kind == 1 ? 4 : 3);
+    return {p[0], p[1], p[2], q[0], q[1], q[2]};
+}
+
 // One workgroup. Top trees of up to kTopLds nodes stay in LDS, slot = list position: a node
 // whose children are both below the top tree (treelet roots) is merged at once, a node with
 // one such child starts its slot with that child's box and merges the other at its level.
@@ -959,53 +1005,54 @@ __device__ inline Box record_box(const DevState& d, int32_t v, int kind) {
 
 __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chunks) {
     __shared__ Box tb[kTopLds];
+    // the list is loaded before the top-tree size arrives (the buffer holds kTopLds entries);
+    // positions past it are masked afterwards
+    int4 e[kTopPer];
+#pragma unroll
+    for (int k = 0; k < kTopPer; ++k) e[k] = d.rp_tlist[threadIdx.x + k * kTopBlock];
     // frame-wide: may KEEP count shared-vertex pairs without a test (every leaf box built from
-    // finite vertices); or over the chunks, reduced at the first barrier so that these loads
-    // overlap the ones below
-    int bad = 0;
-    for (int32_t c = threadIdx.x; c < chunks; c += kTopBlock) bad |= (int)d.rp_cbad[c];
+    // finite vertices); OR over the chunks, four loads per thread in flight at once (indices
+    // clamped: a repeated flag does not change the OR), issued after the box loads below and
+    // reduced at the first barrier
+    auto chunks_bad = [&]() {
+        int bad = 0;
+        for (int32_t c0 = 0; c0 < chunks; c0 += 4 * kTopBlock) {
+            uint32_t a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = d.rp_cbad[min(c0 + (int32_t)threadIdx.x + u * kTopBlock, chunks - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) bad |= (int)a[u];
+        }
+        return bad;
+    };
     const int32_t m = d.rp_tinfo[0], top = d.rp_tinfo[1];
     if (m > kTopLds) {
-        bad = __syncthreads_or(bad);
+        const int bad = __syncthreads_or(chunks_bad());
         if (threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
     } else {
-        int4 e[kTopPer];
+#pragma unroll
+        for (int k = 0; k < kTopPer; ++k)
+            if ((int32_t)threadIdx.x + k * kTopBlock >= m) e[k] = make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
+        // children below the top tree, all loads of the thread in flight together: treelet roots
+        // from the dense hand-over array k_refit wrote, leaves from their records
+        Box cl[kTopPer], cr[kTopPer];
+#pragma unroll
+        for (int k = 0; k < kTopPer; ++k) {
+            cl[k] = below_top_box(d, top_kind(e[k], 0), e[k].y);
+            cr[k] = below_top_box(d, top_kind(e[k], 1), e[k].z);
+        }
+        int bad = chunks_bad();
 #pragma unroll
         for (int k = 0; k < kTopPer; ++k) {
             const int32_t i = threadIdx.x + k * kTopBlock;
-            e[k] = i < m ? d.rp_tlist[i] : make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
-        }
-        // children below the top tree: treelet roots were handed over by k_refit in the entry's
-        // own slots (loaded without waiting for the entry), leaves come from their records; a
-        // group's loads all in flight before its merges, two groups keep the register budget
-        constexpr int kGroup = kTopPer / 2;
-#pragma unroll
-        for (int g = 0; g < kTopPer; g += kGroup) {
-            Box cl[kGroup], cr[kGroup];
-#pragma unroll
-            for (int k = 0; k < kGroup; ++k) {
-                const int32_t i = threadIdx.x + (g + k) * kTopBlock;
-                cl[k] = i < m ? given_box(d.rp_tcb, 2 * (int64_t)i) : Box{};
-                cr[k] = i < m ? given_box(d.rp_tcb, 2 * (int64_t)i + 1) : Box{};
-            }
-#pragma unroll
-            for (int k = 0; k < kGroup; ++k) {
-                const int lk = top_kind(e[g + k], 0), rk = top_kind(e[g + k], 1);
-                if (lk == 1) cl[k] = record_box(d, e[g + k].y, 1);
-                if (rk == 1) cr[k] = record_box(d, e[g + k].z, 1);
-            }
-#pragma unroll
-            for (int k = 0; k < kGroup; ++k) {
-                const int32_t i = threadIdx.x + (g + k) * kTopBlock;
-                const int lk = top_kind(e[g + k], 0), rk = top_kind(e[g + k], 1);
-                if (lk != 2 && rk != 2) {
-                    tb[i] = box_merge(cl[k], cr[k]);
-                    e[g + k].w |= 0xFFFF;  // done
-                } else if (lk != 2) {
-                    tb[i] = cl[k];
-                } else if (rk != 2) {
-                    tb[i] = cr[k];
-                }
+            const int lk = top_kind(e[k], 0), rk = top_kind(e[k], 1);
+            if (lk != 2 && rk != 2) {
+                tb[i] = box_merge(cl[k], cr[k]);
+                e[k].w |= 0xFFFF;  // done
+            } else if (lk != 2) {
+                tb[i] = cl[k];
+            } else if (rk != 2) {
+                tb[i] = cr[k];
             }
         }
         bad = __syncthreads_or(bad);
@@ -1030,13 +1077,13 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
         }
         return;
     }
+    // larger top trees: children are node ids here (kinds 0 and 1 only)
     for (int32_t lev = top; lev >= 0; --lev) {
         const int32_t b0 = d.rp_tlev[lev], b1 = lev > 0 ? d.rp_tlev[lev - 1] : m;
         for (int32_t i = b0 + (int32_t)threadIdx.x; i < b1; i += kTopBlock) {
             const int4 e = d.rp_tlist[i];
-            const int lk = top_kind(e, 0), rk = top_kind(e, 1);
-            const Box l = record_box(d, lk == 2 ? d.rp_tlist[e.y].x : e.y, lk == 2 ? 0 : lk);
-            const Box r = record_box(d, rk == 2 ? d.rp_tlist[e.z].x : e.z, rk == 2 ? 0 : rk);
+            const Box l = record_box(d, e.y, top_kind(e, 0));
+            const Box r = record_box(d, e.z, top_kind(e, 1));
             store_box(&d.inode[e.x], box_merge(l, r));
         }
         __syncthreads();

@@ -198,8 +198,9 @@ int  bvhcd_mlp_set_threshold(bvhcd_ctx* ctx, float threshold);
 /* Cost gate of the learned selection (no counterpart in the reference; the reference's
    heuristic has no cost model): a selection is applied only when its predicted saving over the
    decision interval, in front entries, exceeds repair_cost x (front entries), the KEEP work the
-   incremental front repair after a rebuild costs. Rejected selections space the decisions out
-   (up to 8 bvh cycles). 0 turns the gate off; default 4. */
+   incremental front repair after a rebuild costs. Decisions that change nothing (rejected or
+   empty selections) space the next ones out, up to 8 bvh cycles. 0 turns the gate off;
+   default 4. */
 int  bvhcd_mlp_set_repair_cost(bvhcd_ctx* ctx, float repair_cost);
 /* Restructure the given subtrees (preorder internal-node ids with disjoint leaf ranges) of the
    current BVH now, refit, and invalidate the front (the next step must GENERATE). The
