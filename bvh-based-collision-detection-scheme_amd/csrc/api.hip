@@ -125,7 +125,7 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.rp_tlist, std::max<size_t>(T, kRefitTopLds));
     ok &= dalloc(c, d.rp_tpos, T);
     ok &= dalloc(c, d.rp_tlev, 1024);
-    ok &= dalloc(c, d.rp_tinfo, 2);
+    ok &= dalloc(c, d.rp_tinfo, 4);
     ok &= dalloc(c, d.rp_tcount, 1);
     ok &= dalloc(c, d.rp_tcb, 6 * 2 * (size_t)kRefitTopLds);
     ok &= dalloc(c, d.rp_cbad, (size_t)refit_chunks((int32_t)T));

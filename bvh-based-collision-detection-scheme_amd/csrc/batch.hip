@@ -350,7 +350,7 @@ int bvhcd_batch_create(const bvhcd_config* cfg, int32_t S, const int64_t* tps, c
     ok &= balloc(b, d.rp_tlist, std::max<size_t>(T, kRefitTopLds));
     ok &= balloc(b, d.rp_tpos, T);
     ok &= balloc(b, d.rp_tlev, 1024);
-    ok &= balloc(b, d.rp_tinfo, 2);
+    ok &= balloc(b, d.rp_tinfo, 4);
     ok &= balloc(b, d.rp_tcount, 1);
     ok &= balloc(b, d.rp_tcb, 6 * 2 * (size_t)kRefitTopLds);
     ok &= balloc(b, d.rp_cbad, (size_t)refit_chunks((int32_t)T));

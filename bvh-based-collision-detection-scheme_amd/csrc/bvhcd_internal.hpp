@@ -164,7 +164,8 @@ struct DevState {
     int4* rp_tlist = nullptr;       // T: the same, deepest first, top children as list positions
     int32_t* rp_tpos = nullptr;     // T-1: list position of each top node
     int32_t* rp_tlev = nullptr;     // 1024: first entry of each top-tree depth
-    int32_t* rp_tinfo = nullptr;    // 2: (top nodes, deepest level or -1)
+    int32_t* rp_tinfo = nullptr;    // 4: (top nodes, deepest level or -1, deepest of the levels
+                                    //     one wave merges or -1, their first list position)
     uint32_t* rp_tcount = nullptr;  // 1: append counter
     uint32_t* rp_cbad = nullptr;    // chunks: some leaf of the chunk has a non-finite coordinate
     float* rp_tcb = nullptr;        // 2 x 6144 boxes: treelet-root children of the top tree by

@@ -843,6 +843,15 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
         }
         d.rp_tinfo[0] = m;
         d.rp_tinfo[1] = m > 0 ? (int32_t)dmax : -1;
+        // the shallowest levels holding at most one wave's worth of nodes (positions from p1 on)
+        int32_t l1 = -1, p1 = m;
+        for (int t = 0; t <= (int)dmax && m > 0; ++t) {
+            if (m - (int32_t)hist[t] > 64) break;
+            l1 = t;
+            p1 = (int32_t)hist[t];
+        }
+        d.rp_tinfo[2] = l1;
+        d.rp_tinfo[3] = p1;
     }
     __syncthreads();
     for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
@@ -1041,6 +1050,11 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
             cl[k] = below_top_box(d, top_kind(e[k], 0), e[k].y);
             cr[k] = below_top_box(d, top_kind(e[k], 1), e[k].z);
         }
+        // the shallowest levels (at most 64 nodes, positions p1..m-1) are merged by wave 0 alone,
+        // one level after another without workgroup barriers; lane l holds position p1 + l
+        const int32_t l1 = d.rp_tinfo[2], p1 = d.rp_tinfo[3];
+        int4 ew = make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
+        if (threadIdx.x < 64 && p1 + (int32_t)threadIdx.x < m) ew = d.rp_tlist[p1 + threadIdx.x];
         int bad = chunks_bad();
 #pragma unroll
         for (int k = 0; k < kTopPer; ++k) {
@@ -1057,7 +1071,7 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
         }
         bad = __syncthreads_or(bad);
         if (threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
-        for (int32_t lev = top; lev >= 0; --lev) {
+        for (int32_t lev = top; lev > l1; --lev) {
 #pragma unroll
             for (int k = 0; k < kTopPer; ++k) {
                 if (top_depth(e[k]) == lev) {
@@ -1073,8 +1087,24 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
 #pragma unroll
         for (int k = 0; k < kTopPer; ++k) {
             const int32_t i = threadIdx.x + k * kTopBlock;
-            if (i < m) store_box(&d.inode[e[k].x], tb[i]);
+            if (i < p1) store_box(&d.inode[e[k].x], tb[i]);
         }
+        if (threadIdx.x >= 64) return;
+        // wave 0: LDS operations of one wave complete in issue order, so a level's writes are
+        // seen by the next level's reads; the fences keep the compiler from reordering them
+        const int32_t i = p1 + (int32_t)threadIdx.x;
+        const int lk = top_kind(ew, 0), rk = top_kind(ew, 1);
+        for (int32_t lev = l1; lev >= 0; --lev) {
+            if (top_depth(ew) == lev && (lk == 2 || rk == 2)) {
+                const Box l = tb[lk == 2 ? ew.y : i];
+                const Box r = tb[rk == 2 ? ew.z : i];
+                tb[i] = box_merge(l, r);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (i < m) store_box(&d.inode[ew.x], tb[i]);
         return;
     }
     // larger top trees: children are node ids here (kinds 0 and 1 only)
