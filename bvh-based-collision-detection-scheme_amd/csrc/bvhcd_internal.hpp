@@ -158,8 +158,9 @@ struct DevState {
     int4* itopo = nullptr;          // T-1 packed (lc, rc, par, mark)
     // refit schedule, rebuilt after every topology change (treelet chunks + top tree)
     int32_t* rp_cstart = nullptr;   // chunks + 1: first leaf of each chunk, then T
-    int2* rp_cinfo = nullptr;       // chunks: (internal nodes, deepest level)
-    int4* rp_ent = nullptr;         // T: per chunk (node, LDS slot, child slots, depth), deepest first
+    int2* rp_cinfo = nullptr;       // chunks: (internal nodes, deepest level | treelet roots << 16)
+    int2* rp_ent = nullptr;         // T: per chunk (node, slots | depth), deepest first, then the
+                                    //    roots' hand-over offsets
     int4* rp_tent = nullptr;        // T: top-tree (node, child, child, depth), unordered
     int4* rp_tlist = nullptr;       // T: the same, deepest first, top children as list positions
     int32_t* rp_tpos = nullptr;     // T-1: list position of each top node

@@ -716,6 +716,13 @@ static_assert(kChunkCap % kBlock == 0, "chunk capacity is a whole number of bloc
 static_assert(kChunkCap <= 65536, "slots are packed in 16 bits");
 
 __device__ inline bool rp_is_top(const DevState& d, int32_t p) { return d.int_ry[p] - d.int_rx[p] >= kTreelet; }
+// Chunk entry (node, own slot | right child's slot << 10 | depth << 20); the left child's slot
+// is the node's own. Slots are below kChunkCap, depths below kTreelet.
+static_assert(kChunkCap <= 1024 && kTreelet <= 1024, "chunk entries pack slots and depths in 10 bits");
+__host__ __device__ inline int32_t rp_pack(int32_t own, int32_t rs, int32_t depth) { return own | (rs << 10) | (depth << 20); }
+__device__ inline int32_t rp_depth(const int2& v) { return v.y >> 20; }
+__device__ inline int32_t rp_own(const int2& v) { return v.y & 0x3FF; }
+__device__ inline int32_t rp_right(const int2& v) { return (v.y >> 10) & 0x3FF; }
 
 // Treelet starts (atomic min per window) and the unordered top-tree list.
 __global__ void __launch_bounds__(kBlock) k_rp_starts(int32_t n, DevState d) {
@@ -746,7 +753,7 @@ __global__ void __launch_bounds__(kBlock) k_rp_starts(int32_t n, DevState d) {
 __global__ void __launch_bounds__(kBlock) k_rp_walk(int32_t n, DevState d) {
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
-    const int4 empty = make_int4(-1, -1, 0, -1);
+    const int2 empty = make_int2(-1, -1);
     const int32_t p = d.leaf_par[x];
     int32_t len = 0, r = -1;
     for (int32_t q = p; q >= 0 && len < kTreelet && !rp_is_top(d, q); q = d.int_par[q]) {
@@ -761,56 +768,67 @@ __global__ void __launch_bounds__(kBlock) k_rp_walk(int32_t n, DevState d) {
     if (x == d.int_ry[r]) d.rp_ent[x] = empty;
     const int32_t cs = d.rp_cstart[a / kTreelet];
     const int32_t own = x - cs;  // slot of every node whose leftmost leaf is x (and of x)
-    const bool lds_top = d.rp_tinfo[0] <= kTopLds;
     int32_t q = p;
     for (int32_t i = 0; i < len && d.int_rx[q] == x; ++i) {
         const int4 tp = d.itopo[q];
         const int32_t rs = ((tp.w & 2) ? tp.y : d.int_rx[tp.y]) - cs;
-        // the treelet root hands its box to the top kernel: dense offset from its parent's
-        // entry, side by mark bit 2 (right child)
-        int32_t dest = -1;
-        if (q == r && tp.z >= 0 && lds_top) {
-            const int4 pe = d.rp_tlist[d.rp_tpos[tp.z]];  // the parent's entry, offsets per side
-            dest = (tp.w & 4) ? pe.z : pe.y;
-        }
-        d.rp_ent[q - r + a] = make_int4(q, own | ((len - 1 - i) << 16), own | (rs << 16), dest);
+        d.rp_ent[q - r + a] = make_int2(q, rp_pack(own, rs, len - 1 - i));
         q = tp.z;
     }
 }
 
-// Per chunk: entries sorted in place by depth, deepest first; (count, deepest depth).
+// Per chunk: entries sorted in place by depth, deepest first; (count, deepest depth | treelet
+// roots << 16). The roots (depth 0) end the list; each one's hand-over offset for the top kernel
+// (or -1) follows the list at the root's position + roots: a chunk's treelets of L >= 2 leaves
+// hold L - 1 entries each, so the chunk's leaf range has room for one more per treelet.
 __global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
     __shared__ uint32_t hist[kDepthCap];
-    __shared__ uint32_t dmax;
+    __shared__ uint32_t dmax, roots;
     const int32_t c = blockIdx.x;
     const int32_t s = d.rp_cstart[c], e = d.rp_cstart[c + 1];
     for (int i = threadIdx.x; i < kDepthCap; i += kBlock) hist[i] = 0;
     if (threadIdx.x == 0) dmax = 0;
     __syncthreads();
-    int4 v[kRefitPer];
+    int2 v[kRefitPer];
 #pragma unroll
     for (int k = 0; k < kRefitPer; ++k) {
         const int32_t i = s + threadIdx.x + k * kBlock;
-        v[k] = i < e ? d.rp_ent[i] : make_int4(-1, -1, 0, -1);
+        v[k] = i < e ? d.rp_ent[i] : make_int2(-1, -1);
         if (v[k].x >= 0) {
-            atomicAdd(&hist[v[k].y >> 16], 1u);
-            atomicMax(&dmax, (uint32_t)(v[k].y >> 16));
+            atomicAdd(&hist[rp_depth(v[k])], 1u);
+            atomicMax(&dmax, (uint32_t)rp_depth(v[k]));
         }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        roots = hist[0];
         uint32_t acc = 0;
         for (int t = (int)dmax; t >= 0; --t) {
             const uint32_t h = hist[t];
             hist[t] = acc;
             acc += h;
         }
-        d.rp_cinfo[c] = make_int2((int32_t)acc, (int32_t)dmax);
+        d.rp_cinfo[c] = make_int2((int32_t)acc, (int32_t)(dmax | (roots << 16)));
     }
     __syncthreads();
+    const bool lds_top = d.rp_tinfo[0] <= kTopLds;
 #pragma unroll
-    for (int k = 0; k < kRefitPer; ++k)
-        if (v[k].x >= 0) d.rp_ent[s + atomicAdd(&hist[v[k].y >> 16], 1u)] = v[k];
+    for (int k = 0; k < kRefitPer; ++k) {
+        if (v[k].x < 0) continue;
+        const uint32_t pos = s + atomicAdd(&hist[rp_depth(v[k])], 1u);
+        d.rp_ent[pos] = v[k];
+        if (rp_depth(v[k]) == 0) {
+            // the treelet root hands its box to the top kernel: dense offset from its parent's
+            // entry, side by mark bit 2 (right child)
+            const int4 tp = d.itopo[v[k].x];
+            int32_t dest = -1;
+            if (tp.z >= 0 && lds_top) {
+                const int4 pe = d.rp_tlist[d.rp_tpos[tp.z]];
+                dest = (tp.w & 4) ? pe.z : pe.y;
+            }
+            d.rp_ent[pos + roots] = make_int2(dest, 0);
+        }
+    }
 }
 
 // Top tree sorted deepest first; level t occupies [rp_tlev[t], t ? rp_tlev[t-1] : count).
@@ -927,7 +945,7 @@ __global__ void __launch_bounds__(kRefitBlock) k_refit(int32_t n, DevState d) {
     __shared__ Box sb[kChunkCap];
     const int32_t c = blockIdx.x;
     const int32_t s = d.rp_cstart[c], e = d.rp_cstart[c + 1];
-    const int2 info = d.rp_cinfo[c];  // (internal nodes, deepest level)
+    const int2 info = d.rp_cinfo[c];  // (internal nodes, deepest level | treelet roots << 16)
     // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase. The loads of a
     // leaf run one after another: measured faster than issuing all of a thread's loads at once
     // (k_refit 52 vs 63-69 us on the 2M-triangle cloth), the gathers of neighbouring waves
@@ -959,21 +977,29 @@ __global__ void __launch_bounds__(kRefitBlock) k_refit(int32_t n, DevState d) {
             store_box(&d.leaf[x], b);
         }
     }
-    int4 v[kRefitPerT];
+    int2 v[kRefitPerT];
 #pragma unroll
     for (int k = 0; k < kRefitPerT; ++k) {
         const int32_t i = threadIdx.x + k * kRefitBlock;
-        v[k] = i < info.x ? d.rp_ent[s + i] : make_int4(-1, -1, 0, -1);
+        v[k] = i < info.x ? d.rp_ent[s + i] : make_int2(-1, -1);
     }
     Box out[kRefitPerT];
     bad = __syncthreads_or(bad);
     if (threadIdx.x == 0) d.rp_cbad[c] = (uint32_t)bad;
-    for (int32_t lev = info.y; lev >= 0; --lev) {
+    // treelet roots: their hand-over offsets, loaded now and used at the end
+    const int32_t roots = info.y >> 16;
+    int32_t dest[kRefitPerT];
+#pragma unroll
+    for (int k = 0; k < kRefitPerT; ++k) {
+        const int32_t i = threadIdx.x + k * kRefitBlock;
+        dest[k] = v[k].x >= 0 && rp_depth(v[k]) == 0 ? d.rp_ent[s + i + roots].x : -1;
+    }
+    for (int32_t lev = info.y & 0xFFFF; lev >= 0; --lev) {
 #pragma unroll
         for (int k = 0; k < kRefitPerT; ++k) {
-            if ((v[k].y >> 16) == lev) {
-                out[k] = box_merge(sb[v[k].z & 0xFFFF], sb[v[k].z >> 16]);
-                sb[v[k].y & 0xFFFF] = out[k];
+            if (rp_depth(v[k]) == lev) {
+                out[k] = box_merge(sb[rp_own(v[k])], sb[rp_right(v[k])]);
+                sb[rp_own(v[k])] = out[k];
             }
         }
         __syncthreads();
@@ -984,8 +1010,8 @@ __global__ void __launch_bounds__(kRefitBlock) k_refit(int32_t n, DevState d) {
         store_box(&d.inode[v[k].x], out[k]);
         // a treelet root under an LDS-sized top tree also hands its box to the top kernel, which
         // then loads it without a dependent load
-        if (v[k].w >= 0) {
-            float* dst = d.rp_tcb + 6 * (int64_t)v[k].w;
+        if (dest[k] >= 0) {
+            float* dst = d.rp_tcb + 6 * (int64_t)dest[k];
             dst[0] = out[k].mnx; dst[1] = out[k].mny; dst[2] = out[k].mnz;
             dst[3] = out[k].mxx; dst[4] = out[k].mxy; dst[5] = out[k].mxz;
         }
