@@ -946,10 +946,11 @@ __global__ void __launch_bounds__(kRefitBlock) k_refit(int32_t n, DevState d) {
     const int32_t c = blockIdx.x;
     const int32_t s = d.rp_cstart[c], e = d.rp_cstart[c + 1];
     const int2 info = d.rp_cinfo[c];  // (internal nodes, deepest level | treelet roots << 16)
-    // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase. The loads of a
-    // leaf run one after another: measured faster than issuing all of a thread's loads at once
-    // (k_refit 52 vs 63-69 us on the 2M-triangle cloth), the gathers of neighbouring waves
-    // then share lines.
+    // leaf boxes: LDS + records; per-triangle finiteness for the narrow phase. A leaf's face
+    // indices arrive in one load and its three vertices together, the leaves of a thread one
+    // after another: measured fastest on the 2M-triangle cloth (k_refit 46.8 us, against 50.4
+    // with every index and vertex loaded on its own and 63-69 with all of a thread's loads in
+    // flight at once, which congests the gathers).
     int bad = 0;
 #pragma unroll
     for (int k = 0; k < kRefitPerT; ++k) {
@@ -961,15 +962,24 @@ __global__ void __launch_bounds__(kRefitBlock) k_refit(int32_t n, DevState d) {
                 b = given_box(d.pbox, d.prim_idx[x]);
                 fin = false;
             } else {
-                const int32_t* f = d.lfaces + 3 * (int64_t)x;
+                const int32_t* fp = d.lfaces + 3 * (int64_t)x;
+                const int32_t f[3] = {fp[0], fp[1], fp[2]};
+                float pv[3][3];
+#pragma unroll
                 for (int j = 0; j < 3; ++j) {
                     const float* v = d.verts + 3 * (int64_t)f[j];
-                    const float vx = v[0], vy = v[1], vz = v[2];
+                    pv[j][0] = v[0]; pv[j][1] = v[1]; pv[j][2] = v[2];
+                }
+                int ok = 1;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
                     // |x| <= 1e8 keeps every SAT product finite (dots of edge x normal crosses
                     // stay below 1536 * 1e32): no NaN can drop the shared vertex's projection.
-                    fin = fin && fabsf(vx) <= 1e8f && fabsf(vy) <= 1e8f && fabsf(vz) <= 1e8f;
-                    box_add(b, vx, vy, vz);
+                    ok &= (int)(fabsf(pv[j][0]) <= 1e8f) & (int)(fabsf(pv[j][1]) <= 1e8f) &
+                          (int)(fabsf(pv[j][2]) <= 1e8f);
+                    box_add(b, pv[j][0], pv[j][1], pv[j][2]);
                 }
+                fin = ok != 0;
             }
             d.tri_finite[x] = fin ? 1 : 0;
             bad |= fin ? 0 : 1;
