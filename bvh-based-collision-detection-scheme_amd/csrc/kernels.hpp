@@ -49,7 +49,7 @@ void launch_refit_prep(DevState& d, int32_t n, hipStream_t s);  // after every t
 int32_t refit_chunks(int32_t n);  // refit workgroups (treelet chunks) for n leaves
 // Top trees of up to this many nodes are merged in LDS; the top list and the hand-over boxes
 // (2 per list position) are allocated for at least this many entries.
-constexpr int32_t kRefitTopLds = 6144;
+constexpr int32_t kRefitTopLds = 4096;
 void launch_prim_map(DevState& d, int32_t n, hipStream_t s);
 
 // front.hip

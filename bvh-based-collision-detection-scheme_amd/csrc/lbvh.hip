@@ -21,6 +21,8 @@
 //  * Refit is level-synchronous over treelet-aligned chunks: each workgroup merges its
 //    treelets' nodes one depth level per barrier in LDS (no atomics), then one workgroup
 //    merges the small top tree above the treelets.
+#include <type_traits>
+
 #include "bvhcd_internal.hpp"
 #include "kernels.hpp"
 
@@ -1042,6 +1044,13 @@ __device__ inline Box below_top_box(const DevState& d, int kind, int32_t ref) {
     return {p[0], p[1], p[2], q[0], q[1], q[2]};
 }
 
+// Record box of node v (kind 1: leaf, otherwise internal), without a branch.
+__device__ inline Box below_top_record(const DevState& d, int kind, int32_t v) {
+    const NodeRec* r = kind == 1 ? &d.leaf[v] : &d.inode[v];
+    const float* p = reinterpret_cast<const float*>(r);
+    return {p[0], p[1], p[2], p[4], p[5], p[6]};
+}
+
 // One workgroup. Top trees of up to kTopLds nodes stay in LDS, slot = list position: a node
 // whose children are both below the top tree (treelet roots) is merged at once, a node with
 // one such child starts its slot with that child's box and merges the other at its level.
@@ -1143,15 +1152,36 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
         if (i < m) store_box(&d.inode[ew.x], tb[i]);
         return;
     }
-    // larger top trees: children are node ids here (kinds 0 and 1 only)
+    // larger top trees: children are node ids here (kinds 0 and 1 only), merged level by level
+    // through the node records (records this workgroup wrote: the barrier orders them). Level
+    // bounds sit in LDS; a level goes in batches of up to four entries per thread, each batch's
+    // entry loads, then its child-record loads, all in flight together.
+    int32_t* ls = reinterpret_cast<int32_t*>(tb);  // ls[t]: first position of level t
+    for (int32_t t = threadIdx.x; t <= top; t += kTopBlock) ls[t] = d.rp_tlev[t];
+    __syncthreads();
     for (int32_t lev = top; lev >= 0; --lev) {
-        const int32_t b0 = d.rp_tlev[lev], b1 = lev > 0 ? d.rp_tlev[lev - 1] : m;
-        for (int32_t i = b0 + (int32_t)threadIdx.x; i < b1; i += kTopBlock) {
-            const int4 e = d.rp_tlist[i];
-            const Box l = record_box(d, e.y, top_kind(e, 0));
-            const Box r = record_box(d, e.z, top_kind(e, 1));
-            store_box(&d.inode[e.x], box_merge(l, r));
-        }
+        const int32_t b0 = ls[lev], b1 = lev > 0 ? ls[lev - 1] : m;
+        // batches of K rounds of kTopBlock entries, K = 4 while more than two rounds remain, then
+        // 2, then 1; indices clamped (a repeated entry is merged twice into the same box)
+        auto batch = [&](int32_t base, auto kc) {
+            constexpr int K = decltype(kc)::value;
+            int4 e[K];
+            Box l[K], r[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) e[j] = d.rp_tlist[min(base + (int32_t)threadIdx.x + j * kTopBlock, b1 - 1)];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                l[j] = below_top_record(d, top_kind(e[j], 0), e[j].y);
+                r[j] = below_top_record(d, top_kind(e[j], 1), e[j].z);
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (base + (int32_t)threadIdx.x + j * kTopBlock < b1) store_box(&d.inode[e[j].x], box_merge(l[j], r[j]));
+        };
+        int32_t base = b0;
+        for (; b1 - base > 2 * kTopBlock; base += 4 * kTopBlock) batch(base, std::integral_constant<int, 4>{});
+        for (; b1 - base > kTopBlock; base += 2 * kTopBlock) batch(base, std::integral_constant<int, 2>{});
+        for (; base < b1; base += kTopBlock) batch(base, std::integral_constant<int, 1>{});
         __syncthreads();
     }
 }
