@@ -122,12 +122,17 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.rp_cinfo, (size_t)refit_chunks((int32_t)T));
     ok &= dalloc(c, d.rp_ent, T);
     ok &= dalloc(c, d.rp_tent, T);
-    ok &= dalloc(c, d.rp_tlist, std::max<size_t>(T, kRefitTopLds));
+    ok &= dalloc(c, d.rp_tlist, (size_t)T + kRefitTopLds);
     ok &= dalloc(c, d.rp_tpos, T);
     ok &= dalloc(c, d.rp_tlev, 1024);
-    ok &= dalloc(c, d.rp_tinfo, 4);
+    ok &= dalloc(c, d.rp_tinfo, 8);
+    ok &= dalloc(c, d.rp_seg, kRefitTopGrid);
+    ok &= dalloc(c, d.rp_tsz, T);
+    ok &= dalloc(c, d.rp_tseg, T);
+    ok &= dalloc(c, d.rp_tdone, 1);
+    ok &= dalloc(c, d.rp_tdeep, 1);
     ok &= dalloc(c, d.rp_tcount, 1);
-    ok &= dalloc(c, d.rp_tcb, 6 * 2 * (size_t)kRefitTopLds);
+    ok &= dalloc(c, d.rp_tcb, 6 * (size_t)refit_tcb_boxes((int64_t)T));
     ok &= dalloc(c, d.rp_cbad, (size_t)refit_chunks((int32_t)T));
     ok &= dalloc(c, d.leaf_par, T);
     ok &= dalloc(c, d.leaf_mark, T);

@@ -347,12 +347,17 @@ int bvhcd_batch_create(const bvhcd_config* cfg, int32_t S, const int64_t* tps, c
     ok &= balloc(b, d.rp_cinfo, (size_t)refit_chunks((int32_t)T));
     ok &= balloc(b, d.rp_ent, T);
     ok &= balloc(b, d.rp_tent, T);
-    ok &= balloc(b, d.rp_tlist, std::max<size_t>(T, kRefitTopLds));
+    ok &= balloc(b, d.rp_tlist, (size_t)T + kRefitTopLds);
     ok &= balloc(b, d.rp_tpos, T);
     ok &= balloc(b, d.rp_tlev, 1024);
-    ok &= balloc(b, d.rp_tinfo, 4);
+    ok &= balloc(b, d.rp_tinfo, 8);
+    ok &= balloc(b, d.rp_seg, kRefitTopGrid);
+    ok &= balloc(b, d.rp_tsz, T);
+    ok &= balloc(b, d.rp_tseg, T);
+    ok &= balloc(b, d.rp_tdone, 1);
+    ok &= balloc(b, d.rp_tdeep, 1);
     ok &= balloc(b, d.rp_tcount, 1);
-    ok &= balloc(b, d.rp_tcb, 6 * 2 * (size_t)kRefitTopLds);
+    ok &= balloc(b, d.rp_tcb, 6 * (size_t)refit_tcb_boxes((int64_t)T));
     ok &= balloc(b, d.rp_cbad, (size_t)refit_chunks((int32_t)T));
     ok &= balloc(b, d.leaf_par, T);
     ok &= balloc(b, d.leaf_mark, T);

@@ -162,15 +162,24 @@ struct DevState {
     int2* rp_ent = nullptr;         // T: per chunk (node, slots | depth), deepest first, then the
                                     //    roots' hand-over offsets
     int4* rp_tent = nullptr;        // T: top-tree (node, child, child, depth), unordered
-    int4* rp_tlist = nullptr;       // T: the same, deepest first, top children as list positions
+    int4* rp_tlist = nullptr;       // T + 4096: the same, deepest first (or by segment), top
+                                    //    children as positions, hand-over offsets
     int32_t* rp_tpos = nullptr;     // T-1: list position of each top node
     int32_t* rp_tlev = nullptr;     // 1024: first entry of each top-tree depth
-    int32_t* rp_tinfo = nullptr;    // 4: (top nodes, deepest level or -1, deepest of the levels
-                                    //     one wave merges or -1, their first list position)
+    int32_t* rp_tinfo = nullptr;    // 8: (top nodes, deepest level or -1, unused, unused,
+                                    //     segments or -1 (all through the records), first upper
+                                    //     position, upper nodes, deepest upper level)
+    int4* rp_seg = nullptr;         // 64: per segment (first position, nodes, deepest level,
+                                    //     (last level merged per barrier + 1) | one-wave start << 16)
+    int32_t* rp_tsz = nullptr;      // T: top-tree prep scratch (subtree sizes, new positions),
+                                    //    climb flags per node when the top tree is too deep
+    int32_t* rp_tseg = nullptr;     // T: top-tree prep scratch (segment per position)
+    uint32_t* rp_tdone = nullptr;   // 1: segment workgroups finished (reset by the last one)
+    uint32_t* rp_tdeep = nullptr;   // 1: the top tree is deeper than the level bins
     uint32_t* rp_tcount = nullptr;  // 1: append counter
     uint32_t* rp_cbad = nullptr;    // chunks: some leaf of the chunk has a non-finite coordinate
-    float* rp_tcb = nullptr;        // 2 x 6144 boxes: treelet-root children of the top tree by
-                                    // (list position, side), written by the chunk refit
+    float* rp_tcb = nullptr;        // refit_tcb_boxes(T) boxes: treelet-root children of segment
+                                    // nodes, dense in list order, written by the chunk refit
     float* sa_ref = nullptr;        // T-1 internal-node surface area at its last rebuild
     float* sa_ref_leaf = nullptr;   // T leaf-box surface area at its last rebuild
     int32_t* built_frame = nullptr; // T frame at which each leaf's subtree was last rebuilt

@@ -47,9 +47,13 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
 void launch_refit(DevState& d, int32_t n, hipStream_t s);
 void launch_refit_prep(DevState& d, int32_t n, hipStream_t s);  // after every topology change
 int32_t refit_chunks(int32_t n);  // refit workgroups (treelet chunks) for n leaves
-// Top trees of up to this many nodes are merged in LDS; the top list and the hand-over boxes
-// (2 per list position) are allocated for at least this many entries.
+// Top trees of up to this many nodes are merged in LDS by one workgroup; larger ones are split
+// into at most kRefitTopGrid segments of up to this size (one workgroup each) and the upper
+// nodes above them. The top list holds kRefitTopLds entries past its end (read ahead).
 constexpr int32_t kRefitTopLds = 4096;
+constexpr int32_t kRefitTopGrid = 64;
+// Capacity (boxes) of the hand-over array between the chunk refit and the top-tree merge.
+inline int64_t refit_tcb_boxes(int64_t T) { return 2 * (int64_t)kRefitTopLds + T / 128; }
 void launch_prim_map(DevState& d, int32_t n, hipStream_t s);
 
 // front.hip

@@ -713,6 +713,7 @@ constexpr int kTopBlock = 1024;
 constexpr int kRefitBlock = 512;                       // per-frame chunk refit: 8 waves per chunk
 constexpr int kRefitPerT = kChunkCap / kRefitBlock;     // its leaves / nodes per thread
 constexpr int kTopLds = kRefitTopLds;  // top trees up to this size are merged in LDS
+constexpr int kTopGrid = kRefitTopGrid;  // at most this many LDS segments of larger top trees
 constexpr int kTopPer = kTopLds / kTopBlock;
 static_assert(kChunkCap % kBlock == 0, "chunk capacity is a whole number of block passes");
 static_assert(kChunkCap <= 65536, "slots are packed in 16 bits");
@@ -740,8 +741,11 @@ __global__ void __launch_bounds__(kBlock) k_rp_starts(int32_t n, DevState d) {
         const int32_t par = d.int_par[x];
         if (!top && (par < 0 || rp_is_top(d, par))) atomicMin(&d.rp_cstart[rx / kTreelet], rx);
         if (top) {
-            int32_t depth = 0;
-            for (int32_t q = par; q >= 0 && depth < kDepthCap - 1; q = d.int_par[q]) ++depth;
+            // depth below the root; a top tree deeper than the level bins (chains of equal codes)
+            // is merged by climbing instead
+            int32_t depth = 0, q = par;
+            for (; q >= 0 && depth < kDepthCap - 1; q = d.int_par[q]) ++depth;
+            if (q >= 0) atomicOr(d.rp_tdeep, 1u);
             const int4 tp = d.itopo[x];
             e = make_int4(x, tp.x, tp.y, depth | ((tp.w & 1) << 16) | ((tp.w & 2) << 17));
         }
@@ -813,7 +817,9 @@ __global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
         d.rp_cinfo[c] = make_int2((int32_t)acc, (int32_t)(dmax | (roots << 16)));
     }
     __syncthreads();
-    const bool lds_top = d.rp_tinfo[0] <= kTopLds;
+    // hand-over offsets exist for children of segment nodes (list positions below the upper
+    // nodes), unless the top tree goes through the records as a whole
+    const int32_t seg_end = d.rp_tinfo[4] > 0 ? d.rp_tinfo[5] : 0;
 #pragma unroll
     for (int k = 0; k < kRefitPer; ++k) {
         if (v[k].x < 0) continue;
@@ -824,7 +830,7 @@ __global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
             // entry, side by mark bit 2 (right child)
             const int4 tp = d.itopo[v[k].x];
             int32_t dest = -1;
-            if (tp.z >= 0 && lds_top) {
+            if (tp.z >= 0 && d.rp_tpos[tp.z] < seg_end) {
                 const int4 pe = d.rp_tlist[d.rp_tpos[tp.z]];
                 dest = (tp.w & 4) ? pe.z : pe.y;
             }
@@ -844,6 +850,17 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
     __shared__ uint32_t hist[kDepthCap];
     __shared__ uint32_t dmax;
     const int32_t m = (int32_t)*d.rp_tcount;
+    if (*d.rp_tdeep) {  // deeper than the level bins: no list, the top kernel climbs (rp_tent)
+        if (threadIdx.x == 0) {
+            d.rp_tinfo[0] = m;
+            d.rp_tinfo[1] = -1;
+            d.rp_tinfo[4] = -2;
+            d.rp_tinfo[5] = 0;
+            d.rp_tinfo[6] = 0;
+            d.rp_tinfo[7] = -1;
+        }
+        return;
+    }
     for (int i = threadIdx.x; i < kDepthCap; i += kTopBlock) hist[i] = 0;
     if (threadIdx.x == 0) dmax = 0;
     __syncthreads();
@@ -870,8 +887,13 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
             l1 = t;
             p1 = (int32_t)hist[t];
         }
-        d.rp_tinfo[2] = l1;
-        d.rp_tinfo[3] = p1;
+        // one segment holding the whole top tree, no upper nodes; larger trees are split by
+        // k_rp_top_segments, which overwrites these
+        d.rp_tinfo[4] = 1;
+        d.rp_tinfo[5] = m;
+        d.rp_tinfo[6] = 0;
+        d.rp_tinfo[7] = -1;
+        d.rp_seg[0] = make_int4(0, m, m > 0 ? (int32_t)dmax : -1, (l1 + 1) | (p1 << 16));
     }
     __syncthreads();
     for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
@@ -924,6 +946,144 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
     }
 }
 
+// Top trees of more than kTopLds nodes: segments (maximal top subtrees of at most kTopLds
+// nodes), merged in LDS by one k_refit_top workgroup each, and the upper nodes above them,
+// merged through the records by the segment workgroup that finishes last. One workgroup, on the
+// list k_rp_top_sort left sorted deepest first with node ids as children:
+//  1. top-tree subtree sizes bottom-up, one level per barrier;
+//  2. segment ids top-down: a node whose size fits and whose parent's does not starts one;
+//  3. the list regrouped by segment (segment 0 first, upper nodes last), top children of
+//     segment nodes as positions within their segment, node positions renumbered, and dense
+//     hand-over offsets for the segment nodes' treelet-root children in list order.
+// More than kTopGrid segments, or more hand-over boxes than the buffer holds, leave the list as
+// it is and send the whole top tree through the records (rp_tinfo[4] = -1).
+__global__ void __launch_bounds__(kTopBlock) k_rp_top_segments(DevState d, int32_t tcb_cap) {
+    const int32_t m = d.rp_tinfo[0];
+    if (m <= kTopLds || d.rp_tinfo[4] == -2) return;
+    const int32_t top = d.rp_tinfo[1];
+    __shared__ int32_t ls[kDepthCap];
+    __shared__ uint32_t nseg, nhand, cnt[kTopGrid + 1], fill[kTopGrid + 1], deep[kTopGrid + 1], mind[kTopGrid + 1];
+    __shared__ uint32_t part[kTopBlock];
+    __shared__ int32_t sbase[kTopGrid + 1];
+    for (int32_t t = threadIdx.x; t <= top; t += kTopBlock) ls[t] = d.rp_tlev[t];
+    if (threadIdx.x == 0) { nseg = 0; nhand = 0; }
+    __syncthreads();
+    int32_t* ts = d.rp_tsz;   // per list position: top nodes in the subtree, later the new position
+    int32_t* sg = d.rp_tseg;  // per list position: segment, -1 for upper nodes
+    for (int32_t lev = top; lev >= 0; --lev) {
+        const int32_t b0 = ls[lev], b1 = lev > 0 ? ls[lev - 1] : m;
+        for (int32_t i = b0 + (int32_t)threadIdx.x; i < b1; i += kTopBlock) {
+            const int4 e = d.rp_tlist[i];
+            int32_t sz = 1;
+            if (top_kind(e, 0) == 0 && rp_is_top(d, e.y)) sz += ts[d.rp_tpos[e.y]];
+            if (top_kind(e, 1) == 0 && rp_is_top(d, e.z)) sz += ts[d.rp_tpos[e.z]];
+            ts[i] = sz;
+        }
+        __syncthreads();
+    }
+    for (int32_t lev = 0; lev <= top; ++lev) {
+        const int32_t b0 = ls[lev], b1 = lev > 0 ? ls[lev - 1] : m;
+        for (int32_t i = b0 + (int32_t)threadIdx.x; i < b1; i += kTopBlock) {
+            const int32_t par = d.itopo[d.rp_tlist[i].x].z;  // the parent of a top node is a top node
+            const int32_t pi = par >= 0 ? d.rp_tpos[par] : -1;
+            int32_t sid = -1;
+            if (ts[i] <= kTopLds) sid = (pi < 0 || ts[pi] > kTopLds) ? (int32_t)atomicAdd(&nseg, 1u) : sg[pi];
+            sg[i] = sid;
+        }
+        __syncthreads();
+    }
+    const uint32_t ns = nseg;
+    for (uint32_t g = threadIdx.x; g <= ns && g <= kTopGrid; g += kTopBlock) {
+        cnt[g] = 0; fill[g] = 0; deep[g] = 0; mind[g] = 0xFFFF;
+    }
+    __syncthreads();
+    if (ns > kTopGrid) {
+        if (threadIdx.x == 0) d.rp_tinfo[4] = -1;
+        return;
+    }
+    for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
+        const int4 e = d.rp_tlist[i];
+        const int32_t gi = sg[i] < 0 ? (int32_t)ns : sg[i];
+        atomicAdd(&cnt[gi], 1u);
+        atomicMax(&deep[gi], (uint32_t)top_depth(e));
+        atomicMin(&mind[gi], (uint32_t)top_depth(e));
+        if (sg[i] >= 0) {
+            const uint32_t h = (top_kind(e, 0) == 0 && !rp_is_top(d, e.y) ? 1u : 0u) +
+                               (top_kind(e, 1) == 0 && !rp_is_top(d, e.z) ? 1u : 0u);
+            if (h) atomicAdd(&nhand, h);
+        }
+    }
+    __syncthreads();
+    if (nhand > (uint32_t)tcb_cap) {
+        if (threadIdx.x == 0) d.rp_tinfo[4] = -1;
+        return;
+    }
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (uint32_t g = 0; g <= ns; ++g) {
+            sbase[g] = acc;
+            acc += (int32_t)cnt[g];
+        }
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
+        const int32_t gi = sg[i] < 0 ? (int32_t)ns : sg[i];
+        ts[i] = sbase[gi] + (int32_t)atomicAdd(&fill[gi], 1u);
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
+        int4 e = d.rp_tlist[i];
+        const int32_t gi = sg[i];
+        if (gi >= 0) {
+            if (top_kind(e, 0) == 0 && rp_is_top(d, e.y)) {
+                e.y = ts[d.rp_tpos[e.y]] - sbase[gi];
+                e.w |= 2 << 16;
+            }
+            if (top_kind(e, 1) == 0 && rp_is_top(d, e.z)) {
+                e.z = ts[d.rp_tpos[e.z]] - sbase[gi];
+                e.w |= 2 << 18;
+            }
+        }
+        d.rp_tent[ts[i]] = e;
+    }
+    __syncthreads();
+    for (int32_t j = threadIdx.x; j < m; j += kTopBlock) d.rp_tpos[d.rp_tent[j].x] = j;
+    // dense hand-over offsets over the segment positions, thread t a contiguous run of them
+    const int32_t nsp = sbase[ns];
+    const int32_t per = (nsp + kTopBlock - 1) / kTopBlock;
+    const int32_t j0 = min((int32_t)threadIdx.x * per, nsp), j1 = min(j0 + per, nsp);
+    uint32_t c = 0;
+    for (int32_t j = j0; j < j1; ++j) {
+        const int4 e = d.rp_tent[j];
+        c += (top_kind(e, 0) == 0 ? 1u : 0u) + (top_kind(e, 1) == 0 ? 1u : 0u);
+    }
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 1; off < kTopBlock; off <<= 1) {  // inclusive scan (prep only)
+        const uint32_t add = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int32_t o = (int32_t)(part[threadIdx.x] - c);
+    for (int32_t j = j0; j < j1; ++j) {
+        int4 e = d.rp_tent[j];
+        if (top_kind(e, 0) == 0) e.y = o++;
+        if (top_kind(e, 1) == 0) e.z = o++;
+        d.rp_tlist[j] = e;
+    }
+    for (int32_t j = nsp + (int32_t)threadIdx.x; j < m; j += kTopBlock) d.rp_tlist[j] = d.rp_tent[j];
+    // segments: merged from their deepest level down to their root's (no one-wave levels)
+    for (uint32_t g = threadIdx.x; g < ns; g += kTopBlock)
+        d.rp_seg[g] = make_int4(sbase[g], (int32_t)cnt[g], (int32_t)deep[g], (int32_t)mind[g] | ((int32_t)cnt[g] << 16));
+    if (threadIdx.x == 0) {
+        d.rp_tinfo[4] = (int32_t)ns;
+        d.rp_tinfo[5] = nsp;
+        d.rp_tinfo[6] = (int32_t)cnt[ns];
+        d.rp_tinfo[7] = (int32_t)deep[ns];
+    }
+}
+
 int32_t refit_chunks(int32_t n) { return (n + kTreelet - 1) / kTreelet; }
 
 void launch_refit_prep(DevState& d, int32_t n, hipStream_t s) {
@@ -932,7 +1092,11 @@ void launch_refit_prep(DevState& d, int32_t n, hipStream_t s) {
     fill_i32(d.rp_cstart, nc + 1, n, s);
     fill_u32(d.rp_tcount, 1, 0u, s);
     hipLaunchKernelGGL(k_rp_starts, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d);
-    hipLaunchKernelGGL(k_rp_top_sort, dim3(1), dim3(kTopBlock), 0, s, d);  // positions, read by the walk
+    fill_u32(d.rp_tdone, 1, 0u, s);
+    fill_u32(d.rp_tdeep, 1, 0u, s);
+    fill_i32(d.rp_tsz, n, 0, s);  // climb flags of deep top trees (prep scratch otherwise)
+    hipLaunchKernelGGL(k_rp_top_sort, dim3(1), dim3(kTopBlock), 0, s, d);
+    hipLaunchKernelGGL(k_rp_top_segments, dim3(1), dim3(kTopBlock), 0, s, d, refit_tcb_boxes(n));
     hipLaunchKernelGGL(k_rp_walk, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d);
     hipLaunchKernelGGL(k_rp_chunk_sort, dim3(nc), dim3(kBlock), 0, s, d);
 }
@@ -1051,6 +1215,78 @@ __device__ inline Box below_top_record(const DevState& d, int kind, int32_t v) {
     return {p[0], p[1], p[2], p[4], p[5], p[6]};
 }
 
+// Top-tree nodes merged level by level through the node records (children are node ids,
+// kinds 0 and 1; records of this workgroup are ordered by its barriers). whole: the list sorted
+// deepest first with level starts in rp_tlev (m nodes, deepest level top); otherwise the m upper
+// nodes after the segments, deepest level top, in no order (each level scans them). A level goes
+// in batches of up to four entries per thread, a batch's entry loads and then its child-record
+// loads all in flight together.
+__device__ inline void refit_top_records(const DevState& d, Box* lds, int32_t m, int32_t top, bool whole) {
+    int32_t* ls = reinterpret_cast<int32_t*>(lds);  // ls[t]: first position of level t
+    const int32_t ub = d.rp_tinfo[5];               // first upper position
+    if (whole)
+        for (int32_t t = threadIdx.x; t <= top; t += kTopBlock) ls[t] = d.rp_tlev[t];
+    __syncthreads();
+    for (int32_t lev = top; lev >= 0; --lev) {
+        const int32_t b0 = whole ? ls[lev] : ub, b1 = whole ? (lev > 0 ? ls[lev - 1] : m) : ub + m;
+        // batches of K rounds of kTopBlock entries, K = 4 while more than two rounds remain, then
+        // 2, then 1; indices clamped (a repeated entry is merged twice into the same box)
+        auto batch = [&](int32_t at, auto kc) {
+            constexpr int K = decltype(kc)::value;
+            int4 e[K];
+            Box l[K], r[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) e[j] = d.rp_tlist[min(at + (int32_t)threadIdx.x + j * kTopBlock, b1 - 1)];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                l[j] = below_top_record(d, top_kind(e[j], 0), e[j].y);
+                r[j] = below_top_record(d, top_kind(e[j], 1), e[j].z);
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (at + (int32_t)threadIdx.x + j * kTopBlock < b1 && (whole || top_depth(e[j]) == lev))
+                    store_box(&d.inode[e[j].x], box_merge(l[j], r[j]));
+        };
+        int32_t at = b0;
+        for (; b1 - at > 2 * kTopBlock; at += 4 * kTopBlock) batch(at, std::integral_constant<int, 4>{});
+        for (; b1 - at > kTopBlock; at += 2 * kTopBlock) batch(at, std::integral_constant<int, 2>{});
+        for (; at < b1; at += kTopBlock) batch(at, std::integral_constant<int, 1>{});
+        __syncthreads();
+    }
+}
+
+// Top trees deeper than the level bins (long chains of equal Morton codes): one workgroup
+// climbs from every top node whose children are both below the top tree; at a node with two
+// top children the second arrival merges (flag per node, workgroup-scope acquire/release,
+// reset by the second arrival), at a node with one it merges at once. Each step reads the
+// sibling's record, so the time is the longest chain; only such inputs take this path.
+__device__ inline void refit_top_climb(const DevState& d, int32_t m) {
+    uint32_t* flag = reinterpret_cast<uint32_t*>(d.rp_tsz);
+    for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
+        const int4 e = d.rp_tent[i];
+        const int lk = top_kind(e, 0), rk = top_kind(e, 1);
+        if ((lk == 0 && rp_is_top(d, e.y)) || (rk == 0 && rp_is_top(d, e.z))) continue;
+        Box b = box_merge(below_top_record(d, lk, e.y), below_top_record(d, rk, e.z));
+        store_box(&d.inode[e.x], b);
+        for (int32_t q = e.x;;) {
+            const int4 tq = d.itopo[q];
+            const int32_t u = tq.z;
+            if (u < 0) break;
+            const int4 tu = d.itopo[u];
+            const bool right = (tq.w & 4) != 0;  // q is u's right child
+            const int32_t sib = right ? tu.x : tu.y;
+            const int sk = (right ? (tu.w & 1) : (tu.w & 2)) ? 1 : 0;
+            if (sk == 0 && rp_is_top(d, sib)) {
+                if (__hip_atomic_fetch_add(&flag[u], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) break;
+                flag[u] = 0;
+            }
+            b = box_merge(b, below_top_record(d, sk, sib));
+            store_box(&d.inode[u], b);
+            q = u;
+        }
+    }
+}
+
 // One workgroup. Top trees of up to kTopLds nodes stay in LDS, slot = list position: a node
 // whose children are both below the top tree (treelet roots) is merged at once, a node with
 // one such child starts its slot with that child's box and merges the other at its level.
@@ -1059,15 +1295,12 @@ __device__ inline Box below_top_record(const DevState& d, int kind, int32_t v) {
 
 __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chunks) {
     __shared__ Box tb[kTopLds];
-    // the list is loaded before the top-tree size arrives (the buffer holds kTopLds entries);
-    // positions past it are masked afterwards
-    int4 e[kTopPer];
-#pragma unroll
-    for (int k = 0; k < kTopPer; ++k) e[k] = d.rp_tlist[threadIdx.x + k * kTopBlock];
-    // frame-wide: may KEEP count shared-vertex pairs without a test (every leaf box built from
-    // finite vertices); OR over the chunks, four loads per thread in flight at once (indices
-    // clamped: a repeated flag does not change the OR), issued after the box loads below and
-    // reduced at the first barrier
+    __shared__ uint32_t last;
+    const int32_t g = blockIdx.x;
+    const int32_t nseg = d.rp_tinfo[4];
+    // frame-wide (workgroup 0): may KEEP count shared-vertex pairs without a test (every leaf box
+    // built from finite vertices); OR over the chunks, four loads per thread in flight at once
+    // (indices clamped: a repeated flag does not change the OR)
     auto chunks_bad = [&]() {
         int bad = 0;
         for (int32_t c0 = 0; c0 < chunks; c0 += 4 * kTopBlock) {
@@ -1079,62 +1312,76 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
         }
         return bad;
     };
-    const int32_t m = d.rp_tinfo[0], top = d.rp_tinfo[1];
-    if (m > kTopLds) {
+    if (nseg < 0) {  // too many segments: the whole top tree through the records, deepest first
+        if (g != 0) return;
         const int bad = __syncthreads_or(chunks_bad());
         if (threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
-    } else {
+        if (nseg == -2) refit_top_climb(d, d.rp_tinfo[0]);
+        else refit_top_records(d, tb, d.rp_tinfo[0], d.rp_tinfo[1], true);
+        return;
+    }
+    if (g >= nseg) return;
+    // segment g: positions [base, base + m) of the list, all of its nodes' children in the
+    // segment (kind 2, position relative to base) or below the top tree. Segment 0 starts at
+    // position 0, so its list loads go out before the segment record arrives (the list holds
+    // kTopLds entries past its end); positions past the segment are masked afterwards.
+    int4 e[kTopPer];
+    int32_t base = 0;
+    if (g != 0) base = d.rp_seg[g].x;
 #pragma unroll
-        for (int k = 0; k < kTopPer; ++k)
-            if ((int32_t)threadIdx.x + k * kTopBlock >= m) e[k] = make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
-        // children below the top tree, all loads of the thread in flight together: treelet roots
-        // from the dense hand-over array k_refit wrote, leaves from their records
-        Box cl[kTopPer], cr[kTopPer];
+    for (int k = 0; k < kTopPer; ++k) e[k] = d.rp_tlist[base + threadIdx.x + k * kTopBlock];
+    const int4 sinfo = d.rp_seg[g];  // (base, nodes, deepest level, l1 + 1 | p1 << 16)
+    const int32_t m = sinfo.y, top = sinfo.z, l1 = (sinfo.w & 0xFFFF) - 1, p1 = sinfo.w >> 16;
 #pragma unroll
-        for (int k = 0; k < kTopPer; ++k) {
-            cl[k] = below_top_box(d, top_kind(e[k], 0), e[k].y);
-            cr[k] = below_top_box(d, top_kind(e[k], 1), e[k].z);
+    for (int k = 0; k < kTopPer; ++k)
+        if ((int32_t)threadIdx.x + k * kTopBlock >= m) e[k] = make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
+    // children below the top tree, all loads of the thread in flight together: treelet roots
+    // from the dense hand-over array k_refit wrote, leaves from their records
+    Box cl[kTopPer], cr[kTopPer];
+#pragma unroll
+    for (int k = 0; k < kTopPer; ++k) {
+        cl[k] = below_top_box(d, top_kind(e[k], 0), e[k].y);
+        cr[k] = below_top_box(d, top_kind(e[k], 1), e[k].z);
+    }
+    // the shallowest levels (at most 64 nodes, positions p1..m-1) are merged by wave 0 alone,
+    // one level after another without workgroup barriers; lane l holds position p1 + l
+    int4 ew = make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
+    if (threadIdx.x < 64 && p1 + (int32_t)threadIdx.x < m) ew = d.rp_tlist[base + p1 + threadIdx.x];
+    int bad = g == 0 ? chunks_bad() : 0;
+#pragma unroll
+    for (int k = 0; k < kTopPer; ++k) {
+        const int32_t i = threadIdx.x + k * kTopBlock;
+        const int lk = top_kind(e[k], 0), rk = top_kind(e[k], 1);
+        if (lk != 2 && rk != 2) {
+            tb[i] = box_merge(cl[k], cr[k]);
+            e[k].w |= 0xFFFF;  // done
+        } else if (lk != 2) {
+            tb[i] = cl[k];
+        } else if (rk != 2) {
+            tb[i] = cr[k];
         }
-        // the shallowest levels (at most 64 nodes, positions p1..m-1) are merged by wave 0 alone,
-        // one level after another without workgroup barriers; lane l holds position p1 + l
-        const int32_t l1 = d.rp_tinfo[2], p1 = d.rp_tinfo[3];
-        int4 ew = make_int4(0, 0, 0, 0xFFFF | (2 << 16) | (2 << 18));
-        if (threadIdx.x < 64 && p1 + (int32_t)threadIdx.x < m) ew = d.rp_tlist[p1 + threadIdx.x];
-        int bad = chunks_bad();
+    }
+    bad = __syncthreads_or(bad);
+    if (g == 0 && threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
+    for (int32_t lev = top; lev > l1; --lev) {
 #pragma unroll
         for (int k = 0; k < kTopPer; ++k) {
-            const int32_t i = threadIdx.x + k * kTopBlock;
-            const int lk = top_kind(e[k], 0), rk = top_kind(e[k], 1);
-            if (lk != 2 && rk != 2) {
-                tb[i] = box_merge(cl[k], cr[k]);
-                e[k].w |= 0xFFFF;  // done
-            } else if (lk != 2) {
-                tb[i] = cl[k];
-            } else if (rk != 2) {
-                tb[i] = cr[k];
+            if (top_depth(e[k]) == lev) {
+                const int32_t i = threadIdx.x + k * kTopBlock;
+                const int lk = top_kind(e[k], 0), rk = top_kind(e[k], 1);
+                const Box l = tb[lk == 2 ? e[k].y : i];
+                const Box r = tb[rk == 2 ? e[k].z : i];
+                tb[i] = box_merge(l, r);
             }
         }
-        bad = __syncthreads_or(bad);
-        if (threadIdx.x == 0) d.ctr[C_NO_IMPLICIT] = (uint32_t)bad;
-        for (int32_t lev = top; lev > l1; --lev) {
+        __syncthreads();
+    }
 #pragma unroll
-            for (int k = 0; k < kTopPer; ++k) {
-                if (top_depth(e[k]) == lev) {
-                    const int32_t i = threadIdx.x + k * kTopBlock;
-                    const int lk = top_kind(e[k], 0), rk = top_kind(e[k], 1);
-                    const Box l = tb[lk == 2 ? e[k].y : i];
-                    const Box r = tb[rk == 2 ? e[k].z : i];
-                    tb[i] = box_merge(l, r);
-                }
-            }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int k = 0; k < kTopPer; ++k) {
-            const int32_t i = threadIdx.x + k * kTopBlock;
-            if (i < p1) store_box(&d.inode[e[k].x], tb[i]);
-        }
-        if (threadIdx.x >= 64) return;
+    for (int k = 0; k < kTopPer; ++k) {
+        const int32_t i = threadIdx.x + k * kTopBlock;
+        if (i < p1) store_box(&d.inode[e[k].x], tb[i]);
+    }
+    if (threadIdx.x < 64) {
         // wave 0: LDS operations of one wave complete in issue order, so a level's writes are
         // seen by the next level's reads; the fences keep the compiler from reordering them
         const int32_t i = p1 + (int32_t)threadIdx.x;
@@ -1150,46 +1397,25 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         if (i < m) store_box(&d.inode[ew.x], tb[i]);
-        return;
     }
-    // larger top trees: children are node ids here (kinds 0 and 1 only), merged level by level
-    // through the node records (records this workgroup wrote: the barrier orders them). Level
-    // bounds sit in LDS; a level goes in batches of up to four entries per thread, each batch's
-    // entry loads, then its child-record loads, all in flight together.
-    int32_t* ls = reinterpret_cast<int32_t*>(tb);  // ls[t]: first position of level t
-    for (int32_t t = threadIdx.x; t <= top; t += kTopBlock) ls[t] = d.rp_tlev[t];
+    // the upper nodes above the segments: merged through the node records by the segment
+    // workgroup that finishes last (release of this workgroup's records, then a counter; the
+    // last one acquires every segment's records and resets the counter for the next refit)
+    if (d.rp_tinfo[6] == 0) return;
+    __threadfence();
     __syncthreads();
-    for (int32_t lev = top; lev >= 0; --lev) {
-        const int32_t b0 = ls[lev], b1 = lev > 0 ? ls[lev - 1] : m;
-        // batches of K rounds of kTopBlock entries, K = 4 while more than two rounds remain, then
-        // 2, then 1; indices clamped (a repeated entry is merged twice into the same box)
-        auto batch = [&](int32_t base, auto kc) {
-            constexpr int K = decltype(kc)::value;
-            int4 e[K];
-            Box l[K], r[K];
-#pragma unroll
-            for (int j = 0; j < K; ++j) e[j] = d.rp_tlist[min(base + (int32_t)threadIdx.x + j * kTopBlock, b1 - 1)];
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                l[j] = below_top_record(d, top_kind(e[j], 0), e[j].y);
-                r[j] = below_top_record(d, top_kind(e[j], 1), e[j].z);
-            }
-#pragma unroll
-            for (int j = 0; j < K; ++j)
-                if (base + (int32_t)threadIdx.x + j * kTopBlock < b1) store_box(&d.inode[e[j].x], box_merge(l[j], r[j]));
-        };
-        int32_t base = b0;
-        for (; b1 - base > 2 * kTopBlock; base += 4 * kTopBlock) batch(base, std::integral_constant<int, 4>{});
-        for (; b1 - base > kTopBlock; base += 2 * kTopBlock) batch(base, std::integral_constant<int, 2>{});
-        for (; base < b1; base += kTopBlock) batch(base, std::integral_constant<int, 1>{});
-        __syncthreads();
-    }
+    if (threadIdx.x == 0) last = atomicAdd(d.rp_tdone, 1u) == (uint32_t)(nseg - 1) ? 1u : 0u;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    refit_top_records(d, tb, d.rp_tinfo[6], d.rp_tinfo[7], false);
+    if (threadIdx.x == 0) *d.rp_tdone = 0;
 }
 
 void launch_refit(DevState& d, int32_t n, hipStream_t s) {
     if (n < 1) return;
     hipLaunchKernelGGL(k_refit, dim3(refit_chunks(n)), dim3(kRefitBlock), 0, s, n, d);
-    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(kTopBlock), 0, s, d, refit_chunks(n));
+    hipLaunchKernelGGL(k_refit_top, dim3(kTopGrid), dim3(kTopBlock), 0, s, d, refit_chunks(n));
 }
 
 // ======================================================================= prim map / misc

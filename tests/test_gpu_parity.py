@@ -117,8 +117,9 @@ def test_explicit_ops_parity(pkg, oracle):
 
 
 @pytest.mark.parametrize("name,frames", [
-    ("cloth1024", (0, 37)),   # 2,093,058 triangles: ~4K top-tree nodes, merged in LDS
-    ("layers4", (0, 9)),      # 8,372,232 triangles: ~16K top-tree nodes, merged through the records
+    ("cloth1024", (0, 37)),     # 2,093,058 triangles: 4095 top-tree nodes, merged in LDS
+    ("cloth1024", (100, 137)),  # folded: ~6.8K top-tree nodes, LDS segments + upper nodes
+    ("layers4", (0, 9)),        # 8,372,232 triangles: ~16K+ top-tree nodes, segments + upper nodes
 ])
 def test_refit_large_trees(pkg, oracle, name, frames):
     """Treelet-chunk refit and top-tree merge at full size, after a build and after a later
@@ -139,6 +140,33 @@ def test_refit_large_trees(pkg, oracle, name, frames):
         ref = oracle.refit_boxes({key: b[key] for key in ("int_lc", "int_rc", "int_mark")}, lb)
         assert np.array_equal(b["int_box"], ref), f"{name} frame {fr} internal boxes"
         del tv, lb, ref, b
+    ctx.close()
+
+
+def test_refit_deep_top_chain(pkg, oracle):
+    """3000 identical point triangles among 4096 form a chain of equal Morton codes: a top tree
+    ~2500 levels deep, deeper than the refit's level bins, merged by climbing. Boxes equal the
+    oracle's refit after a build and after a refit of moved vertices."""
+    print("[gpu] refit deep top-tree chain", flush=True)
+    rng = np.random.default_rng(3)
+    T = 4096
+    v = rng.random((3 * T, 3)).astype(np.float32)
+    v[: 3 * 3000] = np.float32(0.25)
+    f = np.arange(3 * T, dtype=np.int32).reshape(T, 3)
+    ctx = pkg.CollisionContext(T, 3 * T)
+    ctx.set_topology(f, 3 * T)
+    for k in range(2):
+        if k:
+            v = v + rng.normal(scale=1e-3, size=v.shape).astype(np.float32)
+            v[: 3 * 3000] = np.float32(0.25)
+        ctx.update_vertices(v)
+        ctx.step(pkg.BVH_BUILD if k == 0 else pkg.BVH_REFIT, pkg.FRONT_PURE_BVH_CD)
+        b = ctx.export_bvh()
+        tv = v[f[b["prim_idx"]]]
+        lb = np.concatenate([tv.min(1), tv.max(1)], 1)
+        assert np.array_equal(b["leaf_box"], lb)
+        ref = oracle.refit_boxes({key: b[key] for key in ("int_lc", "int_rc", "int_mark")}, lb)
+        assert np.array_equal(b["int_box"], ref), f"step {k} internal boxes"
     ctx.close()
 
 
