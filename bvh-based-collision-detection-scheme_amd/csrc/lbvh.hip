@@ -19,8 +19,8 @@
 //    per-tile digit counts, a device-wide exclusive scan of the digit-major count table,
 //    then a stable wave64 multisplit ranking in LDS and coalesced scatter.
 //  * Refit is level-synchronous over treelet-aligned chunks: each workgroup merges its
-//    treelets' nodes one depth level per barrier in LDS (no atomics), then one workgroup
-//    merges the small top tree above the treelets.
+//    treelets' nodes one depth level per barrier in LDS (no atomics), then the top tree above
+//    the treelets is merged in LDS by one workgroup per segment of at most 4096 nodes.
 #include <type_traits>
 
 #include "bvhcd_internal.hpp"
@@ -698,13 +698,14 @@ void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t
 //    Nodes merged at one depth are disjoint subtrees, so each slot is written by at most one
 //    of them, and a right child's slot (its own leftmost leaf) is written only by deeper nodes;
 //  * each chunk lists its internal nodes deepest first (depth below the treelet root) as
-//    (node, LDS slot | depth << 16, child slots, top-kernel slot or -1); the top tree is
-//    listed deepest first (depth below the root) with its children's kinds.
+//    8-byte entries (node, own slot | right child's slot << 10 | depth << 20), followed by
+//    its treelet roots' hand-over offsets; the top tree is listed deepest first (depth below
+//    the root) with its children's kinds, split into LDS-sized segments when it is large.
 // Per refit: one workgroup per chunk writes the leaf boxes into LDS, merges one depth level
-// per barrier in LDS and writes its node boxes at the end; then one workgroup merges the top
-// tree level by level through the node records it writes itself (workgroup-coherent).
-// Depth bins: the tree is a radix tree over at most 96 key bits (code + slot tie-break), so no
-// depth reaches kDepthCap; within a chunk the depth is below the chunk size anyway.
+// per barrier in LDS and writes its node boxes at the end; then k_refit_top merges the top
+// tree: in LDS (one workgroup per segment), the upper nodes above the segments through the
+// node records, or, for top trees deeper than the kDepthCap level bins (long chains of equal
+// Morton codes), by climbing. Within a chunk the depth is below the treelet size.
 constexpr int kTreelet = 512;
 constexpr int kChunkCap = 2 * kTreelet;
 constexpr int kRefitPer = kChunkCap / kBlock;  // entries (leaves, internal nodes) per thread
@@ -1287,11 +1288,11 @@ __device__ inline void refit_top_climb(const DevState& d, int32_t m) {
     }
 }
 
-// One workgroup. Top trees of up to kTopLds nodes stay in LDS, slot = list position: a node
-// whose children are both below the top tree (treelet roots) is merged at once, a node with
-// one such child starts its slot with that child's box and merges the other at its level.
-// Larger top trees go level by level through the node records the workgroup writes itself
-// (the barrier orders them at workgroup scope).
+// One workgroup per segment (the whole top tree when it has at most kTopLds nodes): the
+// segment stays in LDS, slot = position in the segment; a node whose children are both below
+// the top tree (treelet roots or leaves) is merged at once, a node with one such child starts
+// its slot with that child's box and merges the other at its level. The workgroup that
+// finishes last merges the upper nodes above the segments.
 
 __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chunks) {
     __shared__ Box tb[kTopLds];
