@@ -125,7 +125,7 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.rp_tlist, (size_t)T + kRefitTopLds);
     ok &= dalloc(c, d.rp_tpos, T);
     ok &= dalloc(c, d.rp_tlev, 1024);
-    ok &= dalloc(c, d.rp_tinfo, 8);
+    ok &= dalloc(c, d.rp_tinfo, 6);
     ok &= dalloc(c, d.rp_seg, kRefitTopGrid);
     ok &= dalloc(c, d.rp_tsz, T);
     ok &= dalloc(c, d.rp_tseg, T);

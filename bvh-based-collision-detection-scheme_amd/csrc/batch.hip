@@ -350,7 +350,7 @@ int bvhcd_batch_create(const bvhcd_config* cfg, int32_t S, const int64_t* tps, c
     ok &= balloc(b, d.rp_tlist, (size_t)T + kRefitTopLds);
     ok &= balloc(b, d.rp_tpos, T);
     ok &= balloc(b, d.rp_tlev, 1024);
-    ok &= balloc(b, d.rp_tinfo, 8);
+    ok &= balloc(b, d.rp_tinfo, 6);
     ok &= balloc(b, d.rp_seg, kRefitTopGrid);
     ok &= balloc(b, d.rp_tsz, T);
     ok &= balloc(b, d.rp_tseg, T);

@@ -166,8 +166,8 @@ struct DevState {
                                     //    children as positions, hand-over offsets
     int32_t* rp_tpos = nullptr;     // T-1: list position of each top node
     int32_t* rp_tlev = nullptr;     // 1024: first entry of each top-tree depth
-    int32_t* rp_tinfo = nullptr;    // 8: (top nodes, deepest level or -1, unused, unused,
-                                    //     segments or -1 (all through the records), first upper
+    int32_t* rp_tinfo = nullptr;    // 6: (top nodes, deepest level or -1, segments or -1 (all
+                                    //     through the records) or -2 (climb), first upper
                                     //     position, upper nodes, deepest upper level)
     int4* rp_seg = nullptr;         // 64: per segment (first position, nodes, deepest level,
                                     //     (last level merged per barrier + 1) | one-wave start << 16)

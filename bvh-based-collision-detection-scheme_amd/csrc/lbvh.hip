@@ -820,7 +820,7 @@ __global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
     __syncthreads();
     // hand-over offsets exist for children of segment nodes (list positions below the upper
     // nodes), unless the top tree goes through the records as a whole
-    const int32_t seg_end = d.rp_tinfo[4] > 0 ? d.rp_tinfo[5] : 0;
+    const int32_t seg_end = d.rp_tinfo[2] > 0 ? d.rp_tinfo[3] : 0;
 #pragma unroll
     for (int k = 0; k < kRefitPer; ++k) {
         if (v[k].x < 0) continue;
@@ -855,10 +855,10 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
         if (threadIdx.x == 0) {
             d.rp_tinfo[0] = m;
             d.rp_tinfo[1] = -1;
-            d.rp_tinfo[4] = -2;
-            d.rp_tinfo[5] = 0;
-            d.rp_tinfo[6] = 0;
-            d.rp_tinfo[7] = -1;
+            d.rp_tinfo[2] = -2;
+            d.rp_tinfo[3] = 0;
+            d.rp_tinfo[4] = 0;
+            d.rp_tinfo[5] = -1;
         }
         return;
     }
@@ -890,10 +890,10 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
         }
         // one segment holding the whole top tree, no upper nodes; larger trees are split by
         // k_rp_top_segments, which overwrites these
-        d.rp_tinfo[4] = 1;
-        d.rp_tinfo[5] = m;
-        d.rp_tinfo[6] = 0;
-        d.rp_tinfo[7] = -1;
+        d.rp_tinfo[2] = 1;
+        d.rp_tinfo[3] = m;
+        d.rp_tinfo[4] = 0;
+        d.rp_tinfo[5] = -1;
         d.rp_seg[0] = make_int4(0, m, m > 0 ? (int32_t)dmax : -1, (l1 + 1) | (p1 << 16));
     }
     __syncthreads();
@@ -957,10 +957,10 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_sort(DevState d) {
 //     segment nodes as positions within their segment, node positions renumbered, and dense
 //     hand-over offsets for the segment nodes' treelet-root children in list order.
 // More than kTopGrid segments, or more hand-over boxes than the buffer holds, leave the list as
-// it is and send the whole top tree through the records (rp_tinfo[4] = -1).
+// it is and send the whole top tree through the records (rp_tinfo[2] = -1).
 __global__ void __launch_bounds__(kTopBlock) k_rp_top_segments(DevState d, int32_t tcb_cap) {
     const int32_t m = d.rp_tinfo[0];
-    if (m <= kTopLds || d.rp_tinfo[4] == -2) return;
+    if (m <= kTopLds || d.rp_tinfo[2] == -2) return;
     const int32_t top = d.rp_tinfo[1];
     __shared__ int32_t ls[kDepthCap];
     __shared__ uint32_t nseg, nhand, cnt[kTopGrid + 1], fill[kTopGrid + 1], deep[kTopGrid + 1], mind[kTopGrid + 1];
@@ -999,7 +999,7 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_segments(DevState d, int32
     }
     __syncthreads();
     if (ns > kTopGrid) {
-        if (threadIdx.x == 0) d.rp_tinfo[4] = -1;
+        if (threadIdx.x == 0) d.rp_tinfo[2] = -1;
         return;
     }
     for (int32_t i = threadIdx.x; i < m; i += kTopBlock) {
@@ -1016,7 +1016,7 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_segments(DevState d, int32
     }
     __syncthreads();
     if (nhand > (uint32_t)tcb_cap) {
-        if (threadIdx.x == 0) d.rp_tinfo[4] = -1;
+        if (threadIdx.x == 0) d.rp_tinfo[2] = -1;
         return;
     }
     if (threadIdx.x == 0) {
@@ -1078,10 +1078,10 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_segments(DevState d, int32
     for (uint32_t g = threadIdx.x; g < ns; g += kTopBlock)
         d.rp_seg[g] = make_int4(sbase[g], (int32_t)cnt[g], (int32_t)deep[g], (int32_t)mind[g] | ((int32_t)cnt[g] << 16));
     if (threadIdx.x == 0) {
-        d.rp_tinfo[4] = (int32_t)ns;
-        d.rp_tinfo[5] = nsp;
-        d.rp_tinfo[6] = (int32_t)cnt[ns];
-        d.rp_tinfo[7] = (int32_t)deep[ns];
+        d.rp_tinfo[2] = (int32_t)ns;
+        d.rp_tinfo[3] = nsp;
+        d.rp_tinfo[4] = (int32_t)cnt[ns];
+        d.rp_tinfo[5] = (int32_t)deep[ns];
     }
 }
 
@@ -1224,7 +1224,7 @@ __device__ inline Box below_top_record(const DevState& d, int kind, int32_t v) {
 // loads all in flight together.
 __device__ inline void refit_top_records(const DevState& d, Box* lds, int32_t m, int32_t top, bool whole) {
     int32_t* ls = reinterpret_cast<int32_t*>(lds);  // ls[t]: first position of level t
-    const int32_t ub = d.rp_tinfo[5];               // first upper position
+    const int32_t ub = d.rp_tinfo[3];               // first upper position
     if (whole)
         for (int32_t t = threadIdx.x; t <= top; t += kTopBlock) ls[t] = d.rp_tlev[t];
     __syncthreads();
@@ -1298,7 +1298,7 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
     __shared__ Box tb[kTopLds];
     __shared__ uint32_t last;
     const int32_t g = blockIdx.x;
-    const int32_t nseg = d.rp_tinfo[4];
+    const int32_t nseg = d.rp_tinfo[2];
     // frame-wide (workgroup 0): may KEEP count shared-vertex pairs without a test (every leaf box
     // built from finite vertices); OR over the chunks, four loads per thread in flight at once
     // (indices clamped: a repeated flag does not change the OR)
@@ -1402,14 +1402,14 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
     // the upper nodes above the segments: merged through the node records by the segment
     // workgroup that finishes last (release of this workgroup's records, then a counter; the
     // last one acquires every segment's records and resets the counter for the next refit)
-    if (d.rp_tinfo[6] == 0) return;
+    if (d.rp_tinfo[4] == 0) return;
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(d.rp_tdone, 1u) == (uint32_t)(nseg - 1) ? 1u : 0u;
     __syncthreads();
     if (!last) return;
     __threadfence();
-    refit_top_records(d, tb, d.rp_tinfo[6], d.rp_tinfo[7], false);
+    refit_top_records(d, tb, d.rp_tinfo[4], d.rp_tinfo[5], false);
     if (threadIdx.x == 0) *d.rp_tdone = 0;
 }
 
