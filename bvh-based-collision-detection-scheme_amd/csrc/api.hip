@@ -59,6 +59,7 @@ struct bvhcd_ctx {
     float* boxes = nullptr;      // box mode input (6 floats per triangle), allocated on first use
     float* own_verts = nullptr;  // the context's vertex buffer (d.verts may borrow the caller's)
     bool totals_pending = false; // pair totals of the last step not summed yet (done on demand)
+    bool step_zeroed = false;    // this step's refit zeroed the step counters for its KEEP
 };
 
 namespace {
@@ -401,7 +402,8 @@ int run_restructure(bvhcd_ctx* c, int& bvh_op, bool& topo_changed, bool use_lear
         topo_changed = true;
     } else if ((!learned && nint < (I >> 3)) || nsub == 0) {
         begin(c, S_REFIT);
-        launch_refit(d, n, c->stream);
+        launch_refit(d, n, c->stream, true);
+        c->step_zeroed = true;
         end(c, S_REFIT);
         bvh_op = BVHCD_BVH_REFIT;
     } else {
@@ -438,7 +440,8 @@ int run_bvh(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
         }
         case BVHCD_BVH_REFIT:
             begin(c, S_REFIT);
-            launch_refit(c->d, (int32_t)c->T, c->stream);
+            launch_refit(c->d, (int32_t)c->T, c->stream, true);
+            c->step_zeroed = true;
             end(c, S_REFIT);
             break;
         case BVHCD_BVH_RESTRUCTURE: {
@@ -500,7 +503,7 @@ int run_front(bvhcd_ctx* c, int& front_op) {
             break;
         case BVHCD_FRONT_KEEP:
             begin(c, S_FRONT);
-            launch_keep(d, n, s);
+            launch_keep(d, n, s, c->step_zeroed);
             end(c, S_FRONT);
             break;
         case BVHCD_FRONT_RESTRUCTURE:
@@ -529,10 +532,12 @@ int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
         return fail(c, BVHCD_E_INVALID, "bad op");
     begin(c, S_TOTAL);
     bool topo = false;
+    c->step_zeroed = false;
     int st = run_bvh(c, bvh_op, topo);
     if (st) return st;
     if (topo) c->front_valid = false;
     st = run_front(c, front_op);
+    c->step_zeroed = false;
     if (st) return st;
     end(c, S_TOTAL);
     c->last_bvh = bvh_op;

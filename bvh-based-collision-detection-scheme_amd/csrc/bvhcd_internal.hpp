@@ -288,6 +288,14 @@ enum Counter64 : int {
     C64_NUM = 4
 };
 __host__ __device__ inline uint64_t* ctr64(uint32_t* ctr) { return reinterpret_cast<uint64_t*>(ctr + 16); }
+// Zeroes a step's output counters (pair-list segment counters, pair totals, overflow flags,
+// implicit pairs, node tests); lane g of one wave of 64 threads.
+__device__ inline void zero_step_counters(const DevState& d, int g) {
+    d.pcnt[g * kCntStride] = 0u;                // broad-phase segment counters
+    d.pcnt[(kPairSegs + g) * kCntStride] = 0u;  // narrow-phase segment counters
+    if (g <= C_IMPL_NP) d.ctr[g] = 0u;           // C_BP, C_NP, C_OVF, C_IMPLICIT, C_IMPL_NP
+    if (g == 8) ctr64(d.ctr)[C64_VISITS] = 0ull;
+}
 
 __device__ inline Box given_box(const float* __restrict__ b, int64_t t) {
     const float* p = b + 6 * t;

@@ -409,13 +409,7 @@ __host__ __device__ inline FrontSink null_sink() { return FrontSink{nullptr, nul
 
 // One 64-lane launch (a kernel, not memsets: the batched frame graph records it): the per-step
 // pair counters, the visit counter and the segment counters of both pair lists.
-__global__ void k_reset_step(DevState d) {
-    const int g = threadIdx.x;
-    d.pcnt[g * kCntStride] = 0u;                // broad-phase segment counters
-    d.pcnt[(kPairSegs + g) * kCntStride] = 0u;  // narrow-phase segment counters
-    if (g <= C_IMPL_NP) d.ctr[g] = 0u;  // C_BP, C_NP, C_OVF, C_IMPLICIT, C_IMPL_NP
-    if (g == 8) ctr64(d.ctr)[C64_VISITS] = 0ull;
-}
+__global__ void k_reset_step(DevState d) { zero_step_counters(d, threadIdx.x); }
 static void reset_step_counters(DevState& d, hipStream_t s) {
     static_assert(kPairSegs == 64, "one lane per segment counter");
     hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(kPairSegs), 0, s, d);
@@ -889,9 +883,9 @@ void launch_adj_partition(DevState& d, int32_t n, hipStream_t s) {
     d.adj_version = d.front_version;
 }
 
-void launch_keep(DevState& d, int32_t n, hipStream_t s) {
+void launch_keep(DevState& d, int32_t n, hipStream_t s, bool counters_zeroed) {
     if (d.adj_version != d.front_version) launch_adj_partition(d, n, s);
-    reset_step_counters(d, s);
+    if (!counters_zeroed) reset_step_counters(d, s);
     const unsigned sh = tuned_shape(TK_KEEP, n), g = shape_grid(sh);
     switch (shape_unroll(sh)) {
         case 4: hipLaunchKernelGGL(k_keep<4>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), 1); break;

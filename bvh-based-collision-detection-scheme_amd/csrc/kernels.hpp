@@ -44,7 +44,9 @@ void launch_batch_cd(DevState& d, int32_t n, const uint64_t* keys, const int32_t
 void launch_emit_runs(DevState& d, int32_t n, const int32_t* rt_of_leaf, int32_t* leaf_lo, int32_t* leaf_hi,
                       int32_t* split_lo, int32_t* split_hi, int32_t* base, int32_t* keep_par, uint32_t* keep_mark,
                       hipStream_t s);
-void launch_refit(DevState& d, int32_t n, hipStream_t s);
+// zero_step_counters: the top-tree kernel also zeroes the step's output counters, for a KEEP
+// that follows right after (launch_keep(..., true)) without a kernel of its own.
+void launch_refit(DevState& d, int32_t n, hipStream_t s, bool zero_step_counters = false);
 void launch_refit_prep(DevState& d, int32_t n, hipStream_t s);  // after every topology change
 int32_t refit_chunks(int32_t n);  // refit workgroups (treelet chunks) for n leaves
 // Top trees of up to this many nodes are merged in LDS by one workgroup; larger ones are split
@@ -62,7 +64,7 @@ void launch_pure_cd(DevState& d, int32_t n, hipStream_t s);
 // the node-id sort and bucket offsets that only the front-quality heuristic reads.
 void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order = true);  // traversal + ordering
 void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool order = true);
-void launch_keep(DevState& d, int32_t n, hipStream_t s);
+void launch_keep(DevState& d, int32_t n, hipStream_t s, bool counters_zeroed = false);
 void launch_reorder(DevState& d, int32_t n, hipStream_t s);
 // Incremental front restructure: classify the current fronts against the runs in rt_of_leaf
 // (old topology; kept entries to the other buffers, walks to d.rtasks with their count in

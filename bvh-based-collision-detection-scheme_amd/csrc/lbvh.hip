@@ -1294,10 +1294,11 @@ __device__ inline void refit_top_climb(const DevState& d, int32_t m) {
 // its slot with that child's box and merges the other at its level. The workgroup that
 // finishes last merges the upper nodes above the segments.
 
-__global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chunks) {
+__global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chunks, int zero_step) {
     __shared__ Box tb[kTopLds];
     __shared__ uint32_t last;
     const int32_t g = blockIdx.x;
+    if (zero_step && g == 0 && threadIdx.x < 64) zero_step_counters(d, threadIdx.x);
     const int32_t nseg = d.rp_tinfo[2];
     // frame-wide (workgroup 0): may KEEP count shared-vertex pairs without a test (every leaf box
     // built from finite vertices); OR over the chunks, four loads per thread in flight at once
@@ -1413,10 +1414,11 @@ __global__ void __launch_bounds__(kTopBlock) k_refit_top(DevState d, int32_t chu
     if (threadIdx.x == 0) *d.rp_tdone = 0;
 }
 
-void launch_refit(DevState& d, int32_t n, hipStream_t s) {
+void launch_refit(DevState& d, int32_t n, hipStream_t s, bool zero_step_counters) {
     if (n < 1) return;
     hipLaunchKernelGGL(k_refit, dim3(refit_chunks(n)), dim3(kRefitBlock), 0, s, n, d);
-    hipLaunchKernelGGL(k_refit_top, dim3(kTopGrid), dim3(kTopBlock), 0, s, d, refit_chunks(n));
+    hipLaunchKernelGGL(k_refit_top, dim3(kTopGrid), dim3(kTopBlock), 0, s, d, refit_chunks(n),
+                       zero_step_counters ? 1 : 0);
 }
 
 // ======================================================================= prim map / misc
