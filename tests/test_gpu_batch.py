@@ -60,6 +60,26 @@ def test_batch_matches_oracle_per_scene(pkg, oracle):
         orc.close()
 
 
+def test_batch_full_size_matches_single_scenes(pkg, oracle):
+    """The benchmark's 64-scene batch (2.6 M triangles in one tree, whose top tree is merged in
+    LDS segments) gives every scene the pairs of that scene alone in its own context, on a BUILD
+    and a REFIT frame through the captured graphs (single contexts are checked against the
+    oracle by the parity tests)."""
+    print("[gpu] full-size batch vs single scenes", flush=True)
+    scenes = [pkg.Scene(s) for s in pkg.batch_scene_specs()]
+    b, got = _run_batch(pkg, scenes, 2, use_graph=True, ops=[pkg.BVH_BUILD, pkg.BVH_REFIT])
+    b.close()
+    for k, sc in enumerate(scenes):
+        ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts)
+        ctx.set_topology(sc.faces(), sc.num_verts)
+        for f in range(2):
+            ctx.update_vertices(sc.frame(f))
+            ctx.step(pkg.BVH_BUILD if f == 0 else pkg.BVH_REFIT, pkg.FRONT_PURE_BVH_CD)
+            np.testing.assert_array_equal(oracle.pair_set(got[f][k]), oracle.pair_set(ctx.pairs(True)),
+                                          err_msg=f"scene {k} frame {f}")
+        ctx.close()
+
+
 def test_batch_tiny_and_uneven_scenes(pkg, oracle):
     print("[gpu] batch tiny scenes", flush=True)
     rng = np.random.default_rng(5)
