@@ -57,7 +57,12 @@ struct bvhcd_ctx {
     std::vector<void*> allocs;
     LearnedState* learned = nullptr;
     float* boxes = nullptr;      // box mode input (6 floats per triangle), allocated on first use
-    float* own_verts = nullptr;  // the context's vertex buffer (d.verts may borrow the caller's)
+    float* own_verts = nullptr;  // the context's vertex buffers (d.verts may borrow the caller's);
+    float* own_verts2 = nullptr; // host uploads alternate between the two (vbuf: the current one)
+    int vbuf = 0;
+    hipStream_t copy_stream = nullptr;  // host uploads, overlapping the steps still running
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    hipEvent_t consumed[2] = {nullptr, nullptr};  // the last step that read each buffer is done
     bool totals_pending = false; // pair totals of the last step not summed yet (done on demand)
     bool step_zeroed = false;    // this step's refit zeroed the step counters for its KEEP
 };
@@ -99,6 +104,7 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.faces, 3 * T);
     ok &= dalloc(c, d.verts, 3 * V);
     c->own_verts = d.verts;
+    ok &= dalloc(c, c->own_verts2, 3 * V);
     ok &= dalloc(c, d.codes, T);
     ok &= dalloc(c, d.keys_tmp, T);
     ok &= dalloc(c, d.keys_tmp2, T);
@@ -539,6 +545,8 @@ int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     st = run_front(c, front_op);
     c->step_zeroed = false;
     if (st) return st;
+    if (c->d.verts == c->own_verts || c->d.verts == c->own_verts2)  // this step has read that buffer
+        HIPCHK(c, hipEventRecord(c->consumed[c->d.verts == c->own_verts2 ? 1 : 0], c->stream));
     end(c, S_TOTAL);
     c->last_bvh = bvh_op;
     c->last_front = front_op;
@@ -591,6 +599,19 @@ void timing_free(bvhcd_ctx* c) {
 
 extern "C" {
 
+namespace {
+void release_upload(bvhcd_ctx* c) {
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    for (int b = 0; b < 2; ++b) {
+        if (c->copied[b]) (void)hipEventDestroy(c->copied[b]);
+        if (c->consumed[b]) (void)hipEventDestroy(c->consumed[b]);
+        c->copied[b] = c->consumed[b] = nullptr;
+    }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    c->copy_stream = nullptr;
+}
+}  // namespace
+
 int bvhcd_create(const bvhcd_config* cfg, bvhcd_ctx** out) {
     if (!cfg || !out || cfg->max_tris < 1 || cfg->max_verts < 1 || cfg->max_tris > (1ll << 30) ||
         cfg->max_verts > (1ll << 30))
@@ -619,11 +640,18 @@ int bvhcd_create(const bvhcd_config* cfg, bvhcd_ctx** out) {
         c->own_stream = true;
     }
     int st = allocate(c);
-    if (st) {
+    bool ok = st == 0 && hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) == hipSuccess;
+    for (int b = 0; b < 2 && ok; ++b) {
+        ok &= hipEventCreateWithFlags(&c->copied[b], hipEventDisableTiming) == hipSuccess;
+        ok &= hipEventCreateWithFlags(&c->consumed[b], hipEventDisableTiming) == hipSuccess;
+        ok &= ok && hipEventRecord(c->consumed[b], c->stream) == hipSuccess;
+    }
+    if (!ok) {
+        release_upload(c);
         free_all(c);
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
-        return st;
+        return st ? st : BVHCD_E_DEVICE;
     }
     *out = c;
     return BVHCD_OK;
@@ -632,6 +660,7 @@ int bvhcd_create(const bvhcd_config* cfg, bvhcd_ctx** out) {
 int bvhcd_destroy(bvhcd_ctx* c) {
     if (!c) return BVHCD_E_INVALID;
     (void)hipStreamSynchronize(c->stream);
+    release_upload(c);
     learned_destroy(c->learned);
     timing_free(c);
     free_all(c);
@@ -669,10 +698,22 @@ int bvhcd_update_vertices(bvhcd_ctx* c, const float* xyz, int64_t V, int where) 
         c->d.verts = const_cast<float*>(xyz);
         return BVHCD_OK;
     }
-    c->d.verts = c->own_verts;
-    HIPCHK(c, hipMemcpyAsync(c->d.verts, xyz, sizeof(float) * 3 * V,
-                             where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
-    if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (where == BVHCD_MEM_DEVICE) {  // stream-ordered after the steps that read the buffer
+        c->d.verts = c->vbuf ? c->own_verts2 : c->own_verts;
+        HIPCHK(c, hipMemcpyAsync(c->d.verts, xyz, sizeof(float) * 3 * V, hipMemcpyDeviceToDevice, c->stream));
+        return BVHCD_OK;
+    }
+    // Host memory: the upload goes on the copy stream into the buffer the latest step did not
+    // read (once the step before it, which did, is done), so it overlaps the steps still
+    // running; the call returns when the copy is complete (the caller may reuse xyz).
+    const int b = c->vbuf ^ 1;
+    float* dst = b ? c->own_verts2 : c->own_verts;
+    HIPCHK(c, hipStreamWaitEvent(c->copy_stream, c->consumed[b], 0));
+    HIPCHK(c, hipMemcpyAsync(dst, xyz, sizeof(float) * 3 * V, hipMemcpyHostToDevice, c->copy_stream));
+    HIPCHK(c, hipEventRecord(c->copied[b], c->copy_stream));
+    HIPCHK(c, hipEventSynchronize(c->copied[b]));
+    c->vbuf = b;
+    c->d.verts = dst;
     return BVHCD_OK;
 }
 

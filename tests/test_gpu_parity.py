@@ -143,6 +143,33 @@ def test_refit_large_trees(pkg, oracle, name, frames):
     ctx.close()
 
 
+def test_host_upload_overlap(pkg, oracle):
+    """Host vertex uploads go through two device buffers on a copy stream and overlap the steps
+    still running: 24 frames stepped back to back without a synchronisation log the same
+    per-frame counts (the frame log's algorithmic bytes, which carry the pair counts and node
+    tests) as the same frames with a synchronisation after every step, and the last frame's
+    pairs equal brute force."""
+    print("[gpu] host upload overlap", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=96)
+    frames = [sc.frame(100 + k) for k in range(24)]
+    logs = []
+    for sync in (False, True):
+        ctx = _ctx(pkg, sc)
+        ctx.set_timing(len(frames))
+        for k, v in enumerate(frames):
+            ctx.update_vertices(v)
+            ctx.step(pkg.BVH_BUILD if k == 0 else pkg.BVH_REFIT, pkg.FRONT_GENERATE if k == 0 else pkg.FRONT_KEEP)
+            if sync:
+                ctx.stats()
+        ms, by, ops = ctx.timing_frames()
+        logs.append(by)
+        if not sync:
+            ref = oracle.pair_set(oracle.bruteforce_pairs(sc.faces(), frames[-1], True))
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), ref)
+        ctx.close()
+    np.testing.assert_array_equal(logs[0], logs[1])
+
+
 def test_refit_deep_top_chain(pkg, oracle):
     """3000 identical point triangles among 4096 form a chain of equal Morton codes: a top tree
     ~2500 levels deep, deeper than the refit's level bins, merged by climbing. Boxes equal the
