@@ -324,8 +324,10 @@ def main():
         roof["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT)
 
     host_fps = None
-    if args.host_upload:  # same frames, vertices handed over from host memory every frame
-        host_frames = [frames_dev[f].cpu().numpy() for f in range(warmup + steps)]
+    if args.host_upload:  # same frames, vertices handed over from (pinned) host memory every frame
+        pinned = torch.empty((warmup + steps,) + tuple(frames_dev.shape[1:]), dtype=torch.float32, pin_memory=True)
+        pinned.copy_(frames_dev[: warmup + steps].cpu())
+        host_frames = [pinned[f].numpy() for f in range(warmup + steps)]
         ctx = make_ctx(pkg, sc, faces, pkg.SCHED_LEARNED if learned is not None else pkg.SCHED_PLAIN, weights)
         for f in range(warmup):
             ctx.update_vertices(host_frames[f])
