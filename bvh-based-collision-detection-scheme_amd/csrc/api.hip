@@ -1046,7 +1046,7 @@ int bvhcd_autotune(bvhcd_ctx* c) {
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     std::vector<unsigned> cand[2];
-    for (unsigned g : {1024u, 2048u, 4096u, 8192u})
+    for (unsigned g : {1024u, 2048u, 4096u, 8192u, 16384u, 32768u})
         for (unsigned u : {1u, 2u, 4u}) cand[TK_KEEP].push_back(make_shape(g, u));
     for (unsigned g : {1024u, 2048u, 4096u, 8192u, 16384u}) cand[TK_NARROW].push_back(g);
     for (int k : {(int)TK_KEEP, (int)TK_NARROW}) {
