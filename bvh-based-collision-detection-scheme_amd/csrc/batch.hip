@@ -458,6 +458,9 @@ int bvhcd_batch_update_vertices(bvhcd_batch* b, const float* xyz, int where) {
     const hipMemcpyKind kind = where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (hipMemcpyAsync(b->d.verts, xyz, sizeof(float) * 3 * b->V, kind, b->stream) != hipSuccess)
         return bfail(b, BVHCD_E_DEVICE, "vertex copy failed");
+    // host memory (pinned or not) may be reused by the caller as soon as this returns
+    if (where != BVHCD_MEM_DEVICE && hipStreamSynchronize(b->stream) != hipSuccess)
+        return bfail(b, BVHCD_E_DEVICE, "vertex copy failed");
     return BVHCD_OK;
 }
 

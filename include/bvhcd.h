@@ -129,7 +129,9 @@ const char* bvhcd_last_error(const bvhcd_ctx* ctx);
 
 /* faces: T x int32[3] vertex ids. Topology is fixed until the next call (it resets BVH and front). */
 int  bvhcd_set_topology(bvhcd_ctx* ctx, const int32_t* faces, int64_t num_tris, int64_t num_verts, int where);
-/* xyz: V x float[3]. Copied into the context (host: H2D, device: D2D) on the context stream. */
+/* xyz: V x float[3]. Device memory: copied D2D on the context stream. Host memory: uploaded on
+   the context's copy stream into one of two device buffers, overlapping the step still running
+   on the GPU; returns when the copy is complete, so xyz may be reused at once. */
 int  bvhcd_update_vertices(bvhcd_ctx* ctx, const float* xyz, int64_t num_verts, int where);
 /* Box mode, the simulator-facing input of the reference (ARCSimSceneData, world/Scene.h:39-46;
    LBvhFixedDeformable::maintain, LBvhFixedDeformable.cu:89-112; calcMaxBVARCSim / calcMCsARCSim,
@@ -220,7 +222,7 @@ int  bvhcd_batch_create(const bvhcd_config* cfg, int32_t num_scenes, const int64
 int  bvhcd_batch_destroy(bvhcd_batch* b);
 /* faces: concatenated per-scene faces with scene-local vertex ids. */
 int  bvhcd_batch_set_topology(bvhcd_batch* b, const int32_t* faces, int where);
-/* xyz: concatenated per-scene vertices. */
+/* xyz: concatenated per-scene vertices; host memory may be reused when the call returns. */
 int  bvhcd_batch_update_vertices(bvhcd_batch* b, const float* xyz, int where);
 /* One frame for every scene: bvh_op BUILD or REFIT, then the persistent self-CD traversal,
    the narrow phase and the per-scene grouping of the hits. use_graph = 1: the frame is captured
