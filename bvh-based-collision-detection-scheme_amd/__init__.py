@@ -68,6 +68,7 @@ class SceneParams(C.Structure):
         ("seed", C.c_uint64), ("n", C.c_int32), ("frames", C.c_int32), ("layers", C.c_int32),
         ("subdiv", C.c_int32), ("amplitude", C.c_float), ("compress", C.c_float), ("sigma", C.c_float),
         ("radius", C.c_float), ("spacing", C.c_float), ("speed", C.c_float),
+        ("fold", C.c_float), ("fold_radius", C.c_float),
     ]
 
 
@@ -159,6 +160,8 @@ class SceneSpec:
     radius: float = 1.0
     spacing: float = 0.0
     speed: float = 0.0
+    fold: float = 0.0
+    fold_radius: float = 0.0
 
 
 # The five workloads of the benchmark suite (seeded synthetic deforming meshes).
@@ -166,10 +169,13 @@ PRESETS = {
     # two icospheres (20,480 tris each), centres 2.5 apart, closing 0.01/frame, N(0,1e-3) jitter
     "spheres": SceneSpec(SCENE_TWO_SPHERES, seed=1, frames=200, subdiv=5, radius=1.0, sigma=1e-3,
                          spacing=2.5, speed=0.01),
-    # 256x256 folding cloth (130,050 tris), 300 frames
-    "cloth256": SceneSpec(SCENE_CLOTH, seed=2, n=256, frames=300, amplitude=0.15, compress=0.5),
-    # 1024x1024 folding cloth (2,093,058 tris), 400 frames
-    "cloth1024": SceneSpec(SCENE_CLOTH, seed=4, n=1024, frames=400, amplitude=0.15, compress=0.5),
+    # 256x256 folding cloth (130,050 tris), 300 frames; the seeded fold reaches 3.7 rad (212
+    # degrees) at the last frame, so the flap passes through the rest of the sheet
+    "cloth256": SceneSpec(SCENE_CLOTH, seed=2, n=256, frames=300, amplitude=0.15, compress=0.5, fold=3.7,
+                          fold_radius=0.05),
+    # 1024x1024 folding cloth (2,093,058 tris), 400 frames, same recipe, another seed
+    "cloth1024": SceneSpec(SCENE_CLOTH, seed=4, n=1024, frames=400, amplitude=0.15, compress=0.5, fold=3.7,
+                           fold_radius=0.05),
     # 4 stacked 1024x1024 cloths 1e-3 apart (8,372,232 tris), 200 frames
     "layers4": SceneSpec(SCENE_LAYERS, seed=5, n=1024, frames=200, layers=4, amplitude=0.05,
                          compress=0.5, spacing=1e-3),
@@ -177,8 +183,9 @@ PRESETS = {
 
 
 def batch_scene_specs(num=64, seed=3, frames=150, t_min=2e3, t_max=2e5):
-    """The batched workload: `num` folding sheets with triangle counts log-uniform in
-    [t_min, t_max] (an n x n sheet has 2 (n-1)^2 triangles; n = ceil(sqrt(T / 2)) + 1), each
+    """The batched workload: `num` folding sheets (the 256x256 cloth's recipe, fold included) with
+    triangle counts log-uniform in [t_min, t_max] (an n x n sheet has 2 (n-1)^2 triangles;
+    n = ceil(sqrt(T / 2)) + 1), each
     with a rigid icosphere (subdivision 3, radius 0.08) on a seeded straight path through it."""
     rng = np.random.default_rng(seed)
     specs = []
@@ -186,7 +193,7 @@ def batch_scene_specs(num=64, seed=3, frames=150, t_min=2e3, t_max=2e5):
         t = float(np.exp(rng.uniform(np.log(t_min), np.log(t_max))))
         n = int(np.ceil(np.sqrt(t / 2.0))) + 1
         specs.append(SceneSpec(SCENE_SHEET_SPHERE, seed=seed * 1000 + k, n=n, frames=frames, subdiv=3,
-                               radius=0.08, amplitude=0.15, compress=0.5))
+                               radius=0.08, amplitude=0.15, compress=0.5, fold=3.7, fold_radius=0.05))
     return specs
 
 
@@ -196,7 +203,8 @@ class Scene:
     def __init__(self, spec: SceneSpec):
         L = lib()
         p = SceneParams(spec.seed, spec.n, spec.frames, spec.layers, spec.subdiv, spec.amplitude,
-                        spec.compress, spec.sigma, spec.radius, spec.spacing, spec.speed)
+                        spec.compress, spec.sigma, spec.radius, spec.spacing, spec.speed, spec.fold,
+                        spec.fold_radius)
         h = C.c_void_p()
         st = L.bvhcd_scene_create(spec.kind, C.byref(p), C.byref(h))
         if st != OK:

@@ -11,6 +11,12 @@
 //    z = sum_k A(s) sin(kx_k x0 + ky_k y + phi_k) with kx, ky ~ U[2pi, 16pi], phi ~ U[0, 2pi),
 //    A(s) = amplitude * s, s = t/(F-1); then x = x0 (1 - compress s). Quad (i, j) gives the
 //    triangles (v00, v10, v11) and (v00, v11, v01).
+//  * fold (cloth kinds with fold > 0): the part of the sheet on one side of a seeded line is
+//    bent about that line over a cylinder of radius fold_radius, by the angle
+//    theta(s) = fold * (1 - (1 - s)^4). The line passes through (cx (1 - compress s), cy) with
+//    cx, cy ~ U[0.35, 0.65] and direction angle alpha ~ U[0, pi), drawn after the sinusoids.
+//    The height z is an offset along the bent sheet's normal. Past theta = pi the flap
+//    descends through the unbent part, so distant parts of the sheet interpenetrate.
 //  * two spheres: icosahedron + `subdiv` midpoint subdivisions projected to `radius`; centres
 //    at -/+ (spacing/2) on x, each moving `speed/2` per frame towards the other; per frame,
 //    per vertex, per axis jitter N(0, sigma) added to the rest shape (not accumulated).
@@ -117,7 +123,9 @@ struct Cloth {
     int frames = 1;
     double amplitude = 0, compress = 0, zoff = 0;
     double kx[4], ky[4], phi[4];
-    void init(uint64_t seed, int n_, int frames_, double amp, double comp, double z0) {
+    double fold = 0, fold_r = 0, fcx = 0, fcy = 0, fdx = 1, fdy = 0;
+    void init(uint64_t seed, int n_, int frames_, double amp, double comp, double z0,
+              double fold_ = 0.0, double fold_radius = 0.0) {
         n = n_; frames = frames_; amplitude = amp; compress = comp; zoff = z0;
         Rng r(seed);
         for (int k = 0; k < 4; ++k) {
@@ -125,6 +133,13 @@ struct Cloth {
             ky[k] = r.uniform(2.0 * M_PI, 16.0 * M_PI);
             phi[k] = r.uniform(0.0, 2.0 * M_PI);
         }
+        fold = fold_ > 0.0 ? fold_ : 0.0;
+        fold_r = fold_radius > 0.0 ? fold_radius : 0.05;
+        fcx = r.uniform(0.35, 0.65);
+        fcy = r.uniform(0.35, 0.65);
+        const double alpha = r.uniform(0.0, M_PI);
+        fdx = std::cos(alpha);
+        fdy = std::sin(alpha);
     }
     int64_t nv() const { return (int64_t)n * n; }
     int64_t nt() const { return 2 * (int64_t)(n - 1) * (n - 1); }
@@ -140,9 +155,35 @@ struct Cloth {
             }
     }
     double s_of(int frame) const { return frames > 1 ? (double)frame / (double)(frames - 1) : 0.0; }
+    double theta_of(double s) const {
+        const double u = 1.0 - s;
+        return fold * (1.0 - u * u * u * u);
+    }
+    // Bends the point (px, py, h) (h = offset along the unbent normal +z) about the fold line.
+    void bend(double theta, double cx, double& px, double& py, double& h) const {
+        const double ox = px - cx, oy = py - fcy;
+        const double u = ox * fdx + oy * fdy;     // along the line
+        const double sg = -ox * fdy + oy * fdx;   // signed distance (normal (-dy, dx))
+        if (sg <= 0.0 || theta <= 0.0) return;
+        const double R = fold_r, arc = R * theta;
+        double a, b;
+        if (sg <= arc) {
+            const double ph = sg / R, sp = std::sin(ph), cp = std::cos(ph);
+            a = (R - h) * sp;
+            b = R * (1.0 - cp) + h * cp;
+        } else {
+            const double st = std::sin(theta), ct = std::cos(theta), d = sg - arc;
+            a = R * st + d * ct - h * st;
+            b = R * (1.0 - ct) + d * st + h * ct;
+        }
+        px = cx + u * fdx - a * fdy;
+        py = fcy + u * fdy + a * fdx;
+        h = b;
+    }
     // rows [j0, j1)
     void frame_rows(int frame, float* xyz, int j0, int j1) const {
         const double s = s_of(frame), A = amplitude * s, cx = 1.0 - compress * s;
+        const double theta = theta_of(s), lx = fcx * cx;
         const double inv = n > 1 ? 1.0 / (double)(n - 1) : 0.0;
         for (int j = j0; j < j1; ++j) {
             const double y = (double)j * inv;
@@ -150,9 +191,11 @@ struct Cloth {
                 const double x0 = (double)i * inv;
                 double z = 0.0;
                 for (int k = 0; k < 4; ++k) z += A * std::sin(kx[k] * x0 + ky[k] * y + phi[k]);
+                double px = x0 * cx, py = y;
+                if (fold > 0.0) bend(theta, lx, px, py, z);
                 float* p = xyz + 3 * ((int64_t)j * n + i);
-                p[0] = (float)(x0 * cx);
-                p[1] = (float)y;
+                p[0] = (float)px;
+                p[1] = (float)py;
                 p[2] = (float)(z + zoff);
             }
         }
@@ -210,7 +253,8 @@ int bvhcd_scene_create(int kind, const bvhcd_scene_params* p, bvhcd_scene** out)
             for (int k = 0; k < layers; ++k) {
                 uint64_t seed = kind == BVHCD_SCENE_CLOTH ? p->seed : mix_seed(p->seed, (uint64_t)k);
                 s->cloths[k].init(seed, p->n, s->p.frames, p->amplitude, p->compress,
-                                  kind == BVHCD_SCENE_CLOTH ? 0.0 : (double)k * p->spacing);
+                                  kind == BVHCD_SCENE_CLOTH ? 0.0 : (double)k * p->spacing,
+                                  p->fold, p->fold_radius);
                 s->nv += s->cloths[k].nv();
                 s->nt += s->cloths[k].nt();
             }
@@ -219,7 +263,8 @@ int bvhcd_scene_create(int kind, const bvhcd_scene_params* p, bvhcd_scene** out)
         case BVHCD_SCENE_SHEET_SPHERE: {
             if (p->n < 2 || p->subdiv < 0 || p->subdiv > 8) { delete s; return BVHCD_E_INVALID; }
             s->cloths.resize(1);
-            s->cloths[0].init(p->seed, p->n, s->p.frames, p->amplitude, p->compress, 0.0);
+            s->cloths[0].init(p->seed, p->n, s->p.frames, p->amplitude, p->compress, 0.0, p->fold,
+                              p->fold_radius);
             s->sphere = icosphere(p->subdiv, p->radius);
             Rng r(mix_seed(p->seed, 0x5A5A5A5Aull));
             s->path0[0] = r.uniform(0.2, 0.8); s->path0[1] = r.uniform(0.2, 0.8); s->path0[2] = 0.5;

@@ -274,6 +274,8 @@ typedef struct bvhcd_scene_params {
     float    radius;     /* icosphere radius */
     float    spacing;    /* layer spacing (LAYERS) or sphere centre gap (TWO_SPHERES) */
     float    speed;      /* approach speed per frame (TWO_SPHERES) */
+    float    fold;       /* cloth kinds: fold angle at the last frame in radians (0 = no fold) */
+    float    fold_radius;/* cloth kinds: radius of the fold's bend (<= 0: 0.05) */
 } bvhcd_scene_params;
 
 typedef struct bvhcd_scene bvhcd_scene;

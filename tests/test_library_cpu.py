@@ -54,7 +54,58 @@ def test_scene_faces_valid_and_deterministic():
     cl = pkg.Scene.preset("cloth256", n=32)
     f0 = cl.frame(0)
     assert np.all(f0[:, 2] == 0)                            # flat at frame 0
-    assert np.allclose(cl.frame(299)[:, 0].max(), 0.5, atol=1e-6)  # x compressed by 0.5
+    flat = pkg.Scene.preset("cloth256", n=32, fold=0.0)
+    assert np.allclose(flat.frame(299)[:, 0].max(), 0.5, atol=1e-6)  # x compressed by 0.5
+
+
+def test_cloth_fold_is_a_bend():
+    """With the sinusoids off, the fold only bends the compressed flat sheet about a line:
+    every mesh edge keeps its length (the bend is an isometry of the plane), the vertices on
+    one side of the line do not move, and past 180 degrees the flap has crossed below the
+    unbent side's plane."""
+    pkg = load_package()
+    n, F = 48, 100
+    sc = pkg.Scene.preset("cloth256", n=n, frames=F, amplitude=0.0)
+    f = sc.faces()
+    edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+    for fr in (0, 30, 60, 99):
+        v = sc.frame(fr).astype(np.float64)
+        s = fr / (F - 1)
+        x0 = np.tile(np.arange(n) / (n - 1), n) * (1 - 0.5 * s)
+        y0 = np.repeat(np.arange(n) / (n - 1), n)
+        flat = np.stack([x0, y0, np.zeros_like(x0)], 1)
+        le = np.linalg.norm(v[edges[:, 0]] - v[edges[:, 1]], axis=1)
+        lf = np.linalg.norm(flat[edges[:, 0]] - flat[edges[:, 1]], axis=1)
+        # a bend never stretches an edge; only edges across the curved strip (radius 0.05,
+        # about 2 edge lengths wide here) become chords of the arc, shorter by < 1 %
+        assert np.all(le <= lf + 2e-6)
+        assert np.all(le >= 0.99 * lf)
+        exact = np.abs(le - lf) <= 2e-6
+        assert exact.mean() > 0.8
+        still = np.all(v == flat.astype(np.float32), axis=1)
+        if fr > 0:
+            assert 0.2 * n * n < still.sum() < 0.9 * n * n  # one side stays put, the flap moves
+    theta = 3.7 * (1 - (1 - 99 / (F - 1)) ** 4)
+    assert theta > np.pi and v[:, 2].min() < -0.05      # the flap has passed through z = 0
+
+
+def test_folding_cloth_has_self_contact(oracle):
+    """The folding cloths bring distant parts of the sheet into contact: the narrow phase finds
+    pairs of triangles that share no vertex on the later frames (frames 100..299 of the 256^2
+    cloth, 150..399 of the 1024^2 cloth), which a height field never has."""
+    pkg = load_package()
+    for name, frames in (("cloth256", range(100, 300, 33)), ("cloth1024", (150, 399))):
+        sc = pkg.Scene.preset(name)
+        f = sc.faces()
+        ctx = oracle.OracleContext(f, sc.num_verts)
+        for fr in frames:
+            ctx.update_vertices(sc.frame(fr))
+            ctx.step(0, 0)  # BUILD + pure traversal
+            p = ctx.pairs(True)
+            fa, fb = f[p[:, 0]], f[p[:, 1]]
+            shared = (fa[:, :, None] == fb[:, None, :]).any(axis=(1, 2))
+            assert (~shared).sum() > 1000, (name, fr, int((~shared).sum()))
+        ctx.close()
 
 
 def test_scene_frames_match_golden_hashes():
