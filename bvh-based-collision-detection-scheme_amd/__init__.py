@@ -494,12 +494,18 @@ class Batch:
     def set_topology(self, faces):
         """faces: concatenated per-scene faces with scene-local vertex ids (host)."""
         f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+        if f.shape[0] != self.toff[-1]:
+            raise ValueError(f"batch_set_topology: {f.shape[0]} faces, the batch has {int(self.toff[-1])}")
         self._chk(lib().bvhcd_batch_set_topology(self._h, _ptr(f), MEM_HOST), "batch_set_topology")
 
     def update_vertices(self, xyz, where=MEM_HOST):
         if where == MEM_HOST:
             v = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+            if v.shape[0] != self.voff[-1]:
+                raise ValueError(f"batch_update_vertices: {v.shape[0]} vertices, the batch has {int(self.voff[-1])}")
             self._chk(lib().bvhcd_batch_update_vertices(self._h, _ptr(v), MEM_HOST), "batch_update_vertices")
+        elif where != MEM_DEVICE:
+            raise ValueError("batch_update_vertices: where must be MEM_HOST or MEM_DEVICE")
         else:
             self._chk(lib().bvhcd_batch_update_vertices(self._h, C.c_void_p(int(xyz)), MEM_DEVICE),
                       "batch_update_vertices")

@@ -241,6 +241,14 @@ void mark(bvhcd_ctx* c, int stage, bool end) {
 void begin(bvhcd_ctx* c, int stage) { mark(c, stage, false); }
 void end(bvhcd_ctx* c, int stage) { mark(c, stage, true); }
 
+// Sets *bad when a face vertex id lies outside [0, V).
+__global__ void k_check_faces(int64_t n, const int32_t* __restrict__ f, int32_t V, uint32_t* bad) {
+    bool any = false;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        any |= f[i] < 0 || f[i] >= V;
+    if (any) atomicOr(bad, 1u);
+}
+
 __global__ void k_log_frame(DevState d, int cur_int, int cur_ext, uint64_t* out) {
     if (threadIdx.x != 0) return;
     out[0] = ctr64(d.ctr)[C64_VISITS];
@@ -539,14 +547,19 @@ int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     begin(c, S_TOTAL);
     bool topo = false;
     c->step_zeroed = false;
+    // Once kernels that read an own vertex buffer are enqueued, the next host upload into that
+    // buffer must wait for them, also when a later part of the step fails.
+    auto mark_consumed = [c]() {
+        if (c->d.verts == c->own_verts || c->d.verts == c->own_verts2)
+            (void)hipEventRecord(c->consumed[c->d.verts == c->own_verts2 ? 1 : 0], c->stream);
+    };
     int st = run_bvh(c, bvh_op, topo);
-    if (st) return st;
+    if (st) { mark_consumed(); return st; }
     if (topo) c->front_valid = false;
     st = run_front(c, front_op);
     c->step_zeroed = false;
+    mark_consumed();
     if (st) return st;
-    if (c->d.verts == c->own_verts || c->d.verts == c->own_verts2)  // this step has read that buffer
-        HIPCHK(c, hipEventRecord(c->consumed[c->d.verts == c->own_verts2 ? 1 : 0], c->stream));
     end(c, S_TOTAL);
     c->last_bvh = bvh_op;
     c->last_front = front_op;
@@ -674,9 +687,26 @@ const char* bvhcd_last_error(const bvhcd_ctx* c) { return c ? c->err.c_str() : "
 int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V, int where) {
     if (!c) return BVHCD_E_INVALID;
     if (!faces || T < 1 || V < 1 || T > c->d.maxT || V > c->d.maxV) return fail(c, BVHCD_E_INVALID, "bad topology size");
+    if (where != BVHCD_MEM_HOST && where != BVHCD_MEM_DEVICE) return fail(c, BVHCD_E_INVALID, "bad memory kind");
+    c->T = 0;  // no steps until a valid topology is in place
+    // Every vertex id must lie in [0, V): the kernels gather vertices through them unchecked.
+    if (where == BVHCD_MEM_HOST) {
+        for (int64_t k = 0; k < 3 * T; ++k)
+            if (faces[k] < 0 || faces[k] >= V) return fail(c, BVHCD_E_INVALID, "face vertex id out of range");
+    }
     HIPCHK(c, hipMemcpyAsync(c->d.faces, faces, sizeof(int32_t) * 3 * T,
                              where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
-    if (where != BVHCD_MEM_DEVICE) HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (where == BVHCD_MEM_DEVICE) {
+        uint32_t bad = 0;
+        HIPCHK(c, hipMemsetAsync(c->d.ctr + C_BADFACE, 0, sizeof(uint32_t), c->stream));
+        hipLaunchKernelGGL(k_check_faces, dim3(std::min<unsigned>(grid_for(3 * T), 4096)), dim3(kBlock), 0, c->stream,
+                           3 * T, (const int32_t*)c->d.faces, (int32_t)V, c->d.ctr + C_BADFACE);
+        HIPCHK(c, hipMemcpyAsync(&bad, c->d.ctr + C_BADFACE, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (bad) return fail(c, BVHCD_E_INVALID, "face vertex id out of range");
+    } else {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     c->T = T;
     c->V = V;
     c->built = c->front_valid = c->restr_pending = c->learned_frame = c->repair_pending = false;

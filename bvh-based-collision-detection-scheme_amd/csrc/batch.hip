@@ -99,14 +99,22 @@ __global__ void k_tri_scene(int64_t T, int32_t S, const int64_t* __restrict__ to
     }
 }
 
+// Scene-local vertex ids to global ones. An id outside its scene's vertex range sets *bad and
+// is replaced by the scene's first vertex, so no later kernel gathers out of bounds.
 __global__ void k_globalize(int64_t T, const int32_t* __restrict__ tri_scene, const int64_t* __restrict__ voff,
-                            int32_t* __restrict__ faces) {
+                            int32_t* __restrict__ faces, uint32_t* bad) {
+    bool any = false;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t o = (int32_t)voff[tri_scene[t]];
-        faces[3 * t] += o;
-        faces[3 * t + 1] += o;
-        faces[3 * t + 2] += o;
+        const int32_t s = tri_scene[t];
+        const int32_t o = (int32_t)voff[s], nv = (int32_t)(voff[s + 1] - voff[s]);
+        for (int k = 0; k < 3; ++k) {
+            const int32_t v = faces[3 * t + k];
+            const bool ok = v >= 0 && v < nv;
+            any |= !ok;
+            faces[3 * t + k] = (ok ? v : 0) + o;
+        }
     }
+    if (any) atomicOr(bad, 1u);
 }
 
 __global__ void k_sbox_init(int32_t S, uint32_t* __restrict__ box_u) {
@@ -434,6 +442,7 @@ int bvhcd_batch_destroy(bvhcd_batch* b) {
 
 int bvhcd_batch_set_topology(bvhcd_batch* b, const int32_t* faces, int where) {
     if (!b || !faces) return BVHCD_E_INVALID;
+    if (where != BVHCD_MEM_HOST && where != BVHCD_MEM_DEVICE) return bfail(b, BVHCD_E_INVALID, "bad memory kind");
     if (where == BVHCD_MEM_HOST) {  // vertex ids must lie inside their scene
         for (int32_t s = 0; s < b->S; ++s) {
             const int64_t nv = b->voff[s + 1] - b->voff[s];
@@ -444,9 +453,16 @@ int bvhcd_batch_set_topology(bvhcd_batch* b, const int32_t* faces, int where) {
     const hipMemcpyKind kind = where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (hipMemcpyAsync(b->d.faces, faces, sizeof(int32_t) * 3 * b->T, kind, b->stream) != hipSuccess)
         return bfail(b, BVHCD_E_DEVICE, "faces copy failed");
+    uint32_t bad = 0;
+    b->topo = false;
+    if (hipMemsetAsync(b->d.ctr + C_BADFACE, 0, sizeof(uint32_t), b->stream) != hipSuccess)
+        return bfail(b, BVHCD_E_DEVICE, "set_topology failed");
     hipLaunchKernelGGL(k_globalize, dim3(std::min<unsigned>(grid_for(b->T), 4096)), dim3(kBlock), 0, b->stream, b->T,
-                       b->tri_scene, b->voff_dev, b->d.faces);
-    if (hipStreamSynchronize(b->stream) != hipSuccess) return bfail(b, BVHCD_E_DEVICE, "set_topology failed");
+                       b->tri_scene, b->voff_dev, b->d.faces, b->d.ctr + C_BADFACE);
+    if (hipMemcpyAsync(&bad, b->d.ctr + C_BADFACE, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return bfail(b, BVHCD_E_DEVICE, "set_topology failed");
+    if (bad) return bfail(b, BVHCD_E_INVALID, "face vertex id out of range");
     b->topo = true;
     b->built = false;
     drop_graphs(b);
@@ -455,6 +471,7 @@ int bvhcd_batch_set_topology(bvhcd_batch* b, const int32_t* faces, int where) {
 
 int bvhcd_batch_update_vertices(bvhcd_batch* b, const float* xyz, int where) {
     if (!b || !xyz) return BVHCD_E_INVALID;
+    if (where != BVHCD_MEM_HOST && where != BVHCD_MEM_DEVICE) return bfail(b, BVHCD_E_INVALID, "bad memory kind");
     const hipMemcpyKind kind = where == BVHCD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (hipMemcpyAsync(b->d.verts, xyz, sizeof(float) * 3 * b->V, kind, b->stream) != hipSuccess)
         return bfail(b, BVHCD_E_DEVICE, "vertex copy failed");

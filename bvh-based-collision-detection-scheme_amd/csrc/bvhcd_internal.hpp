@@ -280,6 +280,7 @@ enum Counter : int {
     C_ADJ = 24,        // shared-vertex entries heading the current external front (last partition)
     C_NO_IMPLICIT = 25,// the last refit cannot vouch for shared-vertex pairs: a vertex coordinate
                        // above 1e8 in magnitude (or NaN / inf), or boxes given by the caller
+    C_BADFACE = 26,    // set_topology: a face vertex id outside its vertex range
     C_NUM = 32
 };
 enum Counter64 : int {

@@ -831,7 +831,9 @@ __global__ void __launch_bounds__(kBlock) k_rp_chunk_sort(DevState d) {
             // entry, side by mark bit 2 (right child)
             const int4 tp = d.itopo[v[k].x];
             int32_t dest = -1;
-            if (tp.z >= 0 && d.rp_tpos[tp.z] < seg_end) {
+            // rp_tpos holds positions only when the top tree has a list (seg_end > 0); deep
+            // top trees merged by climbing never write it
+            if (seg_end > 0 && tp.z >= 0 && d.rp_tpos[tp.z] < seg_end) {
                 const int4 pe = d.rp_tlist[d.rp_tpos[tp.z]];
                 dest = (tp.w & 4) ? pe.z : pe.y;
             }
