@@ -4,12 +4,19 @@
 One step = one frame of the hot path: vertices of frame f (already in HBM, read in place as a
 simulator hands its device buffer over: BVHCD_MEM_DEVICE_BORROW) -> BVH maintenance
 (build / refit / restructure) -> BVTT front maintenance -> narrow phase -> colliding pairs.
-The run replays the seeded 400-frame sequence from frame 0 (warm-up frames first, then the
-timed frames), with the plain scheduler (reference STATIC_MANDATORY schedule + restructure
-heuristic) and, when weights are available, the learned restructure scheduler.
+The timed frames are the last `steps` frames of the seeded 400-frame sequence (the folded part:
+the flap passes through the sheet from about frame 100 on), after `warmup` frames just before
+them; each schedule starts its own context at the first warm-up frame:
 
-Output: one JSON line (metric, value = frames/s of the reported scheduler, roofline of the
-dominant kernel from live HIP-event timing, CPU baseline from the host oracle on a sample).
+  plain             the reference's active scheme (STATIC_MANDATORY with divergent mark) plus
+                    its restructure heuristic
+  plain_refit_only  the reference's REFIT_ONLY_FRONT scheme (one BUILD, then REFIT; front cycle)
+  learned_no_net    the learned schedule's skeleton with the network off
+  learned           the learned restructure scheduler (trained weights)
+
+Output: one JSON line (metric, value = frames/s of the learned scheduler, every schedule's rate,
+the roofline of the dominant kernel from live HIP-event timing, the CPU baseline from the host
+oracle on a sample, and short runs of the 4-layer stress scene and the batched scenes).
 """
 from __future__ import annotations
 
@@ -78,20 +85,19 @@ def frame_digests(pkg, sc, faces, frames_dev, count, scheduler, weights):
     return out
 
 
-def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights=None, digests_upto=0,
-            autotune=True, instrument=False):
-    """Warm-up frames, then `steps` timed frames. instrument=False: wall clock only (the rate that
-    is reported); instrument=True: per-stage hipEvents and counters on every frame (each event is
-    a stream barrier, so this pass is slower and is used only for the stage breakdown)."""
+def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights=None, autotune=False,
+            instrument=False, readout=None):
+    """Warm-up frames, then `steps` timed frames (frames_dev[0] is the first warm-up frame).
+    instrument=False: wall clock only (the rate that is reported); instrument=True: per-stage
+    hipEvents and counters on every frame (each event is a stream barrier, so this pass is slower
+    and only feeds the stage breakdown). readout = (device pointer, capacity in pairs): after
+    every timed step the narrow-phase pairs are gathered into that device buffer."""
     ctx = make_ctx(pkg, sc, faces, scheduler, weights)
     nbytes = sc.num_verts * 12
     base = frames_dev.data_ptr()
-    digests = []
     for f in range(warmup):
         ctx.update_vertices(base + f * nbytes, pkg.MEM_DEVICE_BORROW)
         ctx.step_scheduled()
-        if f < digests_upto:
-            digests.append(ctx.digest(True))
     if autotune:  # launch shapes tuned on this workload before the timed frames (process-wide table)
         ctx.autotune()
         chosen = [ln for ln in pkg.autotune_report().splitlines() if ln.endswith("*")]
@@ -100,21 +106,25 @@ def run_gpu(pkg, sc, faces, frames_dev, warmup, steps, scheduler, torch, weights
     if instrument:
         ctx.set_timing(steps)
     torch.cuda.synchronize()
+    peak_pairs = 0
     t0 = time.perf_counter()
     for k in range(steps):
         f = warmup + k
         ctx.update_vertices(base + f * nbytes, pkg.MEM_DEVICE_BORROW)
         ctx.step_scheduled()
+        if readout is not None:
+            peak_pairs = max(peak_pairs, ctx.pairs_to_device(readout[0], readout[1]))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms, by, ops = ctx.timing_frames() if instrument else (None, None, None)
+    counts = ctx.timing_counts() if instrument else None
     st = ctx.stats()
     last_digest = ctx.digest(True)
     ctx.close()
     restructured = int((ops[:, 0] == pkg.BVH_RESTRUCTURE).sum()) if instrument else None
     updates = int((ops[:, 1] == pkg.FRONT_UPDATE).sum()) if instrument else None
-    return dict(wall_s=t1 - t0, ms=ms, bytes=by, ops=ops, stats=st, digests=digests, last_digest=last_digest,
-                restructured=restructured, updates=updates)
+    return dict(wall_s=t1 - t0, fps=steps / (t1 - t0), ms=ms, bytes=by, ops=ops, counts=counts, stats=st,
+                last_digest=last_digest, restructured=restructured, updates=updates, peak_pairs=peak_pairs)
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_cloth1024.md")
@@ -137,32 +147,57 @@ def profiled_traffic(kernel, workload, schedule):
     return None
 
 
-def roofline(res):
-    """Dominant (stage, op) group over the timed frames, algorithmic bytes / event time."""
-    ms, by, ops = res["ms"], res["bytes"], res["ops"]
+OPS = ["PURE", "GENERATE", "UPDATE", "REORDER", "KEEP", "RESTRUCTURE"]
+STAGE_KERNEL = {0: "build kernels", 1: "k_refit+k_refit_top", 2: "restructure kernels", 4: "ordering kernels",
+                5: "k_narrow"}
+
+
+def compulsory_bytes(stage, op, T, V, by, cnt):
+    """Bytes a frame's stage must move at least once: KEEP reads each tested front entry (8 B)
+    and at most every node and leaf record once (32 B each) and writes its listed pairs (8 B);
+    the other stages' byte formulas already count every access once."""
+    if stage == 3 and op == 4:
+        tested = cnt[3] + cnt[4] - cnt[6]
+        return 8.0 * tested + 32.0 * (2 * T - 1) + 8.0 * (cnt[1] - cnt[6])
+    return by
+
+
+def roofline(res, T, V):
+    """Stage groups (stage, front op) over the timed frames; the dominant one is the
+    byte-moving group with the largest total event time (the scheduler's own stage moves no
+    per-triangle data and is reported beside it)."""
+    ms, by, ops, cnt = res["ms"], res["bytes"], res["ops"], res["counts"]
     groups = {}
     for f in range(ms.shape[0]):
         for s in range(7):
             if ms[f, s] <= 0:
                 continue
             key = (s, int(ops[f, 1]) if s in (3, 4) else -1)
-            g = groups.setdefault(key, [0.0, 0.0, 0])
+            g = groups.setdefault(key, [0.0, 0.0, 0.0, 0])
             g[0] += ms[f, s]
             g[1] += by[f, s]
-            g[2] += 1
-    (s, op), (t, b, n) = max(groups.items(), key=lambda kv: kv[1][0])
-    name = {0: "build kernels", 1: "k_refit", 2: "restructure kernels", 4: "ordering kernels",
-            5: "k_narrow", 6: "restructure selection"}.get(s, FRONT_KERNEL.get(op, "front"))
-    achieved = b / (t * 1e-3) / 1e9
-    stage_summary = {}
-    for (s2, op2), (t2, b2, n2) in sorted(groups.items()):
-        label = STAGES[s2] + (f"/{['PURE', 'GENERATE', 'UPDATE', 'REORDER', 'KEEP', 'RESTRUCTURE'][op2]}" if op2 >= 0 else "")
-        stage_summary[label] = {"frames": n2, "avg_ms": round(t2 / n2, 4),
-                                "GBps": round(b2 / (t2 * 1e-3) / 1e9, 1),
-                                "frac_peak": round(b2 / (t2 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
-    return {"kernel": name, "stage": STAGES[s], "launches": n, "avg_ms": t / n,
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None}, stage_summary
+            g[2] += compulsory_bytes(s, key[1], T, V, by[f, s], cnt[f])
+            g[3] += 1
+    summary = {}
+    for (s2, op2), (t2, b2, c2, n2) in sorted(groups.items()):
+        label = STAGES[s2] + (f"/{OPS[op2]}" if op2 >= 0 else "")
+        summary[label] = {"frames": n2, "avg_ms": round(t2 / n2, 4),
+                          "formula_GBps": round(b2 / (t2 * 1e-3) / 1e9, 1),
+                          "compulsory_GBps": round(c2 / (t2 * 1e-3) / 1e9, 1),
+                          "frac_peak": round(c2 / (t2 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    moving = {k: v for k, v in groups.items() if k[0] != 6}
+    (s, op), (t, b, c, n) = max(moving.items(), key=lambda kv: kv[1][0])
+    name = STAGE_KERNEL.get(s, FRONT_KERNEL.get(op, "front"))
+    achieved = c / (t * 1e-3) / 1e9
+    roof = {"kernel": name, "stage": STAGES[s] + (f"/{OPS[op]}" if op >= 0 else ""), "launches": n,
+            "avg_ms": round(t / n, 5), "bytes_per_launch": round(c / n), "bound": "hbm",
+            "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "formula_bytes_per_launch": round(b / n), "formula_GBps": round(b / (t * 1e-3) / 1e9, 1)}
+    if (6, -1) in groups:
+        t6, _, _, n6 = groups[(6, -1)]
+        roof["schedule_stage"] = {"frames": n6, "avg_ms": round(t6 / n6, 4)}
+    return roof, summary
 
 
 def copy_ceiling(torch):
@@ -182,16 +217,29 @@ def copy_ceiling(torch):
     return round(2 * (1 << 30) / (ms * 1e-3) / 1e9, 1)
 
 
-def cpu_baseline(pkg, sc, faces, budget_s, threads, gpu_digests):
-    """Host oracle (restatement of the reference algorithm) on frames 0..S-1, same schedule."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc, faces, first, budget_s, threads, gpu_digests):
+    """Host oracle (restatement of the reference algorithm, same plain schedule) on the frames
+    from `first` on, as many as fit in budget_s seconds of stepping; digests checked against
+    the GPU plain schedule's digests of the same frames."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # test infrastructure, used only as the CPU baseline and checker
 
     orc = oracle.OracleContext(faces, sc.num_verts)
     host = np.empty((sc.num_verts, 3), np.float32)
     done, t_total, match = 0, 0.0, True
-    while t_total < budget_s and done < sc.frames:
-        sc.frame(done, host)
+    while t_total < budget_s and first + done < sc.frames:
+        sc.frame(first + done, host)
         orc.update_vertices(host)
         t0 = time.perf_counter()
         orc.step_scheduled(threads=threads)
@@ -237,6 +285,7 @@ def batch_bench(pkg, torch, steps, warmup, no_autotune):
     dt = time.perf_counter() - t0
     counts = b.pair_counts()
     b.close()
+    del dev
     n = frames - warmup
     return {"metric": "batched CD triangles/sec (64 scenes, 2e3-2e5 tris each, exact pairs)",
             "value": round(T * n / dt, 1), "unit": "triangles/s", "n_gpus": 1, "steps": n, "warmup": warmup,
@@ -244,6 +293,32 @@ def batch_bench(pkg, torch, steps, warmup, no_autotune):
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"batch64: {len(scenes)} sheet+icosphere scenes, {T} triangles", "graph": True},
             "pairs_last_frame": int(counts.sum())}
+
+
+def stress_bench(pkg, torch, weights, steps=10, warmup=6):
+    """The 4-layer stress scene (8,372,232 triangles): its last `steps` frames, plain and learned
+    schedules, peak front size and the refit / KEEP rates of an instrumented learned pass."""
+    sc = pkg.Scene.preset("layers4")
+    first = sc.frames - steps - warmup
+    faces = sc.faces()
+    frames_dev = upload_frames(pkg, sc, first, steps + warmup, torch)
+    out = {"workload": f"layers4: {sc.num_tris} triangles, frames {first + warmup}..{sc.frames - 1} of {sc.frames}"}
+    plain = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_PLAIN, torch)
+    out["plain_fps"] = round(plain["fps"], 3)
+    sched = pkg.SCHED_LEARNED if weights is not None else pkg.SCHED_LEARNED_NO_NET
+    learned = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, sched, torch, weights=weights)
+    out["learned_fps"] = round(learned["fps"], 3)
+    out["learned_vs_plain"] = round(learned["fps"] / plain["fps"], 4)
+    out["pairs_identical_last_frame"] = plain["last_digest"] == learned["last_digest"]
+    out["pairs_last_frame"] = learned["last_digest"][1]
+    inst = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, sched, torch, weights=weights, instrument=True)
+    cnt = inst["counts"]
+    out["peak_front_entries"] = int((cnt[:, 3] + cnt[:, 4]).max())
+    _, stages = roofline(inst, sc.num_tris, sc.num_verts)
+    out["stages"] = {k: v for k, v in stages.items() if k in ("refit", "front/KEEP", "narrow")}
+    del frames_dev
+    torch.cuda.synchronize()
+    return out
 
 
 def main():
@@ -254,8 +329,10 @@ def main():
     ap.add_argument("--workload", default="cloth1024",
                     choices=["cloth1024", "cloth256", "layers4", "spheres", "batch64"],
                     help="batch64: the batched-scene workload (its own JSON line, triangles/s)")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work, all cores")
+    ap.add_argument("--cpu-budget-1t", type=float, default=10.0, help="seconds of oracle CPU work, one thread")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the 4-layer and batched-scene runs")
     ap.add_argument("--weights", default=os.path.join(ROOT, "bvh-based-collision-detection-scheme_amd",
                                                       "mlp_scheduler.npz"),
                     help="trained scheduler (params + threshold) written by train_scheduler.py")
@@ -280,55 +357,72 @@ def main():
               flush=True)
         return
     sc = pkg.Scene.preset(args.workload)
-    steps = min(args.steps, sc.frames - args.warmup)
-    warmup = args.warmup
+    steps = max(1, min(args.steps, sc.frames - 1))
+    warmup = max(1, min(args.warmup, sc.frames - steps))
+    first = sc.frames - steps - warmup  # the timed frames are the sequence's last `steps`
     faces = sc.faces()
-    log(f"{args.workload}: {sc.num_tris} triangles, {sc.num_verts} vertices, warmup {warmup}, steps {steps}")
-    frames_dev = upload_frames(pkg, sc, 0, warmup + steps, torch)
+    T, V = sc.num_tris, sc.num_verts
+    log(f"{args.workload}: {T} triangles, {V} vertices, frames {first}..{sc.frames - 1} "
+        f"(warmup {warmup}, timed {first + warmup}..{sc.frames - 1})")
+    frames_dev = upload_frames(pkg, sc, first, warmup + steps, torch)
 
-    ndig = min(warmup, 12)
-    plain = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_PLAIN, torch, digests_upto=ndig,
-                    autotune=not args.no_autotune)
-    plain_fps = steps / plain["wall_s"]
-    log(f"plain: {plain_fps:.1f} frames/s ({1e3 * plain['wall_s'] / steps:.3f} ms/frame)")
-
-    learned = None
     weights = None
-    identical = None
     if os.path.exists(args.weights):
         z = np.load(args.weights, allow_pickle=False)
         weights = (z["params"], float(z["threshold"]))
-        learned = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED, torch, weights=weights,
-                          autotune=not args.no_autotune)
-        log(f"learned: {steps / learned['wall_s']:.1f} frames/s")
-        if not args.no_verify:  # untimed replay: pair sets of every frame, learned vs plain
-            dp = frame_digests(pkg, sc, faces, frames_dev, warmup + steps, pkg.SCHED_PLAIN, None)
-            dl = frame_digests(pkg, sc, faces, frames_dev, warmup + steps, pkg.SCHED_LEARNED, weights)
-            identical = dp == dl
-            log(f"pair sets identical on all {len(dp)} frames: {identical}")
+    schedules = [("plain", pkg.SCHED_PLAIN), ("plain_refit_only", pkg.SCHED_REFIT_ONLY_FRONT),
+                 ("learned_no_net", pkg.SCHED_LEARNED_NO_NET)]
+    if weights is not None:
+        schedules.append(("learned", pkg.SCHED_LEARNED))
+    rates, runs = {}, {}
+    for k, (name, sched) in enumerate(schedules):
+        r = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, sched, torch,
+                    weights=weights if sched == pkg.SCHED_LEARNED else None,
+                    autotune=(k == 0 and not args.no_autotune))
+        rates[name] = round(r["fps"], 3)
+        runs[name] = r
+        log(f"{name}: {r['fps']:.1f} frames/s ({1e3 * r['wall_s'] / steps:.3f} ms/frame)")
+    rep_name = "learned" if weights is not None else "plain"
+    rep = runs[rep_name]
 
-    rep = learned if learned is not None else plain
-    value = steps / rep["wall_s"]
+    identical, plain_digests = None, []
+    if not args.no_verify:  # untimed replays: pair sets of every frame under every schedule
+        digests = {name: frame_digests(pkg, sc, faces, frames_dev, warmup + steps, sched,
+                                       weights if sched == pkg.SCHED_LEARNED else None)
+                   for name, sched in schedules}
+        plain_digests = digests["plain"]
+        identical = all(d == plain_digests for d in digests.values())
+        log(f"pair sets identical under all {len(digests)} schedules on all {len(plain_digests)} frames: {identical}")
+
+    # the reported schedule again with the pairs gathered into a device buffer after every frame
+    cap = int(rep["last_digest"][1] * 1.25) + (1 << 20)
+    buf = torch.empty((cap, 2), dtype=torch.int32, device="cuda")
+    ro = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED if weights is not None else
+                 pkg.SCHED_PLAIN, torch, weights=weights, readout=(buf.data_ptr(), cap))
+    del buf
+    log(f"{rep_name} with device pair read-out: {ro['fps']:.1f} frames/s (up to {ro['peak_pairs']} pairs)")
+
     # stage breakdown: the same frames again with per-stage events (launch shapes already tuned)
-    inst = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED if learned is not None
-                   else pkg.SCHED_PLAIN, torch, weights=weights, autotune=False, instrument=True)
-    log(f"instrumented pass: {steps / inst['wall_s']:.1f} frames/s ({inst['restructured']} restructure frames, "
+    inst = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED if weights is not None
+                   else pkg.SCHED_PLAIN, torch, weights=weights, instrument=True)
+    log(f"instrumented pass: {inst['fps']:.1f} frames/s ({inst['restructured']} restructure frames, "
         f"{inst['updates']} UPDATE frames)")
-    roof, stages = roofline(inst)
+    roof, stages = roofline(inst, T, V)
     roof["copy_ceiling_GBps"] = copy_ceiling(torch)
-    traffic = profiled_traffic(roof["kernel"], args.workload, "learned" if learned is not None else "plain")
+    traffic = profiled_traffic(roof["kernel"], args.workload, rep_name)
     if traffic is not None:  # measured HBM bytes per launch; its rate over the same event time
         roof["traffic"] = traffic
         roof["traffic_GBps"] = round(traffic / (roof["avg_ms"] * 1e-3) / 1e9, 1)
         roof["traffic_frac"] = round(roof["traffic_GBps"] / PEAK_HBM_GBS, 4)
         roof["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT)
+    cnt = inst["counts"]
 
     host_fps = None
     if args.host_upload:  # same frames, vertices handed over from (pinned) host memory every frame
         pinned = torch.empty((warmup + steps,) + tuple(frames_dev.shape[1:]), dtype=torch.float32, pin_memory=True)
         pinned.copy_(frames_dev[: warmup + steps].cpu())
         host_frames = [pinned[f].numpy() for f in range(warmup + steps)]
-        ctx = make_ctx(pkg, sc, faces, pkg.SCHED_LEARNED if learned is not None else pkg.SCHED_PLAIN, weights)
+        ctx = make_ctx(pkg, sc, faces, pkg.SCHED_LEARNED if weights is not None else pkg.SCHED_PLAIN, weights)
         for f in range(warmup):
             ctx.update_vertices(host_frames[f])
             ctx.step_scheduled()
@@ -341,20 +435,34 @@ def main():
         host_fps = round(steps / (time.perf_counter() - t0), 3)
         ctx.close()
         log(f"host-upload (PCIe-inclusive) rate: {host_fps} frames/s")
+    del frames_dev
+    torch.cuda.synchronize()
+
+    extras = {}
+    if not args.no_extras and args.workload == "cloth1024":
+        extras["layers4"] = stress_bench(pkg, torch, weights)
+        log(f"layers4: {extras['layers4']}")
+        extras["batch64"] = batch_bench(pkg, torch, 20, 12, args.no_autotune)
+        log(f"batch64: {extras['batch64']['value']} triangles/s")
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        done, t_cpu, match, checked = cpu_baseline(pkg, sc, faces, args.cpu_budget, threads, plain["digests"])
-        cpu = {"value": round(done / t_cpu, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"frames 0..{done - 1} of the same sequence, plain schedule, host oracle "
-                         f"(C++ restatement, std::thread x{threads})",
+        # the box grants one GPU's share of its host cores (16); all of them, then one
+        cores = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1)
+        done, t_cpu, match, checked = cpu_baseline(sc, faces, first, args.cpu_budget, cores, plain_digests)
+        cpu = {"value": round(done / t_cpu, 4), "unit": "frames/s", "cores": cores, "kind": "port",
+               "cpu_model": cpu_model(),
+               "sample": f"frames {first}..{first + done - 1} of the same sequence (the first warm-up frame "
+                         f"on), plain schedule, host oracle (C++ restatement, std::thread x{cores})",
                "pairs_match_gpu_frames": checked, "pairs_match": bool(match)}
-        log(f"cpu oracle: {cpu['value']} frames/s over {done} frames, GPU digests match: {match} ({checked} frames)")
+        log(f"cpu oracle: {cpu['value']} frames/s over {done} frames on {cores} threads, GPU digests match: "
+            f"{match} ({checked} frames)")
+        done1, t1, match1, _ = cpu_baseline(sc, faces, first, args.cpu_budget_1t, 1, plain_digests)
+        cpu["single_thread"] = {"value": round(done1 / t1, 4), "frames": done1, "pairs_match": bool(match1)}
 
     out = {
         "metric": "CD frames/sec @2M-tri cloth, learned vs plain sched, exact pairs; HBM GB/s",
-        "value": round(value, 3),
+        "value": round(rep["fps"], 3),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": steps,
@@ -365,21 +473,28 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: seeded folding cloth, {sc.num_tris} triangles, "
-                               f"{sc.num_verts} vertices, frames {warmup}..{warmup + steps - 1} of {sc.frames}",
-                   "scheduler": "learned" if learned is not None else "plain",
-                   "narrow_phase": True},
-        "plain_fps": round(plain_fps, 3),
-        "learned_fps": round(steps / learned["wall_s"], 3) if learned is not None else None,
-        "learned_vs_plain": round((steps / learned["wall_s"]) / plain_fps, 4) if learned is not None else None,
-        "learned_pairs_identical_all_frames": identical,
+        "config": {"workload": f"{args.workload}: seeded folding cloth, {T} triangles, {V} vertices, "
+                               f"timed frames {first + warmup}..{sc.frames - 1} of {sc.frames} "
+                               f"(warm-up from frame {first})",
+                   "scheduler": rep_name, "narrow_phase": True},
+        "schedules_fps": rates,
+        "plain_fps": rates["plain"],
+        "learned_fps": rates.get("learned"),
+        "learned_vs_plain": round(rates["learned"] / rates["plain"], 4) if "learned" in rates else None,
+        "learned_vs_no_net": round(rates["learned"] / rates["learned_no_net"], 4) if "learned" in rates else None,
+        "learned_vs_refit_only": round(rates["learned"] / rates["plain_refit_only"], 4) if "learned" in rates else None,
+        "pairs_identical_all_schedules_all_frames": identical,
+        "fps_with_device_pair_readout": round(ro["fps"], 3),
         "host_upload_fps": host_fps,
         "restructure_frames": inst["restructured"],
-        "instrumented_fps": round(steps / inst["wall_s"], 3),
+        "update_frames": inst["updates"],
+        "instrumented_fps": round(inst["fps"], 3),
         "pairs_last_frame": rep["last_digest"][1],
+        "peak_front_entries": int((cnt[:, 3] + cnt[:, 4]).max()),
         "roofline": roof,
         "stages": stages,
         "cpu_baseline": cpu,
+        "other_configs": extras or None,
     }
     print(json.dumps(out), flush=True)
     if args.calibrate_copy:

@@ -30,6 +30,8 @@ MEM_HOST, MEM_DEVICE, MEM_DEVICE_BORROW = 0, 1, 2
 BVH_BUILD, BVH_REFIT, BVH_RESTRUCTURE, BVH_NONE = 0, 1, 2, 3
 FRONT_PURE_BVH_CD, FRONT_GENERATE, FRONT_UPDATE, FRONT_REORDER, FRONT_KEEP, FRONT_RESTRUCTURE = 0, 1, 2, 3, 4, 5
 SCHED_PLAIN, SCHED_LEARNED, SCHED_GENERATE = 0, 1, 2
+SCHED_FRONT_GENERATE, SCHED_STATIC_NODIV, SCHED_REFIT_ONLY_FRONT, SCHED_REFIT_ONLY_FRONT_NODIV = 3, 4, 5, 6
+SCHED_ARCSIM_STATIC, SCHED_ARCSIM_PERIODIC, SCHED_LEARNED_NO_NET = 7, 8, 9
 SCENE_TWO_SPHERES, SCENE_CLOTH, SCENE_LAYERS, SCENE_SHEET_SPHERE = 0, 1, 2, 3
 MLP_NPARAM = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
@@ -101,6 +103,7 @@ def lib():
         "bvhcd_get_stats": (C.c_int, [P, C.POINTER(Stats)]),
         "bvhcd_set_timing": (C.c_int, [P, C.c_int]),
         "bvhcd_timing_frames": (C.c_int, [P, I64, P, P, P, C.POINTER(I64)]),
+        "bvhcd_timing_counts": (C.c_int, [P, I64, P, C.POINTER(I64)]),
         "bvhcd_export_bvh": (C.c_int, [P] + [P] * 14),
         "bvhcd_export_front": (C.c_int, [P, C.c_int, P, I64, C.POINTER(I64)]),
         "bvhcd_export_scene_box": (C.c_int, [P, P]),
@@ -126,6 +129,7 @@ def lib():
         "bvhcd_autotune_set": (C.c_int, [C.c_int, I64, C.c_uint32, C.c_uint32]),
         "bvhcd_update_boxes": (C.c_int, [P, P, I64, C.c_int]),
         "bvhcd_mlp_set_repair_cost": (C.c_int, [P, C.c_float]),
+        "bvhcd_schedule_ops": (C.c_int, [C.c_int, I64, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "bvhcd_scene_create": (C.c_int, [C.c_int, C.POINTER(SceneParams), C.POINTER(P)]),
         "bvhcd_scene_destroy": (None, [P]),
         "bvhcd_scene_num_tris": (I64, [P]),
@@ -331,6 +335,16 @@ class CollisionContext:
                       "timing_frames")
         return ms, by, ops
 
+    def timing_counts(self):
+        """counts[n, 8] of the recorded frames: node tests, bp pairs, np pairs, internal and
+        external front length, overflow flags, implicit pairs, implicit narrow-phase hits."""
+        n = C.c_int64()
+        self._chk(lib().bvhcd_timing_counts(self._h, 0, None, C.byref(n)), "timing_counts")
+        out = np.zeros((n.value, 8), np.int64)
+        if n.value:
+            self._chk(lib().bvhcd_timing_counts(self._h, n.value, _ptr(out), C.byref(n)), "timing_counts")
+        return out
+
     def pairs(self, filtered=True) -> np.ndarray:
         n = C.c_int64()
         self._chk(lib().bvhcd_get_pairs(self._h, None, 0, C.byref(n), MEM_HOST, 1 if filtered else 0), "get_pairs")
@@ -339,6 +353,14 @@ class CollisionContext:
             self._chk(lib().bvhcd_get_pairs(self._h, _ptr(out), n.value, C.byref(n), MEM_HOST,
                                             1 if filtered else 0), "get_pairs")
         return out
+
+    def pairs_to_device(self, ptr, cap, filtered=True) -> int:
+        """Writes the pairs (int32 pairs of triangle ids) into the device buffer at `ptr` (room for
+        `cap` pairs), stream-ordered; returns the total pair count (may exceed cap)."""
+        n = C.c_int64()
+        self._chk(lib().bvhcd_get_pairs(self._h, C.c_void_p(int(ptr)), int(cap), C.byref(n), MEM_DEVICE,
+                                        1 if filtered else 0), "get_pairs")
+        return int(n.value)
 
     def digest(self, filtered=True):
         d = C.c_uint64()
@@ -531,19 +553,13 @@ class Batch:
         self._chk(lib().bvhcd_batch_autotune(self._h), "batch_autotune")
 
 
-def plain_schedule(frame, bvh_cycle=12, front_cycle=6):
-    """BenchmarkLogic::maintainScheme, STATIC_MANDATORY with divergent mark
+def plain_schedule(frame, bvh_cycle=12, front_cycle=6, scheduler=None):
+    """The ops a plain scheme schedules at `frame` (bvhcd_schedule_ops): by default the
+    reference's BenchmarkLogic::maintainScheme, STATIC_MANDATORY with divergent mark
     (system/Logic/BenchmarkLogic.hpp:102-117)."""
-    bvh = BVH_REFIT if frame % bvh_cycle else BVH_BUILD
-    fr = frame % front_cycle
-    if fr == 0:
-        front = FRONT_GENERATE if frame == 0 else FRONT_UPDATE
-    elif fr == front_cycle - 1:
-        front = FRONT_REORDER
-    else:
-        front = FRONT_KEEP
-    if bvh == BVH_BUILD:
-        front = FRONT_GENERATE
-    if (frame + 1) % bvh_cycle == 0:
-        front = FRONT_KEEP
-    return bvh, front
+    b, f = C.c_int(), C.c_int()
+    sched = SCHED_PLAIN if scheduler is None else int(scheduler)
+    st = lib().bvhcd_schedule_ops(sched, int(frame), int(bvh_cycle), int(front_cycle), C.byref(b), C.byref(f))
+    if st != OK:
+        raise BvhcdError(st, "schedule_ops")
+    return b.value, f.value

@@ -69,6 +69,8 @@ struct bvhcd_ctx {
 
 namespace {
 
+bool is_learned(int sched) { return sched == BVHCD_SCHED_LEARNED || sched == BVHCD_SCHED_LEARNED_NO_NET; }
+
 int fail(bvhcd_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     return code;
@@ -459,7 +461,7 @@ int run_bvh(bvhcd_ctx* c, int& bvh_op, bool& topo_changed) {
             end(c, S_REFIT);
             break;
         case BVHCD_BVH_RESTRUCTURE: {
-            int st = run_restructure(c, bvh_op, topo_changed, c->cfg.scheduler == BVHCD_SCHED_LEARNED);
+            int st = run_restructure(c, bvh_op, topo_changed, is_learned(c->cfg.scheduler));
             if (st) return st;
             break;
         }
@@ -485,7 +487,7 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     // front-quality heuristic: its fronts are sorted by query leaf (neighbouring queries
     // together, the best locality for KEEP without descents), or by node id once KEEP has been
     // seen descending often (lanes then descend the same subtrees).
-    const bool plain_order = c->cfg.scheduler != BVHCD_SCHED_LEARNED;
+    const bool plain_order = !is_learned(c->cfg.scheduler);
     const bool quality = c->cfg.enable_restructure != 0 && plain_order;
     const bool order = plain_order || c->learned_node_order;  // node order, else query order
     if (front_op != BVHCD_FRONT_KEEP && front_op != BVHCD_FRONT_PURE_BVH_CD) d.front_version++;
@@ -570,15 +572,49 @@ int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     return BVHCD_OK;
 }
 
-// BenchmarkLogic::maintainScheme, STATIC_MANDATORY with divergent mark.
-void plain_schedule(int64_t f, int bc, int fc, int& bvh_op, int& front_op) {
-    bvh_op = (f % bc) ? BVHCD_BVH_REFIT : BVHCD_BVH_BUILD;
+// The reference's plain schemes (BenchmarkLogic::maintainScheme, BenchmarkLogic.hpp:90-156, and
+// getSchemeOpt, :56-87) at frame f counted from the first frame: bc = BVH rebuild cycle,
+// fc = front cycle. The learned schemes take the PLAIN table as their starting point.
+void plain_schedule(int sched, int64_t f, int bc, int fc, int& bvh_op, int& front_op) {
     const int64_t fr = f % fc;
-    if (fr == 0) front_op = f == 0 ? BVHCD_FRONT_GENERATE : BVHCD_FRONT_UPDATE;
-    else if (fr == fc - 1) front_op = BVHCD_FRONT_REORDER;
-    else front_op = BVHCD_FRONT_KEEP;
-    if (bvh_op == BVHCD_BVH_BUILD) front_op = BVHCD_FRONT_GENERATE;
-    if ((f + 1) % bc == 0) front_op = BVHCD_FRONT_KEEP;
+    // front cycle: GENERATE on the first frame, UPDATE at later cycle starts, then KEEP; with
+    // the divergent mark the last frame of each cycle REORDERs
+    auto cycle = [&](bool divergent) {
+        if (fr == 0) front_op = f == 0 ? BVHCD_FRONT_GENERATE : BVHCD_FRONT_UPDATE;
+        else if (divergent && fr == fc - 1) front_op = BVHCD_FRONT_REORDER;
+        else front_op = BVHCD_FRONT_KEEP;
+    };
+    switch (sched) {
+        case BVHCD_SCHED_GENERATE:
+            bvh_op = BVHCD_BVH_BUILD;
+            front_op = BVHCD_FRONT_PURE_BVH_CD;
+            break;
+        case BVHCD_SCHED_FRONT_GENERATE:
+            bvh_op = BVHCD_BVH_BUILD;
+            front_op = BVHCD_FRONT_GENERATE;
+            break;
+        case BVHCD_SCHED_STATIC_NODIV:
+            bvh_op = (f % bc) ? BVHCD_BVH_REFIT : BVHCD_BVH_BUILD;
+            cycle(false);
+            if (bvh_op == BVHCD_BVH_BUILD) front_op = BVHCD_FRONT_GENERATE;
+            break;
+        case BVHCD_SCHED_REFIT_ONLY_FRONT:
+        case BVHCD_SCHED_REFIT_ONLY_FRONT_NODIV:
+        case BVHCD_SCHED_ARCSIM_STATIC:
+            bvh_op = f ? BVHCD_BVH_REFIT : BVHCD_BVH_BUILD;
+            cycle(sched != BVHCD_SCHED_REFIT_ONLY_FRONT_NODIV);
+            break;
+        case BVHCD_SCHED_ARCSIM_PERIODIC:
+            bvh_op = (f % bc) ? BVHCD_BVH_REFIT : BVHCD_BVH_BUILD;
+            front_op = fr == 0 ? BVHCD_FRONT_GENERATE : BVHCD_FRONT_KEEP;
+            break;
+        default:  // STATIC_MANDATORY with divergent mark
+            bvh_op = (f % bc) ? BVHCD_BVH_REFIT : BVHCD_BVH_BUILD;
+            cycle(true);
+            if (bvh_op == BVHCD_BVH_BUILD) front_op = BVHCD_FRONT_GENERATE;
+            if ((f + 1) % bc == 0) front_op = BVHCD_FRONT_KEEP;
+            break;
+    }
 }
 
 int check_overflow(bvhcd_ctx* c) {
@@ -627,7 +663,8 @@ void release_upload(bvhcd_ctx* c) {
 
 int bvhcd_create(const bvhcd_config* cfg, bvhcd_ctx** out) {
     if (!cfg || !out || cfg->max_tris < 1 || cfg->max_verts < 1 || cfg->max_tris > (1ll << 30) ||
-        cfg->max_verts > (1ll << 30))
+        cfg->max_verts > (1ll << 30) || cfg->scheduler < BVHCD_SCHED_PLAIN ||
+        cfg->scheduler > BVHCD_SCHED_LEARNED_NO_NET)
         return BVHCD_E_INVALID;
     *out = nullptr;
     int ndev = 0;
@@ -764,6 +801,14 @@ int bvhcd_update_boxes(bvhcd_ctx* c, const float* boxes, int64_t T, int where) {
     return BVHCD_OK;
 }
 
+int bvhcd_schedule_ops(int sched, int64_t frame, int bvh_cycle, int front_cycle, int* bvh_op, int* front_op) {
+    if (!bvh_op || !front_op || frame < 0 || sched < BVHCD_SCHED_PLAIN || sched > BVHCD_SCHED_LEARNED_NO_NET)
+        return BVHCD_E_INVALID;
+    plain_schedule(sched, frame, bvh_cycle > 0 ? bvh_cycle : 12, front_cycle > 0 ? front_cycle : 6, *bvh_op,
+                   *front_op);
+    return BVHCD_OK;
+}
+
 int bvhcd_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     if (!c) return BVHCD_E_INVALID;
     return do_step(c, bvh_op, front_op);
@@ -772,9 +817,10 @@ int bvhcd_step(bvhcd_ctx* c, int bvh_op, int front_op) {
 int bvhcd_step_scheduled(bvhcd_ctx* c) {
     if (!c) return BVHCD_E_INVALID;
     int bvh_op, front_op;
-    if (c->cfg.scheduler == BVHCD_SCHED_GENERATE) return do_step(c, BVHCD_BVH_BUILD, BVHCD_FRONT_PURE_BVH_CD);
-    plain_schedule(c->frame, c->cfg.bvh_cycle, c->cfg.front_cycle, bvh_op, front_op);
-    if (c->cfg.scheduler == BVHCD_SCHED_LEARNED && c->built && learned_ready(c->learned)) {
+    const int sched = c->cfg.scheduler;
+    plain_schedule(sched, c->frame, c->cfg.bvh_cycle, c->cfg.front_cycle, bvh_op, front_op);
+    const bool net = sched == BVHCD_SCHED_LEARNED && learned_ready(c->learned);
+    if (c->built && (net || sched == BVHCD_SCHED_LEARNED_NO_NET)) {
         // Learned schedule: the periodic global maintenance of the plain schedule (rebuild +
         // regenerate every bvh_cycle frames, UPDATE / REORDER every front_cycle) is replaced by
         // per-subtree decisions. Every frame refits and keeps the front; every bvh_cycle frames
@@ -786,7 +832,8 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
         front_op = BVHCD_FRONT_KEEP;
         const int bc = c->cfg.bvh_cycle > 0 ? c->cfg.bvh_cycle : 12;
         const int fc = c->cfg.front_cycle > 0 ? c->cfg.front_cycle : 6;
-        if (c->frame % ((int64_t)bc << c->learned_backoff) == 0 || (c->cfg.enable_restructure && c->restr_pending))
+        if (net && (c->frame % ((int64_t)bc << c->learned_backoff) == 0 ||
+                    (c->cfg.enable_restructure && c->restr_pending)))
             c->learned_frame = true;
         if (c->frame % fc == 0 && c->last_front == BVHCD_FRONT_KEEP) {
             uint64_t visits = 0;
@@ -874,6 +921,18 @@ int bvhcd_set_timing(bvhcd_ctx* c, int frames) {
     return BVHCD_OK;
 }
 
+int bvhcd_timing_counts(bvhcd_ctx* c, int64_t cap, int64_t* counts, int64_t* n) {
+    if (!c || !n) return BVHCD_E_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *n = c->tn;
+    const int64_t k = std::min<int64_t>(cap, c->tn);
+    if (k <= 0 || !counts) return BVHCD_OK;
+    std::vector<uint64_t> log((size_t)kLog * k);
+    HIPCHK(c, hipMemcpy(log.data(), c->dlog, sizeof(uint64_t) * log.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < log.size(); ++i) counts[i] = (int64_t)log[i];
+    return BVHCD_OK;
+}
+
 int bvhcd_timing_frames(bvhcd_ctx* c, int64_t cap, double* ms, double* bytes, int32_t* ops, int64_t* n) {
     if (!c || !n) return BVHCD_E_INVALID;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -895,7 +954,7 @@ int bvhcd_timing_frames(bvhcd_ctx* c, int64_t cap, double* ms, double* bytes, in
         }
         if (bytes)
             frame_bytes((double)c->T, (double)c->V, bo, fo, (double)L[0], fi0, fe0, (double)L[3], (double)L[4],
-                        (double)L[1], (double)L[2], (double)c->trestr[f], c->cfg.scheduler != BVHCD_SCHED_LEARNED,
+                        (double)L[1], (double)L[2], (double)c->trestr[f], !is_learned(c->cfg.scheduler),
                         (double)L[6], (double)L[7], bytes + 8 * f);
         if (ops) { ops[2 * f] = bo; ops[2 * f + 1] = fo; }
         fi0 = (double)L[3];
@@ -941,7 +1000,7 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     }
     frame_bytes((double)c->T, (double)c->V, c->last_bvh, c->last_front, (double)c64[C64_VISITS], 0, 0,
                 (double)out->int_front, (double)out->ext_front, (double)out->bp_pairs, (double)out->np_pairs,
-                (double)c->restr_leaf, c->cfg.scheduler != BVHCD_SCHED_LEARNED, (double)ctr[C_IMPLICIT],
+                (double)c->restr_leaf, !is_learned(c->cfg.scheduler), (double)ctr[C_IMPLICIT],
                 (double)ctr[C_IMPL_NP], out->stage_bytes);
     if (ctr[C_OVF]) {
         if (ctr[C_OVF] & 16u) return fail(c, BVHCD_E_DEVICE, "traversal step bound exceeded (corrupted tree)");

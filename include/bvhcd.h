@@ -75,10 +75,30 @@ extern "C" {
                                        BvttFrontLooseIntra.cu:330-400): run by the library on the step
                                        after a repairing subtree rebuild; not accepted by bvhcd_step */
 
-/* ---- scheduler (bvhcd_step_scheduled) ---- */
-#define BVHCD_SCHED_PLAIN   0   /* reference STATIC_MANDATORY schedule + restructure heuristic */
+/* ---- scheduler (bvhcd_step_scheduled) ----
+   The plain schemes are the reference's CDSchemeType cases of BenchmarkLogic::maintainScheme
+   (system/Logic/BenchmarkLogic.hpp:90-156) and the ARCSim schemes of getSchemeOpt (:56-87),
+   counted from the first frame after bvhcd_set_topology. In every plain scheme a REFIT frame
+   after a flagging quality check becomes a RESTRUCTURE (LBvhFixedDeformable.cu:68-87) when
+   enable_restructure is set; a BUILD frame always rebuilds. */
+#define BVHCD_SCHED_PLAIN   0   /* STATIC_MANDATORY with divergent mark (the reference's active setting):
+                                   BUILD every bvh_cycle frames; front GENERATE / UPDATE at front-cycle
+                                   starts, REORDER at their ends, KEEP otherwise */
 #define BVHCD_SCHED_LEARNED 1   /* MLP per-subtree restructure choice, heuristic fallback */
-#define BVHCD_SCHED_GENERATE 2  /* reference GENERATE scheme: BUILD + PURE_BVH_CD each frame */
+#define BVHCD_SCHED_GENERATE 2  /* GENERATE scheme: BUILD + PURE_BVH_CD each frame */
+#define BVHCD_SCHED_FRONT_GENERATE 3  /* FRONT_GENERATE scheme: BUILD + front GENERATE each frame */
+#define BVHCD_SCHED_STATIC_NODIV 4    /* STATIC_MANDATORY without divergent mark (no REORDER) */
+#define BVHCD_SCHED_REFIT_ONLY_FRONT 5 /* REFIT_ONLY_FRONT with divergent mark: one BUILD, then REFIT;
+                                   front cycle as in PLAIN (BenchmarkLogic.hpp:129-141, whose missing
+                                   break falls into unreachable code; this is that branch's schedule) */
+#define BVHCD_SCHED_REFIT_ONLY_FRONT_NODIV 6 /* REFIT_ONLY_FRONT without divergent mark (no REORDER) */
+#define BVHCD_SCHED_ARCSIM_STATIC 7   /* getSchemeOpt scheme 0: one BUILD; front cycle with REORDER */
+#define BVHCD_SCHED_ARCSIM_PERIODIC 8 /* getSchemeOpt scheme 1: BUILD every bvh_cycle frames, front
+                                   GENERATE every front_cycle frames, KEEP otherwise */
+#define BVHCD_SCHED_LEARNED_NO_NET 9  /* the learned schedule with the network off (no reference
+                                   counterpart: the control that isolates what the MLP adds): one
+                                   BUILD, then REFIT + KEEP, UPDATE when KEEP's descents outgrow the
+                                   front; no subtree is ever restructured */
 
 typedef struct bvhcd_config {
     int64_t max_tris;        /* capacity in triangles (>= 1) */
@@ -146,6 +166,10 @@ int  bvhcd_update_boxes(bvhcd_ctx* ctx, const float* boxes, int64_t num_tris, in
 int  bvhcd_step(bvhcd_ctx* ctx, int bvh_op, int front_op);
 /* One frame with ops chosen by the configured scheduler. */
 int  bvhcd_step_scheduled(bvhcd_ctx* ctx);
+/* The ops a plain scheme schedules at `frame` before any restructure override (host only, no
+   context): BenchmarkLogic::maintainScheme / getSchemeOpt (system/Logic/BenchmarkLogic.hpp:56-156).
+   The learned schedulers start from the PLAIN table. cycles <= 0: 12 and 6. */
+int  bvhcd_schedule_ops(int sched, int64_t frame, int bvh_cycle, int front_cycle, int* bvh_op, int* front_op);
 
 /* Pairs of the last step as (tri_a, tri_b) int32 pairs with tri_a < tri_b, unordered list.
    filtered=1: narrow-phase hits; 0: broad-phase (AABB) pairs. Synchronises the stream. */
@@ -162,6 +186,11 @@ int  bvhcd_set_timing(bvhcd_ctx* ctx, int frames);
    bvhcd_stats.stage_ms), 8 algorithmic byte counts (layout of stage_bytes) and the executed
    (bvh_op, front_op). Synchronises the stream. */
 int  bvhcd_timing_frames(bvhcd_ctx* ctx, int64_t cap, double* ms, double* bytes, int32_t* ops, int64_t* n);
+/* The counters behind those byte counts, per recorded frame (8 values each): node tests,
+   broad-phase pairs, narrow-phase hits, internal and external front length after the step,
+   overflow flags, implicit shared-vertex pairs, of those narrow-phase hits (the reference's
+   Logger record of front sizes and pair counts, BvttFrontLooseIntra.cu:50-61). */
+int  bvhcd_timing_counts(bvhcd_ctx* ctx, int64_t cap, int64_t* counts, int64_t* n);
 
 /* Export the current BVH to host arrays (any pointer may be NULL). n = leaves = num_tris.
    codes/prim_idx/prim_map: n; int_*: n-1 (preorder); leaf_par/leaf_mark: n; leaf_lca: n+1;

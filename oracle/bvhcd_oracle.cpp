@@ -375,6 +375,7 @@ struct Ctx {
     int64_t frame = 0;
     int bvh_cycle = 12, front_cycle = 6;
     bool enable_restr = true;
+    int scheme = 0;              // scheme_ops id used by orc_step_scheduled
     int last_bvh_op = -1, last_front_op = -1;
     int64_t visits = 0, restr_subtrees = 0, restr_leaves = 0;
     int narrow = 1;
@@ -931,6 +932,42 @@ void plain_schedule(int64_t f, int bvh_cycle, int front_cycle, int& bvh_op, int&
     if ((f + 1) % bvh_cycle == 0) front_op = 4;
 }
 
+// The other schemes, case by case. Scheme ids follow the library's BVHCD_SCHED_* values:
+// 0 STATIC_MANDATORY divergent, 2 GENERATE, 3 FRONT_GENERATE, 4 STATIC_MANDATORY without the
+// divergent mark, 5 / 6 REFIT_ONLY_FRONT with / without it, 7 / 8 getSchemeOpt 0 / 1.
+// Ops: bvh 0 BUILD 1 REFIT; front 0 PURE_BVH_CD 1 GENERATE 2 UPDATE 3 REORDER 4 KEEP.
+void scheme_ops(int scheme, int64_t f, int bc, int fc, int& bvh_op, int& front_op) {
+    switch (scheme) {
+        case 2:  // GENERATE (BenchmarkLogic.hpp:94-97)
+            bvh_op = 0; front_op = 0; return;
+        case 3:  // FRONT_GENERATE (:98-101)
+            bvh_op = 0; front_op = 1; return;
+        case 4:  // STATIC_MANDATORY, no divergent mark (:118-126)
+            bvh_op = (f % bc) ? 1 : 0;
+            front_op = (f % fc == 0) ? (f == 0 ? 1 : 2) : 4;
+            if (bvh_op == 0) front_op = 1;
+            return;
+        case 5:  // REFIT_ONLY_FRONT, divergent mark (:130-140; its missing break reaches
+                 // unreachable code, the ops it computed are taken)
+        case 7:  // getSchemeOpt scheme 0 (:62-70)
+            bvh_op = f ? 1 : 0;
+            if (f % fc == 0) front_op = f == 0 ? 1 : 2;
+            else if (f % fc == fc - 1) front_op = 3;
+            else front_op = 4;
+            return;
+        case 6:  // REFIT_ONLY_FRONT, no divergent mark (:142-149)
+            bvh_op = f ? 1 : 0;
+            front_op = (f % fc == 0) ? (f == 0 ? 1 : 2) : 4;
+            return;
+        case 8:  // getSchemeOpt scheme 1 (:72-77)
+            bvh_op = (f % bc) ? 1 : 0;
+            front_op = (f % fc == 0) ? 1 : 4;
+            return;
+        default:
+            plain_schedule(f, bc, fc, bvh_op, front_op);
+    }
+}
+
 }  // namespace
 
 // ====================================================================== C API
@@ -1073,13 +1110,18 @@ int orc_step(void* p, int bvh_op, int front_op, int threads) { return step(*stat
 int orc_step_scheduled(void* p, int threads) {
     Ctx* c = static_cast<Ctx*>(p);
     int bvh_op, front_op;
-    plain_schedule(c->frame, c->bvh_cycle, c->front_cycle, bvh_op, front_op);
+    scheme_ops(c->scheme, c->frame, c->bvh_cycle, c->front_cycle, bvh_op, front_op);
+    // a pending restructure flag turns a REFIT into a RESTRUCTURE (LBvhFixedDeformable.cu:72-78)
     if (bvh_op == 1 && c->enable_restr && c->restr_pending) bvh_op = 2;
     return step(*c, bvh_op, front_op, threads);
 }
 void orc_plain_schedule(int64_t f, int bvh_cycle, int front_cycle, int* bvh_op, int* front_op) {
     plain_schedule(f, bvh_cycle, front_cycle, *bvh_op, *front_op);
 }
+void orc_scheme_ops(int scheme, int64_t f, int bvh_cycle, int front_cycle, int* bvh_op, int* front_op) {
+    scheme_ops(scheme, f, bvh_cycle, front_cycle, *bvh_op, *front_op);
+}
+void orc_set_scheme(void* p, int scheme) { static_cast<Ctx*>(p)->scheme = scheme; }
 
 int64_t orc_get_pairs(void* p, int filtered, int32_t* out, int64_t cap) {
     Ctx* c = static_cast<Ctx*>(p);

@@ -48,6 +48,8 @@ def lib():
         "orc_step": (I32, [P, I32, I32, I32]),
         "orc_step_scheduled": (I32, [P, I32]),
         "orc_plain_schedule": (None, [I64, I32, I32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "orc_scheme_ops": (None, [I32, I64, I32, I32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "orc_set_scheme": (None, [P, I32]),
         "orc_get_pairs": (I64, [P, I32, P, I64]),
         "orc_export_front": (I64, [P, I32, P, I64]),
         "orc_export_bvh": (None, [P] + [P] * 14),
@@ -147,12 +149,14 @@ def bruteforce_pairs(faces, verts, narrow=True, threads=0):
 class OracleContext:
     """Host restatement of the per-frame pipeline (same ops and schedule as the GPU context)."""
 
-    def __init__(self, faces, num_verts, bvh_cycle=12, front_cycle=6, enable_restructure=True, narrow=True):
+    def __init__(self, faces, num_verts, bvh_cycle=12, front_cycle=6, enable_restructure=True, narrow=True,
+                 scheme=0):
         f = np.ascontiguousarray(faces, np.int32)
         self.T = f.shape[0]
         self.V = int(num_verts)
         self._h = lib().orc_create(self.T, self.V, _p(f), bvh_cycle, front_cycle, 1 if enable_restructure else 0,
                                    1 if narrow else 0)
+        lib().orc_set_scheme(self._h, int(scheme))
 
     def close(self):
         if self._h:
@@ -222,6 +226,12 @@ class OracleContext:
 def plain_schedule(frame, bvh_cycle=12, front_cycle=6):
     b, f = C.c_int(), C.c_int()
     lib().orc_plain_schedule(frame, bvh_cycle, front_cycle, C.byref(b), C.byref(f))
+    return b.value, f.value
+
+
+def scheme_ops(scheme, frame, bvh_cycle=12, front_cycle=6):
+    b, f = C.c_int(), C.c_int()
+    lib().orc_scheme_ops(int(scheme), frame, bvh_cycle, front_cycle, C.byref(b), C.byref(f))
     return b.value, f.value
 
 

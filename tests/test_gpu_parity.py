@@ -94,6 +94,34 @@ def test_plain_schedule_parity(pkg, oracle, name, kw, frames):
     ctx.close()
 
 
+@pytest.mark.parametrize("scheme", [3, 4, 5, 6, 7, 8])
+def test_scheme_parity(pkg, oracle, scheme):
+    """The reference's other plain schemes (FRONT_GENERATE, STATIC_MANDATORY without the divergent
+    mark, REFIT_ONLY_FRONT with and without it, the two ARCSim schemes) through
+    bvhcd_step_scheduled: ops, pair sets and fronts equal the oracle's every frame, with the
+    restructure heuristic on."""
+    print(f"[gpu] scheme {scheme}", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=56, frames=40)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc, scheduler=scheme)
+    orc = oracle.OracleContext(f, sc.num_verts, scheme=scheme)
+    for fr in range(0, 40, 1):
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        orc.update_vertices(v)
+        ctx.step_scheduled()
+        orc.step_scheduled()
+        gs, os_ = ctx.stats(), orc.stats()
+        assert (gs["last_bvh_op"], gs["last_front_op"]) == (os_["last_bvh_op"], os_["last_front_op"]), fr
+        for narrow in (False, True):
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(narrow)), oracle.pair_set(orc.pairs(narrow)),
+                                          err_msg=f"frame {fr} narrow {narrow}")
+        for w in (0, 1):
+            np.testing.assert_array_equal(_rows(ctx.export_front(w)), _rows(orc.front(w)),
+                                          err_msg=f"front {w} frame {fr}")
+    ctx.close()
+
+
 def test_explicit_ops_parity(pkg, oracle):
     print("[gpu] explicit ops", flush=True)
     sc = pkg.Scene.preset("cloth256", n=48, frames=25)

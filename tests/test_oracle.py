@@ -410,3 +410,34 @@ def test_plain_schedule_table(oracle):
     # frame 0 build+generate; 1-4 refit+keep; 5 refit+reorder; 6 refit+update; 11 keep; 12 build+generate
     assert got[0] == (0, 1) and got[1] == (1, 4) and got[5] == (1, 3) and got[6] == (1, 2)
     assert got[11] == (1, 4) and got[12] == (0, 1) and got[17] == (1, 3) and got[18] == (1, 2)
+
+
+# Known answers read off BenchmarkLogic.hpp:56-153 for bvh_cycle 12, front_cycle 6:
+# (frame, bvh op, front op); bvh 0 BUILD 1 REFIT, front 0 PURE 1 GENERATE 2 UPDATE 3 REORDER 4 KEEP.
+SCHEME_KNOWN = {
+    2: [(0, 0, 0), (7, 0, 0), (12, 0, 0)],                                   # GENERATE
+    3: [(0, 0, 1), (7, 0, 1), (12, 0, 1)],                                   # FRONT_GENERATE
+    4: [(0, 0, 1), (1, 1, 4), (5, 1, 4), (6, 1, 2), (11, 1, 4), (12, 0, 1), (18, 1, 2)],
+    5: [(0, 0, 1), (1, 1, 4), (5, 1, 3), (6, 1, 2), (11, 1, 3), (12, 1, 2), (24, 1, 2), (29, 1, 3)],
+    6: [(0, 0, 1), (5, 1, 4), (6, 1, 2), (11, 1, 4), (12, 1, 2), (13, 1, 4)],
+    7: [(0, 0, 1), (1, 1, 4), (5, 1, 3), (6, 1, 2), (12, 1, 2), (17, 1, 3)],
+    8: [(0, 0, 1), (1, 1, 4), (6, 1, 1), (7, 1, 4), (12, 0, 1), (18, 1, 1), (24, 0, 1)],
+    0: [(0, 0, 1), (5, 1, 3), (6, 1, 2), (11, 1, 4), (12, 0, 1), (17, 1, 3), (23, 1, 4)],
+}
+
+
+@pytest.mark.parametrize("scheme", sorted(SCHEME_KNOWN))
+def test_scheme_tables(oracle, scheme):
+    """Every plain scheme: the oracle's restatement matches the known answers, and the
+    library's bvhcd_schedule_ops matches the oracle for 60 frames at two cycle settings."""
+    pkg = load_package()
+    for fr, b, f in SCHEME_KNOWN[scheme]:
+        assert oracle.scheme_ops(scheme, fr) == (b, f), (scheme, fr)
+    for bc, fc in ((12, 6), (8, 4)):
+        want = [oracle.scheme_ops(scheme, fr, bc, fc) for fr in range(60)]
+        got = [pkg.plain_schedule(fr, bc, fc, scheduler=scheme) for fr in range(60)]
+        assert got == want, (scheme, bc, fc)
+    # the learned schedulers start from the plain table
+    for sched in (pkg.SCHED_LEARNED, pkg.SCHED_LEARNED_NO_NET):
+        assert [pkg.plain_schedule(fr, scheduler=sched) for fr in range(30)] == \
+            [oracle.plain_schedule(fr) for fr in range(30)]
