@@ -870,6 +870,7 @@ void narrow(Ctx& c, int threads) {
 // BVH op then front op (LBvhFixedDeformable::maintain, BvttFrontLooseIntra::maintain).
 int step(Ctx& c, int bvh_op, int front_op, int threads) {
     if (c.T <= 0) return -1;
+    c.restr_subtrees = c.restr_leaves = 0;  // counts of this step's restructure only
     int tag = 0;
     bool topo_changed = false;
     if (bvh_op == 2) {  // restructure
@@ -1166,6 +1167,49 @@ void orc_stats(void* p, int64_t* out) {
     out[8] = c->restr_subtrees; out[9] = c->restr_leaves; out[10] = c->restr_pending ? 1 : 0;
 }
 // Restructure log of the last quality check: intrange (n entries, I+1 used) and rtroot (n).
+// The quality heuristic on caller-given preorder ranges and ordering offsets (known-answer
+// tests): do_snapshot = 1 writes frontSnapshot into snapshot[n-1]; then checkFrontQuality
+// against snapshot[] fills intrange[n] and rtroot[n].
+void orc_quality_eval(int64_t n, const int32_t* rx, const int32_t* ry, const uint32_t* intbeg,
+                      const uint32_t* extbeg, float* snapshot, int do_snapshot, int32_t* intrange,
+                      int32_t* rtroot) {
+    Ctx c;
+    c.T = n;
+    c.bvh.n = n;
+    c.bvh.rx.assign(rx, rx + (n - 1));
+    c.bvh.ry.assign(ry, ry + (n - 1));
+    c.log.setup(n);
+    std::memcpy(c.log.intbeg.data(), intbeg, 4 * n);
+    std::memcpy(c.log.extbeg.data(), extbeg, 4 * (n + 1));
+    if (do_snapshot) {
+        calc_snapshot(c);
+        std::memcpy(snapshot, c.snapshot.data(), 4 * (n - 1));
+    } else {
+        c.snapshot.assign(snapshot, snapshot + (n - 1));
+    }
+    check_quality(c);
+    std::memcpy(intrange, c.intrange.data(), 4 * n);
+    std::memcpy(rtroot, c.rtroot.data(), 4 * n);
+}
+
+// The restructure decision (LBvhFixedDeformable.cu:210-267) on caller-given marks: returns the
+// tag (0 refit, 1 restructure, 2 build), the outermost flagged roots and their internal nodes.
+int orc_restr_decision(int64_t n, const int32_t* rx, const int32_t* ry, const int32_t* intrange,
+                       const int32_t* rtroot, int32_t* roots, int64_t* nroots, int64_t* num_int) {
+    Ctx c;
+    c.T = n;
+    c.bvh.n = n;
+    c.bvh.rx.assign(rx, rx + (n - 1));
+    c.bvh.ry.assign(ry, ry + (n - 1));
+    c.intrange.assign(intrange, intrange + n);
+    c.rtroot.assign(rtroot, rtroot + n);
+    std::vector<int32_t> r;
+    const int tag = restructure_decision(c, r, *num_int);
+    *nroots = (int64_t)r.size();
+    if (roots) std::memcpy(roots, r.data(), 4 * r.size());
+    return tag;
+}
+
 void orc_export_restr_log(void* p, int32_t* intrange, int32_t* rtroot) {
     Ctx* c = static_cast<Ctx*>(p);
     if (intrange) std::memcpy(intrange, c->intrange.data(), 4 * c->intrange.size());

@@ -55,6 +55,8 @@ def lib():
         "orc_export_bvh": (None, [P] + [P] * 14),
         "orc_stats": (None, [P, P]),
         "orc_export_restr_log": (None, [P, P, P]),
+        "orc_quality_eval": (None, [I64, P, P, P, P, P, I32, P, P]),
+        "orc_restr_decision": (I32, [I64, P, P, P, P, P, C.POINTER(I64), C.POINTER(I64)]),
         "orc_restructure_roots": (I32, [P, P, I64, I32]),
     }
     for name, (res, args) in sig.items():
@@ -221,6 +223,35 @@ class OracleContext:
         keys = ["frame", "last_bvh_op", "last_front_op", "int_front", "ext_front", "bp_pairs", "np_pairs",
                 "node_visits", "restructured_subtrees", "restructured_leaves", "restr_pending"]
         return dict(zip(keys, (int(x) for x in s)))
+
+
+def quality_eval(rx, ry, intbeg, extbeg, snapshot=None):
+    """frontSnapshot (when snapshot is None) and checkFrontQuality on given preorder ranges and
+    ordering offsets: returns (snapshot, intrange, rtroot)."""
+    n = len(rx) + 1
+    rx = np.ascontiguousarray(rx, np.int32)
+    ry = np.ascontiguousarray(ry, np.int32)
+    ib = np.ascontiguousarray(intbeg, np.uint32)
+    eb = np.ascontiguousarray(extbeg, np.uint32)
+    assert ib.size == n and eb.size == n + 1
+    snap = np.zeros(n - 1, np.float32) if snapshot is None else np.array(snapshot, np.float32)
+    ir = np.zeros(n, np.int32)
+    rt = np.zeros(n, np.int32)
+    lib().orc_quality_eval(n, _p(rx), _p(ry), _p(ib), _p(eb), _p(snap), 1 if snapshot is None else 0, _p(ir), _p(rt))
+    return snap, ir, rt
+
+
+def restr_decision(rx, ry, intrange, rtroot):
+    """(tag, outermost roots, internal nodes in them) of the restructure decision."""
+    n = len(rx) + 1
+    rx = np.ascontiguousarray(rx, np.int32)
+    ry = np.ascontiguousarray(ry, np.int32)
+    ir = np.ascontiguousarray(intrange, np.int32)
+    rt = np.ascontiguousarray(rtroot, np.int32)
+    roots = np.zeros(n, np.int32)
+    k, m = C.c_int64(), C.c_int64()
+    tag = lib().orc_restr_decision(n, _p(rx), _p(ry), _p(ir), _p(rt), _p(roots), C.byref(k), C.byref(m))
+    return int(tag), roots[: k.value].copy(), int(m.value)
 
 
 def plain_schedule(frame, bvh_cycle=12, front_cycle=6):

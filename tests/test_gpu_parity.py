@@ -122,6 +122,61 @@ def test_scheme_parity(pkg, oracle, scheme):
     ctx.close()
 
 
+def _scrambled_frames(sc, frames, frac=0.5, per_frame=20, seed=0):
+    """A cloth frame whose vertices in one corner patch exchange positions, more of them every
+    frame: the tree built at the first frame degrades there, which makes the front-quality
+    heuristic flag subtrees (plain schedule with a long BVH cycle)."""
+    n = sc.spec.n
+    base = sc.frame(20)
+    V = sc.num_verts
+    x, y = np.arange(V) % n, np.arange(V) // n
+    m = np.nonzero((x < int(n * frac)) & (y < int(n * frac)))[0]
+    rng = np.random.default_rng(seed)
+    out = []
+    for fr in range(frames):
+        v = base.copy()
+        k = min(m.size, fr * per_frame)
+        p = m[:k]
+        v[p] = base[rng.permutation(p)]
+        out.append(v)
+    return out
+
+
+def test_quality_heuristic_flags_parity(pkg, oracle):
+    """The reference's restructure heuristic on a case where it fires: UPDATE frames flag
+    subtrees (checkFrontQuality), the next REFIT becomes a subtree restructure in the middle
+    tier (I/8 <= flagged internal nodes <= I/2) and the front is regenerated. Ops, restructure
+    counts, topology after each restructure, fronts and pair sets equal the oracle's every
+    frame."""
+    print("[gpu] quality heuristic flags", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=48, frames=40)
+    f = sc.faces()
+    frames = _scrambled_frames(sc, 36)
+    ctx = _ctx(pkg, sc, bvh_cycle=100)
+    orc = oracle.OracleContext(f, sc.num_verts, bvh_cycle=100)
+    restructured = 0
+    for fr, v in enumerate(frames):
+        ctx.update_vertices(v)
+        orc.update_vertices(v)
+        ctx.step_scheduled()
+        orc.step_scheduled()
+        gs, os_ = ctx.stats(), orc.stats()
+        assert (gs["last_bvh_op"], gs["last_front_op"]) == (os_["last_bvh_op"], os_["last_front_op"]), fr
+        assert (gs["restructured_subtrees"], gs["restructured_leaves"]) == \
+            (os_["restructured_subtrees"], os_["restructured_leaves"]), fr
+        if gs["last_bvh_op"] == pkg.BVH_RESTRUCTURE and gs["restructured_subtrees"] > 0:
+            restructured += 1
+            compare_bvh(ctx.export_bvh(), orc.export_bvh(), f"restructure frame {fr}")
+        for narrow in (False, True):
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(narrow)), oracle.pair_set(orc.pairs(narrow)),
+                                          err_msg=f"frame {fr} narrow {narrow}")
+        for w in (0, 1):
+            np.testing.assert_array_equal(_rows(ctx.export_front(w)), _rows(orc.front(w)),
+                                          err_msg=f"front {w} frame {fr}")
+    assert restructured >= 2
+    ctx.close()
+
+
 def test_explicit_ops_parity(pkg, oracle):
     print("[gpu] explicit ops", flush=True)
     sc = pkg.Scene.preset("cloth256", n=48, frames=25)

@@ -441,3 +441,164 @@ def test_scheme_tables(oracle, scheme):
     for sched in (pkg.SCHED_LEARNED, pkg.SCHED_LEARNED_NO_NET):
         assert [pkg.plain_schedule(fr, scheduler=sched) for fr in range(30)] == \
             [oracle.plain_schedule(fr) for fr in range(30)]
+
+
+# ---------------------------------------------------------------- restructure heuristic
+# frontSnapshot / checkFrontQuality (BvttFrontGenerateKernels.cu:26-49) and the decision tiers
+# (LBvhFixedDeformable.cu:210-267), pinned on small hand-built trees and fronts. Front counts
+# per node come from numpy bincount over front entries; the expected values below are computed
+# from preorder subtree ranges with numpy, independently of the oracle's loops.
+
+def _small_tree(oracle, n, seed):
+    rng = np.random.default_rng(seed)
+    codes = np.sort(rng.choice(1 << 20, size=n, replace=False)).astype(np.uint32)
+    return oracle.build_topology(codes)
+
+
+def _offsets(int_nodes, ext_leaves, n):
+    ci = np.bincount(int_nodes, minlength=n - 1).astype(np.int64)
+    ce = np.bincount(ext_leaves, minlength=n).astype(np.int64)
+    intbeg = np.concatenate([[0], np.cumsum(ci)])            # n entries: intbeg[n-1] = total
+    extbeg = np.concatenate([[0], np.cumsum(ce)])            # n + 1 entries
+    return ci, ce, intbeg.astype(np.uint32), extbeg.astype(np.uint32)
+
+
+def _subtree_counts(topo, ci, ce):
+    rx, ry = topo["int_rx"].astype(np.int64), topo["int_ry"].astype(np.int64)
+    I = rx.size
+    ic = np.array([ci[i:i + ry[i] - rx[i]].sum() for i in range(I)], np.int64)  # preorder subtree
+    ec = np.array([ce[rx[i]:ry[i] + 1].sum() for i in range(I)], np.int64)
+    return ic, ec
+
+
+def _expected_flags(ic, ec, snap, total, ext_plus_one=False, range_share=None):
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = ic / (ec + (1 if ext_plus_one else 0))
+    share = 0.7 * (range_share if range_share is not None else ic / total)
+    return ratio > (snap + np.float32(1)).astype(np.float64) - share
+
+
+def test_quality_snapshot_and_check_known_answers(oracle):
+    """Snapshot = intCnt / (extCnt + 1) as float; a node is flagged when, on the later front,
+    intCnt / extCnt > (snapshot + 1) - 0.7 intCnt / total internal entries. The marks are a
+    +1 / -1 difference pair over the node's preorder range and rtroot[rangex] = the smallest
+    flagged node id. Among the seeded cases, some are such that each plausible slip changes the
+    flags: extCnt + 1 in the check, extCnt without + 1 in the snapshot, the 0.7 term on the
+    range length (the reference's commented-out variant) instead of the front share."""
+    n = 40
+    topo = _small_tree(oracle, n, 3)
+    rx, ry = topo["int_rx"], topo["int_ry"]
+    I = n - 1
+    span = (ry - rx).astype(np.float64) / I
+    caught = {"ext_plus_one": False, "range_share": False, "snapshot_plus_one": False}
+    for seed in range(48):
+        rng = np.random.default_rng(seed)
+        # generate-time front, then a later one whose internal entries pile up on the left
+        ci0, ce0, ib0, eb0 = _offsets(rng.integers(0, I, 30), rng.integers(0, n, 60), n)
+        snap, _, _ = oracle.quality_eval(rx, ry, ib0, eb0)
+        ic0, ec0 = _subtree_counts(topo, ci0, ce0)
+        np.testing.assert_array_equal(snap, (ic0 / (ec0 + 1)).astype(np.float32))
+        later_int = np.concatenate([rng.integers(0, I // 3, int(rng.integers(20, 200))), rng.integers(0, I, 20)])
+        ci1, ce1, ib1, eb1 = _offsets(later_int, rng.integers(0, n, int(rng.integers(10, 80))), n)
+        _, ir, rt = oracle.quality_eval(rx, ry, ib1, eb1, snapshot=snap)
+        ic1, ec1 = _subtree_counts(topo, ci1, ce1)
+        total = int(ib1[-1])
+        want = _expected_flags(ic1, ec1, snap, total)
+        want_ir = np.zeros(n, np.int64)
+        want_rt = np.full(n, 2**31 - 1, np.int64)
+        for i in np.nonzero(want)[0]:
+            want_ir[i] += 1
+            want_ir[i + ry[i] - rx[i]] -= 1
+            want_rt[rx[i]] = min(want_rt[rx[i]], i)
+        np.testing.assert_array_equal(ir, want_ir, err_msg=str(seed))
+        np.testing.assert_array_equal(rt, want_rt, err_msg=str(seed))
+        caught["ext_plus_one"] |= not np.array_equal(want, _expected_flags(ic1, ec1, snap, total, ext_plus_one=True))
+        caught["range_share"] |= not np.array_equal(want, _expected_flags(ic1, ec1, snap, total, range_share=span))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            snap_slip = (ic0 / ec0).astype(np.float32)
+        caught["snapshot_plus_one"] |= not np.array_equal(want, _expected_flags(ic1, ec1, snap_slip, total))
+    assert all(caught.values()), caught
+
+
+def test_quality_previous_offsets_matter(oracle):
+    """checkFrontQuality reads the offsets of the previous ordering (it runs before the UPDATE's
+    re-ordering): with the counts of the new front instead the flags differ on this case."""
+    n = 32
+    topo = _small_tree(oracle, n, 5)
+    rx, ry = topo["int_rx"], topo["int_ry"]
+    rng = np.random.default_rng(2)
+    _, _, ib0, eb0 = _offsets(rng.integers(0, n - 1, 20), rng.integers(0, n, 40), n)
+    snap, _, _ = oracle.quality_eval(rx, ry, ib0, eb0)
+    _, _, ib1, eb1 = _offsets(rng.integers(0, 8, 90), rng.integers(0, n, 30), n)
+    _, _, ib2, eb2 = _offsets(rng.integers(0, n - 1, 25), rng.integers(0, n, 45), n)
+    _, ir1, _ = oracle.quality_eval(rx, ry, ib1, eb1, snapshot=snap)
+    _, ir2, _ = oracle.quality_eval(rx, ry, ib2, eb2, snapshot=snap)
+    assert not np.array_equal(ir1, ir2)
+
+
+def _outermost(topo, flagged):
+    par = topo["int_par"]
+    out = []
+    for r in sorted(flagged):
+        p, inner = par[r], False
+        while p >= 0:
+            if p in flagged:
+                inner = True
+                break
+            p = par[p]
+        if not inner:
+            out.append(r)
+    return out
+
+
+def _marks(topo, flagged, n):
+    rx, ry = topo["int_rx"], topo["int_ry"]
+    ir = np.zeros(n, np.int32)
+    rt = np.full(n, 2**31 - 1, np.int32)
+    for i in flagged:
+        ir[i] += 1
+        ir[i + ry[i] - rx[i]] -= 1
+        rt[rx[i]] = min(rt[rx[i]], i)
+    return ir, rt
+
+
+def test_restructure_decision_tiers(oracle):
+    """Outermost flagged subtrees (no flagged ancestor) and the three tiers on their internal
+    node count m: m > I/2 rebuilds (2), m < I/8 refits (0), else restructures (1); both
+    boundaries are strict (m = I >> 1 restructures, m = I >> 3 restructures)."""
+    n = 257
+    topo = _small_tree(oracle, n, 9)
+    rx, ry, par = topo["int_rx"], topo["int_ry"], topo["int_par"]
+    I = n - 1
+    size = (ry - rx).astype(np.int64)
+    rng = np.random.default_rng(4)
+    seen = set()
+    for trial in range(400):
+        k = int(rng.integers(1, 12))
+        flagged = set(int(x) for x in rng.choice(I, size=k, replace=False))
+        ir, rt = _marks(topo, flagged, n)
+        tag, roots, m = oracle.restr_decision(rx, ry, ir, rt)
+        outer = _outermost(topo, flagged)
+        assert sorted(roots.tolist()) == outer
+        assert m == int(size[outer].sum())
+        want = 2 if m > (I >> 1) else (0 if m < (I >> 3) else 1)
+        assert tag == want, (trial, m)
+        seen.add(tag)
+    assert seen == {0, 1, 2}
+    # exact boundaries: find nodes whose subtree holds exactly I >> 1 or I >> 3 internal nodes
+    # when combined; a single node of each size decides the boundary tier
+    for target, lo_tag in ((I >> 1, 1), (I >> 3, 1)):
+        # greedy disjoint set of subtrees summing to target
+        chosen, tot = [], 0
+        for r in np.argsort(-size):
+            r = int(r)
+            if tot + size[r] <= target and all(not (rx[r] <= rx[c] <= ry[r] or rx[c] <= rx[r] <= ry[c]) for c in chosen):
+                chosen.append(r)
+                tot += int(size[r])
+            if tot == target:
+                break
+        if tot != target:
+            continue
+        ir, rt = _marks(topo, set(chosen), n)
+        tag, _, m = oracle.restr_decision(rx, ry, ir, rt)
+        assert m == target and tag == lo_tag
