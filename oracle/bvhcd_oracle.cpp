@@ -1048,6 +1048,31 @@ void orc_refit_boxes(int64_t n, const int32_t* int_lc, const int32_t* int_rc, co
 int orc_sat(const float* P9, const float* Q9) { return sat_tt<false>(P9, Q9) ? 1 : 0; }
 int orc_sat_fma(const float* P9, const float* Q9) { return sat_tt<true>(P9, Q9) ? 1 : 0; }
 
+// Narrow-phase decision of each pair (a, b) under the four evaluation variants that the
+// reference's build could have used: bit 0 = lower triangle id as P (this build's canonical
+// order), bit 1 = higher id as P (the reference passes the lower sorted leaf's triangle first,
+// narrow_phase.cu:11-19, which is either one), bits 2 / 3 = the same two orders with the
+// products fused into FMAs (nvcc's default contraction of cuda_intersect_tritri.h:247-266,
+// 349-433). A mask of 0 or 15 means the four agree.
+void orc_sat_variants(int64_t n, const int32_t* pairs, const int32_t* faces, const float* verts, int threads,
+                      uint8_t* out) {
+    parallel_for(n, threads, [&](int64_t a, int64_t e, int) {
+        for (int64_t k = a; k < e; ++k) {
+            int32_t x = pairs[2 * k], y = pairs[2 * k + 1];
+            if (x > y) std::swap(x, y);
+            float P[9], Q[9];
+            gather_tri(faces, verts, x, P);
+            gather_tri(faces, verts, y, Q);
+            uint8_t m = 0;
+            m |= sat_tt<false>(P, Q) ? 1 : 0;
+            m |= sat_tt<false>(Q, P) ? 2 : 0;
+            m |= sat_tt<true>(P, Q) ? 4 : 0;
+            m |= sat_tt<true>(Q, P) ? 8 : 0;
+            out[k] = m;
+        }
+    });
+}
+
 // Brute-force all-pairs over unordered {a, b}, a != b: AABB overlap and (narrow=1) SAT hit.
 int64_t orc_bruteforce_pairs(int64_t T, const int32_t* faces, const float* verts, int narrow_on, int32_t* out,
                              int64_t cap, int threads) {

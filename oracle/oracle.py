@@ -41,6 +41,7 @@ def lib():
         "orc_refit_boxes": (None, [I64, P, P, P, P, P]),
         "orc_sat": (I32, [P, P]),
         "orc_sat_fma": (I32, [P, P]),
+        "orc_sat_variants": (None, [I64, P, P, P, I32, P]),
         "orc_bruteforce_pairs": (I64, [I64, P, P, I32, P, I64, I32]),
         "orc_create": (P, [I64, I64, P, I32, I32, I32, I32]),
         "orc_destroy": (None, [P]),
@@ -136,6 +137,18 @@ def sat(P, Q, fma=False):
     p = np.ascontiguousarray(P, np.float32).ravel()
     q = np.ascontiguousarray(Q, np.float32).ravel()
     return bool((lib().orc_sat_fma if fma else lib().orc_sat)(_p(p), _p(q)))
+
+
+def sat_variants(pairs, faces, verts, threads=0):
+    """Per pair, the decision bits of the four narrow-phase variants (lower / higher triangle id
+    as P, without / with fused multiply-adds); 0 or 15 = all agree."""
+    p = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    f = np.ascontiguousarray(faces, np.int32)
+    v = np.ascontiguousarray(verts, np.float32)
+    out = np.zeros(p.shape[0], np.uint8)
+    if p.shape[0]:
+        lib().orc_sat_variants(p.shape[0], _p(p), _p(f), _p(v), threads, _p(out))
+    return out
 
 
 def bruteforce_pairs(faces, verts, narrow=True, threads=0):
