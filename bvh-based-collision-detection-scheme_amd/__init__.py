@@ -130,6 +130,14 @@ def lib():
         "bvhcd_update_boxes": (C.c_int, [P, P, I64, C.c_int]),
         "bvhcd_mlp_set_repair_cost": (C.c_int, [P, C.c_float]),
         "bvhcd_schedule_ops": (C.c_int, [C.c_int, I64, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "bvhcd_log_text": (C.c_int, [P, C.c_char_p, I64, C.POINTER(I64)]),
+        "bvhcd_mesh_load": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+        "bvhcd_mesh_destroy": (None, [P]),
+        "bvhcd_mesh_num_verts": (I64, [P]),
+        "bvhcd_mesh_num_tris": (I64, [P]),
+        "bvhcd_mesh_faces": (C.c_int, [P, P]),
+        "bvhcd_mesh_verts": (C.c_int, [P, P]),
+        "bvhcd_frame_path": (C.c_int, [C.c_char_p, I32, C.c_char_p, I64]),
         "bvhcd_scene_create": (C.c_int, [C.c_int, C.POINTER(SceneParams), C.POINTER(P)]),
         "bvhcd_scene_destroy": (None, [P]),
         "bvhcd_scene_num_tris": (I64, [P]),
@@ -354,6 +362,15 @@ class CollisionContext:
                                             1 if filtered else 0), "get_pairs")
         return out
 
+    def log_text(self) -> str:
+        """The frames recorded since set_timing in the reference's log file layout
+        (bvhcd_log_text)."""
+        n = C.c_int64()
+        self._chk(lib().bvhcd_log_text(self._h, None, 0, C.byref(n)), "log_text")
+        buf = C.create_string_buffer(n.value + 1)
+        self._chk(lib().bvhcd_log_text(self._h, buf, n.value + 1, C.byref(n)), "log_text")
+        return buf.value.decode()
+
     def pairs_to_device(self, ptr, cap, filtered=True) -> int:
         """Writes the pairs (int32 pairs of triangle ids) into the device buffer at `ptr` (room for
         `cap` pairs), stream-ordered; returns the total pair count (may exceed cap)."""
@@ -563,3 +580,86 @@ def plain_schedule(frame, bvh_cycle=12, front_cycle=6, scheduler=None):
     if st != OK:
         raise BvhcdError(st, "schedule_ops")
     return b.value, f.value
+
+
+# ----------------------------------------------------------------------------- mesh files, logs
+def load_mesh(path):
+    """(faces int32[T, 3], verts float32[V, 3]) of an OBJ or PLY file (bvhcd_mesh_load): every
+    mesh of the file concatenated with vertex-id offsets, as the reference's Scene::addMesh
+    (world/Scene.cpp:29-42)."""
+    L = lib()
+    h = C.c_void_p()
+    st = L.bvhcd_mesh_load(os.fsencode(path), C.byref(h))
+    if st != OK:
+        raise BvhcdError(st, f"mesh_load({path})")
+    try:
+        f = np.empty((L.bvhcd_mesh_num_tris(h), 3), np.int32)
+        v = np.empty((L.bvhcd_mesh_num_verts(h), 3), np.float32)
+        L.bvhcd_mesh_faces(h, _ptr(f))
+        L.bvhcd_mesh_verts(h, _ptr(v))
+    finally:
+        L.bvhcd_mesh_destroy(h)
+    return f, v
+
+
+def frame_path(pattern, index):
+    """Path of one frame of a sequence named by a printf pattern (bvhcd_frame_path)."""
+    buf = C.create_string_buffer(4096)
+    st = lib().bvhcd_frame_path(os.fsencode(pattern), int(index), buf, 4096)
+    if st != OK:
+        raise BvhcdError(st, f"frame_path({pattern!r}, {index})")
+    return os.fsdecode(buf.value)
+
+
+class MeshSequence:
+    """Frames first .. first+count-1 of a mesh-file sequence (the reference's Benchmark
+    {stIdx, len, inputFileFormat}, setting/CDBenchmarkSettings.cpp:17-60). The topology is the
+    first frame's; every later frame must have the same vertex count."""
+
+    def __init__(self, pattern, first, count):
+        self.pattern, self.first, self.frames = pattern, int(first), int(count)
+        self._faces, v0 = load_mesh(frame_path(pattern, self.first))
+        self.num_verts, self.num_tris = v0.shape[0], self._faces.shape[0]
+
+    def faces(self):
+        return self._faces.copy()
+
+    def frame(self, f):
+        _, v = load_mesh(frame_path(self.pattern, self.first + int(f)))
+        if v.shape[0] != self.num_verts:
+            raise ValueError(f"frame {f}: {v.shape[0]} vertices, the sequence's topology has {self.num_verts}")
+        return v
+
+
+def write_log(path, text):
+    """Writes a log text to `path` as the reference's Logger::record does
+    (system/Log/Logger.hpp:68-94): the parent directory is created, and an existing file is
+    never overwritten (stem + "_" is tried until the name is free). Returns the path used."""
+    parent = os.path.dirname(path)
+    if parent and not os.path.exists(parent):
+        os.makedirs(parent)
+    stem, ext = os.path.splitext(os.path.basename(path))
+    while os.path.exists(path):
+        stem += "_"
+        path = os.path.join(parent, stem + ext)
+    with open(path, "w") as fh:
+        fh.write(text)
+    return path
+
+
+def front_status(text, keyword="front"):
+    """Rows (int front, ext front, cpNum, actualCpNum) of every "front (a, b) cpNum c
+    actualCpNum d" line of a log text, as the reference's FrontAnalyser data extractor reads them
+    (Tools/FrontAnalyser/DataExtractor/Source.cpp:15-32)."""
+    rows = []
+    toks = text.split()
+    k = 0
+    while k < len(toks):
+        if toks[k] == keyword and k + 6 < len(toks):
+            a = toks[k + 1].strip("(,")
+            b = toks[k + 2].strip(")")
+            rows.append((int(a), int(b), int(toks[k + 4]), int(toks[k + 6])))
+            k += 7
+        else:
+            k += 1
+    return rows

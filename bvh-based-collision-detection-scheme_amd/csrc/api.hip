@@ -933,6 +933,57 @@ int bvhcd_timing_counts(bvhcd_ctx* c, int64_t cap, int64_t* counts, int64_t* n) 
     return BVHCD_OK;
 }
 
+int bvhcd_log_text(bvhcd_ctx* c, char* buf, int64_t cap, int64_t* len) {
+    if (!c || !len) return BVHCD_E_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<uint64_t> log((size_t)kLog * c->tn);
+    if (c->tn > 0)
+        HIPCHK(c, hipMemcpy(log.data(), c->dlog, sizeof(uint64_t) * log.size(), hipMemcpyDeviceToHost));
+    static const char* kFrontMsg[] = {"pure bvhcd. ", "generate front. ", "update front. ", "balance front. ",
+                                      "preserve front. ", "restructure front(in restr) "};
+    static const char* kFrontSection[] = {"broad_phase_cd_bvh", "broad_phase_cd_gen", "broad_phase_cd_update",
+                                          "broad_phase_cd_balance", "broad_phase_cd_preserve",
+                                          "broad_phase_restr_front"};
+    std::string msgs, gpu;
+    char line[256];
+    for (int64_t f = 0; f < c->tn; ++f) {
+        const uint64_t* L = &log[kLog * f];
+        const int bo = c->tops[2 * f], fo = c->tops[2 * f + 1];
+        if (bo == BVHCD_BVH_RESTRUCTURE && c->trestr[f] > 0) {
+            snprintf(line, sizeof(line), "%lld leaves in restructured subtrees. restructure\n", (long long)c->trestr[f]);
+            msgs += line;
+        }
+        if (fo >= 0 && fo <= BVHCD_FRONT_RESTRUCTURE) msgs += std::string(kFrontMsg[fo]) + "\n";
+        snprintf(line, sizeof(line), "front (%llu, %llu) cpNum %llu actualCpNum %llu\n", (unsigned long long)L[3],
+                 (unsigned long long)L[4], (unsigned long long)L[1], (unsigned long long)L[2]);
+        msgs += line;
+        float total = 0;
+        for (int st = 0; st < 7; ++st) {
+            const int64_t k = f * kEv + 2 * st;
+            float t = 0;
+            if (!(c->tused[k] && c->tused[k + 1])) continue;
+            (void)hipEventElapsedTime(&t, c->tev[k], c->tev[k + 1]);
+            const char* name = st == S_BUILD ? "construct_bvh" : st == S_REFIT ? "refit_bvh" : st == S_RESTR ? "restr_bvh"
+                             : st == S_FRONT ? (fo >= 0 && fo <= 5 ? kFrontSection[fo] : "broad_phase")
+                             : st == S_ORDER ? "broad_phase_front_ordering" : st == S_NARROW ? "narrow_phase"
+                             : "restructure_schedule";
+            snprintf(line, sizeof(line), "@%s: %g\n\n", name, (double)t);
+            gpu += line;
+            total += t;
+        }
+        snprintf(line, sizeof(line), "#cd_total_time: %g\n###: 0\n", (double)total);
+        gpu += line;
+    }
+    const std::string text = "\n" + msgs + "\n" + gpu + "\n";
+    *len = (int64_t)text.size();
+    if (buf && cap > 0) {
+        const int64_t k = std::min<int64_t>(cap - 1, (int64_t)text.size());
+        std::memcpy(buf, text.data(), (size_t)k);
+        buf[k] = 0;
+    }
+    return BVHCD_OK;
+}
+
 int bvhcd_timing_frames(bvhcd_ctx* c, int64_t cap, double* ms, double* bytes, int32_t* ops, int64_t* n) {
     if (!c || !n) return BVHCD_E_INVALID;
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -412,33 +412,40 @@ __global__ void k_treelet_compact(int32_t I, const uint32_t* __restrict__ flags,
     if (i < I && flags[i]) roots[pos[i]] = i;
 }
 
-// Adds, for every distinct key among the lanes with pred set, the number of such lanes to
-// cnt[key] with one atomic (neighbouring front entries share nodes, so keys repeat within a
-// wave and per-lane atomics would serialise on the same address).
-__device__ inline void wave_count(uint32_t* __restrict__ cnt, int32_t key, bool pred) {
-    uint64_t todo = __ballot(pred);
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int32_t k = __shfl(key, leader);
-        const uint64_t same = __ballot(pred && key == k);
-        if (lane_id() == leader) atomicAdd(&cnt[k], (uint32_t)__popcll(same));
-        todo &= ~same;
+// Adds, for every run of equal keys over consecutive lanes with pred set, the run's length to
+// cnt[key] with one atomic (node-sorted fronts repeat a key over neighbouring entries; on
+// query-sorted ones keys rarely repeat and each lane adds its own). Called by all 64 lanes.
+__device__ inline void run_count(uint32_t* __restrict__ cnt, int32_t key, bool pred) {
+    const int lane = lane_id();
+    const int32_t prev = __shfl(key, lane > 0 ? lane - 1 : 0);
+    const uint64_t on = __ballot(pred);
+    const bool head = pred && (lane == 0 || !((on >> (lane - 1)) & 1ull) || prev != key);
+    const uint64_t stops = __ballot(head) | ~on;  // a run ends before the next head or inactive lane
+    if (head) {
+        const uint64_t after = lane == 63 ? 0ull : stops & (~0ull << (lane + 1));
+        const int end = after ? __ffsll((long long)after) - 1 : 64;
+        atomicAdd(&cnt[key], (uint32_t)(end - lane));
     }
 }
 
 // Front entries per leaf: every entry counts for its node's leftmost leaf (acnt), external
-// entries also for their leaf (lcnt); recomputed only when the front changed.
+// entries also for their leaf (lcnt); recomputed only when the front changed. The loop runs
+// per wave (all lanes together) so that run_count sees whole waves.
 __global__ void __launch_bounds__(kBlock) k_count_front(DevState d, uint32_t* __restrict__ acnt,
                                                         uint32_t* __restrict__ lcnt) {
     const int c0 = d.fcur[0], c1 = d.fcur[1];
     const uint32_t ni = (uint32_t)min((int64_t)d.fsize[c0], d.front_cap[0]);
     const uint32_t ne = (uint32_t)min((int64_t)d.fsize[2 + c1], d.front_cap[1]);
     const uint32_t total = ni + ne;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
-        const bool ext = k >= ni;
-        const int32_t leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
-        wave_count(acnt, leaf, true);
-        wave_count(lcnt, leaf, ext);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < total; base += stride) {
+        const uint32_t k = base + (uint32_t)lane_id();
+        const bool valid = k < total;
+        const bool ext = valid && k >= ni;
+        int32_t leaf = -1;
+        if (valid) leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
+        run_count(acnt, leaf, valid);
+        run_count(lcnt, leaf, ext);
     }
 }
 

@@ -192,6 +192,16 @@ int  bvhcd_timing_frames(bvhcd_ctx* ctx, int64_t cap, double* ms, double* bytes,
    Logger record of front sizes and pair counts, BvttFrontLooseIntra.cu:50-61). */
 int  bvhcd_timing_counts(bvhcd_ctx* ctx, int64_t cap, int64_t* counts, int64_t* n);
 
+/* The recorded frames as the reference's log file text (Logger::record, system/Log/Logger.hpp:68-94):
+   an empty CPU-timer block, then per frame the operation messages and the inspectResults line
+   "front (int, ext) cpNum bp actualCpNum np" (BvttFrontLooseIntra.cu:50-61), then per frame one
+   "@section: ms" record and a blank line per timed stage (CudaTimer::recordSection,
+   utility/CudaTimer.hpp:43-53; sections construct_bvh, refit_bvh, restr_bvh, broad_phase_cd_bvh /
+   _gen / _update / _balance / _preserve, broad_phase_restr_front, broad_phase_front_ordering,
+   narrow_phase) and "#cd_total_time: ms", "###: 0" (blankLine, :54-64). len = full text length;
+   at most cap - 1 bytes are copied, NUL-terminated. Synchronises the stream. */
+int  bvhcd_log_text(bvhcd_ctx* ctx, char* buf, int64_t cap, int64_t* len);
+
 /* Export the current BVH to host arrays (any pointer may be NULL). n = leaves = num_tris.
    codes/prim_idx/prim_map: n; int_*: n-1 (preorder); leaf_par/leaf_mark: n; leaf_lca: n+1;
    int_box/leaf_box: 6 floats per node (minx, miny, minz, maxx, maxy, maxz). */
@@ -315,6 +325,23 @@ int64_t bvhcd_scene_num_verts(const bvhcd_scene* s);
 int     bvhcd_scene_faces(const bvhcd_scene* s, int32_t* faces);
 /* Vertices of frame f (0 <= f < frames) into xyz (V x 3 floats); threads <= 0 = all cores. */
 int     bvhcd_scene_frame(const bvhcd_scene* s, int32_t frame, float* xyz, int32_t threads);
+
+/* ---- mesh files (host) ----
+   One mesh file per frame, as the reference's Scene::update / addMesh read them through assimp
+   (world/Scene.cpp:14-42): every mesh of the file concatenated with vertex-id offsets. Formats:
+   Wavefront OBJ (v / f lines, 1-based or negative indices, polygons split into fans) and PLY
+   (ascii, binary little / big endian; vertex x y z of any scalar type, face vertex_indices
+   list). Coordinates are stored as float. */
+typedef struct bvhcd_mesh bvhcd_mesh;
+int     bvhcd_mesh_load(const char* path, bvhcd_mesh** out);
+void    bvhcd_mesh_destroy(bvhcd_mesh* m);
+int64_t bvhcd_mesh_num_verts(const bvhcd_mesh* m);
+int64_t bvhcd_mesh_num_tris(const bvhcd_mesh* m);
+int     bvhcd_mesh_faces(const bvhcd_mesh* m, int32_t* faces);   /* num_tris x 3 */
+int     bvhcd_mesh_verts(const bvhcd_mesh* m, float* xyz);       /* num_verts x 3 */
+/* Path of frame `index` of a sequence named by a printf pattern with one integer conversion
+   (Benchmark's inputFileFormat, setting/CDBenchmarkSettings.cpp:17-60, e.g. "funnel/%03d.ply"). */
+int     bvhcd_frame_path(const char* pattern, int32_t index, char* out, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -92,3 +92,42 @@ def test_batch_host_array_reused_after_upload(pkg, oracle):
             want = oracle.pair_set(oracle.bruteforce_pairs(faces[k], frames[k], True))
             np.testing.assert_array_equal(oracle.pair_set(b.pairs(k)), want, err_msg=f"frame {f} scene {k}")
     b.close()
+
+
+def test_mesh_sequence_and_log_text(pkg, oracle, tmp_path):
+    """Frames read from PLY files (one file per frame, printf-named) give the same pairs as the
+    generator's frames, and the log text carries one inspectResults line per recorded frame
+    with the step's front sizes and pair counts, plus its timed sections."""
+    import struct
+    print("[gpu] mesh sequence + log text", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=40, frames=14)
+    f = sc.faces()
+    for k in range(sc.frames):
+        v = sc.frame(k)
+        head = ["ply", "format binary_little_endian 1.0", f"element vertex {len(v)}", "property float x",
+                "property float y", "property float z", f"element face {len(f)}",
+                "property list uchar int vertex_indices", "end_header"]
+        with open(tmp_path / ("cloth%d.ply" % (k + 1)), "wb") as fh:
+            fh.write(("\n".join(head) + "\n").encode())
+            fh.write(v.astype("<f4").tobytes())
+            for t in f:
+                fh.write(struct.pack("<Biii", 3, *(int(i) for i in t)))
+    seq = pkg.MeshSequence(str(tmp_path / "cloth%d.ply"), 1, sc.frames)
+    ctx = pkg.CollisionContext(seq.num_tris, seq.num_verts)
+    ctx.set_topology(seq.faces(), seq.num_verts)
+    ctx.set_timing(sc.frames)
+    rows = []
+    for k in range(seq.frames):
+        v = seq.frame(k)
+        ctx.update_vertices(v)
+        ctx.step_scheduled()
+        st = ctx.stats()
+        rows.append((st["int_front"], st["ext_front"], st["bp_pairs"], st["np_pairs"]))
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)),
+                                      oracle.pair_set(oracle.bruteforce_pairs(f, sc.frame(k), True)), err_msg=str(k))
+    text = ctx.log_text()
+    assert pkg.front_status(text) == rows
+    assert text.count("#cd_total_time") == sc.frames and text.count("###: 0") == sc.frames
+    assert text.count("@refit_bvh") >= 1 and text.count("@narrow_phase") == sc.frames
+    assert "preserve front. " in text and "generate front. " in text
+    ctx.close()
