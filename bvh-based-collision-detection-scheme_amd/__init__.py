@@ -131,6 +131,16 @@ def lib():
         "bvhcd_mlp_set_repair_cost": (C.c_int, [P, C.c_float]),
         "bvhcd_schedule_ops": (C.c_int, [C.c_int, I64, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "bvhcd_log_text": (C.c_int, [P, C.c_char_p, I64, C.POINTER(I64)]),
+        "bvhcd_inter_create": (C.c_int, [P, C.POINTER(Config), C.POINTER(P)]),
+        "bvhcd_inter_destroy": (C.c_int, [P]),
+        "bvhcd_inter_last_error": (C.c_char_p, [P]),
+        "bvhcd_inter_set_topology": (C.c_int, [P, P, I64, I64, C.c_int]),
+        "bvhcd_inter_update_vertices": (C.c_int, [P, P, I64, C.c_int]),
+        "bvhcd_inter_step": (C.c_int, [P, C.c_int, C.c_int]),
+        "bvhcd_inter_get_pairs": (C.c_int, [P, P, I64, C.POINTER(I64), C.c_int, C.c_int]),
+        "bvhcd_inter_export_front": (C.c_int, [P, C.c_int, P, I64, C.POINTER(I64)]),
+        "bvhcd_inter_stats": (C.c_int, [P, P]),
+        "bvhcd_inter_export_rigid": (C.c_int, [P, P, P]),
         "bvhcd_mesh_load": (C.c_int, [C.c_char_p, C.POINTER(P)]),
         "bvhcd_mesh_destroy": (None, [P]),
         "bvhcd_mesh_num_verts": (I64, [P]),
@@ -580,6 +590,89 @@ def plain_schedule(frame, bvh_cycle=12, front_cycle=6, scheduler=None):
     if st != OK:
         raise BvhcdError(st, "schedule_ops")
     return b.value, f.value
+
+
+# ----------------------------------------------------------------------------- inter-object front
+class InterFront:
+    """A second object's triangles against a CollisionContext's tree (bvhcd_inter_*): the
+    reference's BvttFrontLooseInter with its rigid LBVH. Step the queried context first each
+    frame; pairs are (queried context's triangle, this object's triangle)."""
+
+    def __init__(self, queried: "CollisionContext", max_tris, max_verts, *, int_front_cap=0, ext_front_cap=0,
+                 pair_cap=0, narrow_phase=True):
+        cfg = Config(int(max_tris), int(max_verts), int(int_front_cap), int(ext_front_cap), int(pair_cap),
+                     1 if narrow_phase else 0, SCHED_PLAIN, 0, 12, 6, 0, None, 0)
+        h = C.c_void_p()
+        st = lib().bvhcd_inter_create(queried._h, C.byref(cfg), C.byref(h))
+        if st != OK:
+            raise BvhcdError(st, "bvhcd_inter_create failed")
+        self._h = h
+        self._queried = queried  # the queried context must outlive this object
+        self.T = 0
+
+    def _chk(self, st, what):
+        if st != OK:
+            msg = lib().bvhcd_inter_last_error(self._h)
+            raise BvhcdError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bvhcd_inter_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_topology(self, faces, num_verts):
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+        self._chk(lib().bvhcd_inter_set_topology(self._h, _ptr(f), f.shape[0], int(num_verts), MEM_HOST),
+                  "inter_set_topology")
+        self.T = f.shape[0]
+
+    def update_vertices(self, xyz, where=MEM_HOST):
+        if where == MEM_HOST:
+            v = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+            self._chk(lib().bvhcd_inter_update_vertices(self._h, _ptr(v), v.shape[0], MEM_HOST), "inter_update_vertices")
+        else:
+            ptr, V = xyz
+            self._chk(lib().bvhcd_inter_update_vertices(self._h, C.c_void_p(int(ptr)), int(V), int(where)),
+                      "inter_update_vertices")
+
+    def step(self, rigid_op, front_op):
+        self._chk(lib().bvhcd_inter_step(self._h, int(rigid_op), int(front_op)), "inter_step")
+
+    def pairs(self, filtered=True) -> np.ndarray:
+        n = C.c_int64()
+        self._chk(lib().bvhcd_inter_get_pairs(self._h, None, 0, C.byref(n), MEM_HOST, 1 if filtered else 0),
+                  "inter_get_pairs")
+        out = np.empty((n.value, 2), dtype=np.int32)
+        if n.value:
+            self._chk(lib().bvhcd_inter_get_pairs(self._h, _ptr(out), n.value, C.byref(n), MEM_HOST,
+                                                  1 if filtered else 0), "inter_get_pairs")
+        return out
+
+    def export_front(self, which) -> np.ndarray:
+        n = C.c_int64()
+        self._chk(lib().bvhcd_inter_export_front(self._h, int(which), None, 0, C.byref(n)), "inter_export_front")
+        out = np.empty((n.value, 2), dtype=np.int32)
+        if n.value:
+            self._chk(lib().bvhcd_inter_export_front(self._h, int(which), _ptr(out), n.value, C.byref(n)),
+                      "inter_export_front")
+        return out
+
+    def stats(self) -> dict:
+        out = np.zeros(6, np.int64)
+        self._chk(lib().bvhcd_inter_stats(self._h, _ptr(out)), "inter_stats")
+        keys = ["int_front", "ext_front", "bp_pairs", "np_pairs", "node_visits", "last_front_op"]
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def export_rigid(self):
+        codes, idx = np.empty(self.T, np.uint32), np.empty(self.T, np.int32)
+        self._chk(lib().bvhcd_inter_export_rigid(self._h, _ptr(codes), _ptr(idx)), "inter_export_rigid")
+        return codes, idx
 
 
 # ----------------------------------------------------------------------------- mesh files, logs

@@ -24,9 +24,14 @@ namespace {
 enum Stage { S_BUILD = 0, S_REFIT = 1, S_RESTR = 2, S_FRONT = 3, S_ORDER = 4, S_NARROW = 5, S_SCHED = 6, S_TOTAL = 7 };
 constexpr int kEv = 16;   // begin/end event per stage
 constexpr int kLog = 8;   // device log words per frame
-// Learned schedule: a KEEP frame whose node tests exceed this multiple of the front size
-// (each entry is one test unless its internal node overlaps and is descended) asks for UPDATE.
-constexpr double kLearnedUpdateRatio = 1.25;
+// Learned schedule: the tree and front are rebuilt (BUILD + GENERATE) once the KEEP node tests
+// accumulated above those of the first KEEP after the last rebuild reach this many of those
+// first KEEPs: a rebuild with its front generation costs about as much as eight KEEP frames of
+// a fresh front (measured on the 2M-triangle folding cloth: 6.2 ms against 0.7 ms), so the tree
+// is rebuilt when keeping the degraded one has cost as much as rebuilding it would.
+constexpr double kLearnedRebuildCost = 8.0;
+// A front this close to its capacity is replaced by a fresh one at the next check.
+constexpr double kLearnedCapacityShare = 0.75;
 constexpr double kLearnedNodeOrderRatio = 1.05;
 constexpr int kLearnedMaxBackoff = 3;
 }  // namespace
@@ -63,6 +68,8 @@ struct bvhcd_ctx {
     hipStream_t copy_stream = nullptr;  // host uploads, overlapping the steps still running
     hipEvent_t copied[2] = {nullptr, nullptr};
     hipEvent_t consumed[2] = {nullptr, nullptr};  // the last step that read each buffer is done
+    double keep_base = -1.0;     // learned schedule: node tests of the first KEEP after the last rebuild
+    double keep_excess = 0.0;    // node tests above keep_base accumulated since that rebuild
     bool totals_pending = false; // pair totals of the last step not summed yet (done on demand)
     bool step_zeroed = false;    // this step's refit zeroed the step counters for its KEEP
 };
@@ -229,6 +236,8 @@ int allocate(bvhcd_ctx* c) {
     HIPCHK(c, hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(d.pcnt, 0, 2 * kPairSegs * kCntStride * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(d.fsize, 0, 4 * sizeof(uint32_t), c->stream));
+    d.qleaf = d.leaf;  // self-CD: the tree's own leaves are the queries
+    d.qprim = d.prim_idx;
     return BVHCD_OK;
 }
 
@@ -747,6 +756,8 @@ int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V,
     c->T = T;
     c->V = V;
     c->built = c->front_valid = c->restr_pending = c->learned_frame = c->repair_pending = false;
+    c->keep_base = -1.0;
+    c->keep_excess = 0.0;
     c->d.front_version++;
     c->frame = 0;
     c->d.fcur[0] = c->d.fcur[1] = 0;
@@ -823,11 +834,13 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
     if (c->built && (net || sched == BVHCD_SCHED_LEARNED_NO_NET)) {
         // Learned schedule: the periodic global maintenance of the plain schedule (rebuild +
         // regenerate every bvh_cycle frames, UPDATE / REORDER every front_cycle) is replaced by
-        // per-subtree decisions. Every frame refits and keeps the front; every bvh_cycle frames
-        // (and on the heuristic's restructure flag) the MLP scores the candidate subtrees, and
-        // the chosen ones are rebuilt with an incremental front restructure. At front-cycle
-        // points a front whose KEEP work has outgrown its size (descents below internal entries
-        // that now overlap) gets an UPDATE. The pair set is the same for every schedule.
+        // decisions. Every frame refits and keeps the front; every bvh_cycle frames (and on the
+        // heuristic's restructure flag) the MLP scores the candidate subtrees, and the chosen
+        // ones are rebuilt with an incremental front restructure. At front-cycle points the
+        // KEEP work (node tests) is compared with that of the first KEEP after the last
+        // rebuild: once the excess accumulated since then reaches kLearnedRebuildCost such
+        // KEEPs (or a front nears its capacity), the whole tree is rebuilt and its front
+        // generated anew. The pair set is the same for every schedule.
         bvh_op = BVHCD_BVH_REFIT;
         front_op = BVHCD_FRONT_KEEP;
         const int bc = c->cfg.bvh_cycle > 0 ? c->cfg.bvh_cycle : 12;
@@ -844,14 +857,25 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
             HIPCHK(c, hipMemcpyAsync(&implicit, c->d.ctr + C_IMPLICIT, sizeof(implicit), hipMemcpyDeviceToHost,
                                      c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
+            const double fi = (double)sz[c->d.fcur[0]], fe = (double)sz[2 + c->d.fcur[1]];
             // entries KEEP tested (the implicit shared-vertex head takes no node test)
-            const double entries = (double)sz[c->d.fcur[0]] + (double)sz[2 + c->d.fcur[1]] - (double)implicit;
-            if ((double)visits > kLearnedUpdateRatio * entries + 1024.0) front_op = BVHCD_FRONT_UPDATE;
+            const double entries = fi + fe - (double)implicit;
+            const double work = std::max((double)visits, 1.0);
+            if (c->keep_base < 0) c->keep_base = work;
+            c->keep_excess += std::max(0.0, work - c->keep_base) * fc;
+            const bool full = fi > kLearnedCapacityShare * (double)c->d.front_cap[0] ||
+                              fe > kLearnedCapacityShare * (double)c->d.front_cap[1];
+            if (full || c->keep_excess >= kLearnedRebuildCost * c->keep_base) {
+                bvh_op = BVHCD_BVH_BUILD;
+                front_op = BVHCD_FRONT_GENERATE;
+                c->learned_frame = false;  // the rebuild replaces this frame's subtree decision
+            }
             // KEEP's descents below now-overlapping internal entries are cheaper on node-ordered
             // fronts (lanes descend the same subtree); without them, query order keeps each
             // wave's records local. The next produced front takes the order that fits.
             c->learned_node_order = (double)visits > kLearnedNodeOrderRatio * entries + 1024.0;
         }
+        if (bvh_op == BVHCD_BVH_BUILD) c->keep_base = -1.0, c->keep_excess = 0.0;
         return do_step(c, bvh_op, front_op);
     }
     if (bvh_op == BVHCD_BVH_REFIT && c->cfg.enable_restructure && c->restr_pending) bvh_op = BVHCD_BVH_RESTRUCTURE;
@@ -1276,6 +1300,318 @@ int bvhcd_restructure(bvhcd_ctx* c, const int32_t* roots, int64_t k) {
     if (st) return st;
     c->restr_pending = false;
     return check_overflow(c);
+}
+
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- inter-object front
+// BvttFrontLooseInter (collision/bvtt_front/BvttFrontLooseInter.cu:16-260) with LBvhRigid
+// (collision/lbvh/LBvhRigid.cu): a second object's triangles queried against a context's tree.
+// The second object ("rigid": its topology is fixed, BUILD orders its triangles, REFIT updates
+// their boxes) is held in a context of its own that shares the first one's stream; its sorted
+// leaves are the queries, each walking the whole tree of the first context (the reference's
+// generate kernel leaves the last query out, BvttFrontGenerateKernels.cu:95; here every
+// triangle is a query). The front, pair lists and counters belong to this object; the front
+// kernels run on a view of the first context's tree that points at them.
+struct bvhcd_inter {
+    bvhcd_ctx* def = nullptr;    // the tree queried (not owned)
+    bvhcd_ctx* rigid = nullptr;  // the querying object (owned)
+    std::vector<void*> allocs;
+    std::string err;
+    DevState own;                // front, pairs, counters and ordering offsets of this object
+    bool front_valid = false;
+    int64_t def_version = -1;    // topology version of def the front was made on
+    int last_front = -1;
+    bool narrow = true;
+};
+
+namespace {
+
+int ifail(bvhcd_inter* x, int code, const std::string& msg) {
+    if (x) x->err = msg;
+    return code;
+}
+#define IHIPCHK(x, expr)                                                                       \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return ifail((x), BVHCD_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+bool ialloc(bvhcd_inter* x, T*& p, size_t n) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return false;
+    x->allocs.push_back(q);
+    p = static_cast<T*>(q);
+    return true;
+}
+
+// The first context's tree with this object's front, pairs and counters, and the second
+// object's sorted leaves as queries.
+DevState inter_view(bvhcd_inter* x) {
+    DevState v = x->def->d;
+    const DevState& o = x->own;
+    const DevState& r = x->rigid->d;
+    for (int k = 0; k < 2; ++k) {
+        v.front_cap[k] = o.front_cap[k];
+        v.fcur[k] = o.fcur[k];
+        for (int b = 0; b < 2; ++b) {
+            v.fnode[k][b] = o.fnode[k][b];
+            v.fquery[k][b] = o.fquery[k][b];
+        }
+    }
+    v.fsize = o.fsize;
+    v.fcnt = o.fcnt;
+    v.front_version = o.front_version;
+    v.adj_version = o.front_version;
+    v.intbeg = o.intbeg;
+    v.extbeg = o.extbeg;
+    v.bp = o.bp;
+    v.np = o.np;
+    v.pair_tmp = o.pair_tmp;
+    v.pair_cap = o.pair_cap;
+    v.pair_seg_cap = o.pair_seg_cap;
+    v.pcnt = o.pcnt;
+    v.ctr = o.ctr;
+    v.skip_adj = 0;
+    v.inter = 1;
+    v.qleaf = r.leaf;
+    v.qprim = r.prim_idx;
+    v.qlfaces = r.lfaces;
+    v.qverts = r.verts;
+    v.nq = (int32_t)x->rigid->T;
+    return v;
+}
+// Host-side front state the launchers changed (buffer flips, swapped arrays).
+void inter_keep_view(bvhcd_inter* x, const DevState& v) {
+    DevState& o = x->own;
+    for (int k = 0; k < 2; ++k) {
+        o.fcur[k] = v.fcur[k];
+        for (int b = 0; b < 2; ++b) {
+            o.fnode[k][b] = v.fnode[k][b];
+            o.fquery[k][b] = v.fquery[k][b];
+        }
+    }
+    o.front_version = v.front_version;
+}
+
+int inter_check_overflow(bvhcd_inter* x) {
+    uint32_t ovf = 0;
+    hipStream_t s = x->def->stream;
+    IHIPCHK(x, hipMemcpyAsync(&ovf, x->own.ctr + C_OVF, sizeof(ovf), hipMemcpyDeviceToHost, s));
+    IHIPCHK(x, hipStreamSynchronize(s));
+    if (ovf & 16u) return ifail(x, BVHCD_E_DEVICE, "traversal step bound exceeded (corrupted tree)");
+    if (ovf) return ifail(x, BVHCD_E_CAPACITY, "inter front or pair capacity exceeded");
+    return BVHCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bvhcd_inter_create(bvhcd_ctx* def, const bvhcd_config* cfg, bvhcd_inter** out) {
+    if (!def || !cfg || !out || cfg->max_tris < 1 || cfg->max_verts < 1) return BVHCD_E_INVALID;
+    *out = nullptr;
+    bvhcd_inter* x = new (std::nothrow) bvhcd_inter();
+    if (!x) return BVHCD_E_OOM;
+    x->def = def;
+    x->narrow = cfg->narrow_phase != 0;
+    bvhcd_config rc = *cfg;  // the second object's tree: no front or pair lists of its own to speak of
+    rc.int_front_cap = rc.ext_front_cap = rc.pair_cap = kPairSegs;
+    rc.scheduler = BVHCD_SCHED_PLAIN;
+    rc.enable_restructure = 0;
+    rc.device = def->cfg.device;
+    rc.stream = def->stream;
+    int st = bvhcd_create(&rc, &x->rigid);
+    if (st != BVHCD_OK) {
+        delete x;
+        return st;
+    }
+    DevState& o = x->own;
+    const int64_t Tq = cfg->max_tris, Td = def->d.maxT, lim = 0x7FFFFFFFll;
+    int64_t ci = cfg->int_front_cap > 0 ? cfg->int_front_cap : 48 * Tq + (1 << 20);
+    int64_t ce = cfg->ext_front_cap > 0 ? cfg->ext_front_cap : 24 * Tq + (1 << 20);
+    int64_t cp = cfg->pair_cap > 0 ? cfg->pair_cap : 24 * Tq + (1 << 20);
+    ci = std::min(ci, lim); ce = std::min(ce, lim); cp = std::min(cp, lim);
+    o.front_cap[0] = kPairSegs * ((ci + kPairSegs - 1) / kPairSegs);
+    o.front_cap[1] = kPairSegs * ((ce + kPairSegs - 1) / kPairSegs);
+    o.pair_seg_cap = std::max<int64_t>((cp + kPairSegs - 1) / kPairSegs, 1);
+    o.pair_cap = o.pair_seg_cap * kPairSegs;
+    bool ok = true;
+    for (int k = 0; k < 2; ++k)
+        for (int b = 0; b < 2; ++b) {
+            ok &= ialloc(x, o.fnode[k][b], o.front_cap[k]);
+            ok &= ialloc(x, o.fquery[k][b], o.front_cap[k]);
+        }
+    ok &= ialloc(x, o.fsize, 4);
+    ok &= ialloc(x, o.fcnt, 2 * kPairSegs * kCntStride);
+    ok &= ialloc(x, o.intbeg, Td + 2);
+    ok &= ialloc(x, o.extbeg, Td + 2);
+    ok &= ialloc(x, o.bp, o.pair_cap);
+    ok &= ialloc(x, o.np, o.pair_cap);
+    ok &= ialloc(x, o.pair_tmp, o.pair_cap);
+    ok &= ialloc(x, o.pcnt, 2 * kPairSegs * kCntStride);
+    ok &= ialloc(x, o.ctr, 32);
+    // the front ordering sorts this object's fronts with the first context's sort scratch: it
+    // must hold the larger of the two
+    ok &= std::max(o.front_cap[0], o.front_cap[1]) <= std::max<int64_t>(Td, std::max(def->d.front_cap[0],
+                                                                                       def->d.front_cap[1]));
+    hipStream_t s = def->stream;
+    ok = ok && hipMemsetAsync(o.ctr, 0, 32 * sizeof(uint32_t), s) == hipSuccess &&
+         hipMemsetAsync(o.pcnt, 0, 2 * kPairSegs * kCntStride * sizeof(uint32_t), s) == hipSuccess &&
+         hipMemsetAsync(o.fsize, 0, 4 * sizeof(uint32_t), s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) {
+        bvhcd_inter_destroy(x);
+        return BVHCD_E_OOM;
+    }
+    *out = x;
+    return BVHCD_OK;
+}
+
+int bvhcd_inter_destroy(bvhcd_inter* x) {
+    if (!x) return BVHCD_E_INVALID;
+    if (x->def) (void)hipStreamSynchronize(x->def->stream);
+    for (void* p : x->allocs) (void)hipFree(p);
+    if (x->rigid) bvhcd_destroy(x->rigid);
+    delete x;
+    return BVHCD_OK;
+}
+
+const char* bvhcd_inter_last_error(const bvhcd_inter* x) {
+    if (!x) return "null inter";
+    return !x->err.empty() ? x->err.c_str() : x->rigid ? x->rigid->err.c_str() : "";
+}
+
+int bvhcd_inter_set_topology(bvhcd_inter* x, const int32_t* faces, int64_t T, int64_t V, int where) {
+    if (!x) return BVHCD_E_INVALID;
+    x->front_valid = false;
+    x->err.clear();
+    return bvhcd_set_topology(x->rigid, faces, T, V, where);
+}
+
+int bvhcd_inter_update_vertices(bvhcd_inter* x, const float* xyz, int64_t V, int where) {
+    if (!x) return BVHCD_E_INVALID;
+    x->err.clear();
+    return bvhcd_update_vertices(x->rigid, xyz, V, where);
+}
+
+int bvhcd_inter_step(bvhcd_inter* x, int rigid_op, int front_op) {
+    if (!x) return BVHCD_E_INVALID;
+    x->err.clear();
+    bvhcd_ctx* def = x->def;
+    bvhcd_ctx* rg = x->rigid;
+    if (!def->built || def->T < 1) return ifail(x, BVHCD_E_NOT_BUILT, "step the queried context first");
+    if (rg->T < 1) return ifail(x, BVHCD_E_INVALID, "set_topology first");
+    if ((rigid_op != BVHCD_BVH_BUILD && rigid_op != BVHCD_BVH_REFIT) || front_op < BVHCD_FRONT_PURE_BVH_CD ||
+        front_op > BVHCD_FRONT_KEEP)
+        return ifail(x, BVHCD_E_INVALID, "bad op");
+    if (rigid_op == BVHCD_BVH_REFIT && !rg->built) return ifail(x, BVHCD_E_NOT_BUILT, "first inter step must BUILD");
+    hipStream_t s = def->stream;
+    // the querying object's tree: build (codes, sort, emission) or refit its boxes
+    if (rigid_op == BVHCD_BVH_BUILD) {
+        int st = build_and_refit(rg);
+        if (st) return ifail(x, st, "inter build: " + rg->err);
+        rg->built = true;
+        x->front_valid = false;
+    } else {
+        launch_refit(rg->d, (int32_t)rg->T, s, false);
+    }
+    if (x->def_version != def->d.topo_version) x->front_valid = false;
+    if (front_op != BVHCD_FRONT_PURE_BVH_CD && !x->front_valid) front_op = BVHCD_FRONT_GENERATE;
+    DevState v = inter_view(x);
+    const int32_t n = (int32_t)def->T;
+    if (front_op != BVHCD_FRONT_KEEP && front_op != BVHCD_FRONT_PURE_BVH_CD) v.front_version++;
+    switch (front_op) {
+        case BVHCD_FRONT_PURE_BVH_CD: launch_pure_cd(v, n, s); break;
+        case BVHCD_FRONT_GENERATE:
+            launch_generate(v, n, s, true);
+            x->front_valid = true;
+            x->def_version = def->d.topo_version;
+            break;
+        case BVHCD_FRONT_UPDATE: launch_update(v, n, 0, s, true); break;
+        case BVHCD_FRONT_REORDER: launch_reorder(v, n, s); break;
+        default: launch_keep(v, n, s, false); break;
+    }
+    if (x->narrow) launch_narrow(v, s, false);
+    else clear_np(v, s);
+    launch_pair_totals(v, s, x->narrow);
+    inter_keep_view(x, v);
+    x->last_front = front_op;
+    // the next host upload into the querying object's vertex buffer waits for these kernels
+    if (rg->d.verts == rg->own_verts || rg->d.verts == rg->own_verts2)
+        IHIPCHK(x, hipEventRecord(rg->consumed[rg->d.verts == rg->own_verts2 ? 1 : 0], s));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ifail(x, BVHCD_E_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return BVHCD_OK;
+}
+
+int bvhcd_inter_get_pairs(bvhcd_inter* x, int32_t* out, int64_t cap, int64_t* num, int where, int filtered) {
+    if (!x || !num) return BVHCD_E_INVALID;
+    hipStream_t s = x->def->stream;
+    uint32_t cnt = 0;
+    IHIPCHK(x, hipMemcpyAsync(&cnt, x->own.ctr + (filtered ? C_NP : C_BP), sizeof(cnt), hipMemcpyDeviceToHost, s));
+    const int st = inter_check_overflow(x);
+    *num = cnt;
+    if (st || !out || cap <= 0 || cnt == 0) return st;
+    DevState v = inter_view(x);
+    const int64_t k = std::min<int64_t>(cap, cnt);
+    if (where == BVHCD_MEM_DEVICE) {
+        launch_pair_gather(v, filtered ? 1 : 0, reinterpret_cast<int2*>(out), 0, k, s);
+    } else {
+        launch_pair_gather(v, filtered ? 1 : 0, v.pair_tmp, 0, k, s);
+        IHIPCHK(x, hipMemcpyAsync(out, v.pair_tmp, sizeof(int2) * k, hipMemcpyDeviceToHost, s));
+    }
+    IHIPCHK(x, hipStreamSynchronize(s));
+    return BVHCD_OK;
+}
+
+int bvhcd_inter_export_front(bvhcd_inter* x, int which, int32_t* out, int64_t cap, int64_t* n) {
+    if (!x || !n || which < 0 || which > 1) return BVHCD_E_INVALID;
+    hipStream_t s = x->def->stream;
+    uint32_t sz = 0;
+    IHIPCHK(x, hipMemcpyAsync(&sz, x->own.fsize + 2 * which + x->own.fcur[which], sizeof(sz), hipMemcpyDeviceToHost,
+                              s));
+    IHIPCHK(x, hipStreamSynchronize(s));
+    *n = sz;
+    const int64_t k = std::min<int64_t>(cap, sz);
+    if (!out || k <= 0) return BVHCD_OK;
+    std::vector<uint32_t> node((size_t)k);
+    std::vector<int32_t> query((size_t)k);
+    IHIPCHK(x, hipMemcpy(node.data(), x->own.fnode[which][x->own.fcur[which]], 4 * k, hipMemcpyDeviceToHost));
+    IHIPCHK(x, hipMemcpy(query.data(), x->own.fquery[which][x->own.fcur[which]], 4 * k, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < k; ++i) {
+        out[2 * i] = query[i];
+        out[2 * i + 1] = (int32_t)node[i];
+    }
+    return BVHCD_OK;
+}
+
+int bvhcd_inter_stats(bvhcd_inter* x, int64_t* out) {
+    if (!x || !out) return BVHCD_E_INVALID;
+    hipStream_t s = x->def->stream;
+    uint32_t sz[4], ctr[4];
+    uint64_t visits = 0;
+    IHIPCHK(x, hipMemcpyAsync(sz, x->own.fsize, sizeof(sz), hipMemcpyDeviceToHost, s));
+    IHIPCHK(x, hipMemcpyAsync(ctr, x->own.ctr, sizeof(ctr), hipMemcpyDeviceToHost, s));
+    IHIPCHK(x, hipMemcpyAsync(&visits, ctr64(x->own.ctr) + C64_VISITS, sizeof(visits), hipMemcpyDeviceToHost, s));
+    IHIPCHK(x, hipStreamSynchronize(s));
+    out[0] = sz[x->own.fcur[0]];
+    out[1] = sz[2 + x->own.fcur[1]];
+    out[2] = ctr[C_BP];
+    out[3] = ctr[C_NP];
+    out[4] = (int64_t)visits;
+    out[5] = x->last_front;
+    return inter_check_overflow(x);
+}
+
+int bvhcd_inter_export_rigid(bvhcd_inter* x, uint32_t* codes, int32_t* prim_idx) {
+    if (!x) return BVHCD_E_INVALID;
+    if (!x->rigid->built) return ifail(x, BVHCD_E_NOT_BUILT, "no tree yet");
+    const int64_t T = x->rigid->T;
+    if (codes) IHIPCHK(x, hipMemcpy(codes, x->rigid->d.codes, 4 * T, hipMemcpyDeviceToHost));
+    if (prim_idx) IHIPCHK(x, hipMemcpy(prim_idx, x->rigid->d.prim_idx, 4 * T, hipMemcpyDeviceToHost));
+    return BVHCD_OK;
 }
 
 }  // extern "C"

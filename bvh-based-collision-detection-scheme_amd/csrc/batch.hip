@@ -404,6 +404,8 @@ int bvhcd_batch_create(const bvhcd_config* cfg, int32_t S, const int64_t* tps, c
         delete b;
         return BVHCD_E_OOM;
     }
+    d.qleaf = d.leaf;  // self-CD queries: the tree's own leaves
+    d.qprim = d.prim_idx;
     // static per-scene tables: sorted leaves of scene s are [toff[s], toff[s + 1])
     std::vector<int32_t> last(S);
     for (int32_t s = 0; s < S; ++s) last[s] = (int32_t)(b->toff[s + 1] - 1);

@@ -257,6 +257,17 @@ struct DevState {
     uint32_t* pcnt = nullptr;       // [2][kPairSegs * kCntStride]: bp, np segment counts
     // counters block: see Counter enum
     uint32_t* ctr = nullptr;        // 32 words
+    // Queries of the front kernels: the tree's own leaves for self-CD (qleaf = leaf, qprim =
+    // prim_idx, each query walking only the leaves after it). An inter-object view (inter = 1)
+    // points them at another object's sorted leaves instead, its nq queries each walking the
+    // whole tree; qlfaces / qverts are that object's leaf-ordered faces and vertices.
+    const NodeRec* qleaf = nullptr;
+    const int32_t* qprim = nullptr;
+    const int32_t* qlfaces = nullptr;
+    const float* qverts = nullptr;
+    int32_t nq = 0;
+    int inter = 0;
+    int64_t topo_version = 0;       // host-side: bumped by every topology change (refit preparation)
 };
 
 enum Counter : int {

@@ -468,9 +468,11 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int32_t n, DevState d, Fron
     const Out out = make_out(st, fi, fe, d);
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t v = 0;
-    if (i < n - 1) {
-        const float4 lo = d.leaf[i].lo, hi = d.leaf[i].hi;
-        v = stackless<FRONT>(d, out, i, rec_box(lo, hi), f2i(hi.w), -1);
+    // self-CD: query leaf i walks the leaves after it (from lca[i + 1], its record's escape);
+    // inter-object: every query walks the whole tree from its root lca[0]
+    if (i < (d.inter ? d.nq : n - 1)) {
+        const float4 lo = d.qleaf[i].lo, hi = d.qleaf[i].hi;
+        v = stackless<FRONT>(d, out, i, rec_box(lo, hi), d.inter ? d.leaf_lca[0] : f2i(hi.w), -1);
     }
     if (FRONT) {
         out.final_flush();
@@ -480,11 +482,14 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int32_t n, DevState d, Fron
     add_visits(d, v);
 }
 
+// Queries of a traversal: the leaves before the last one (self-CD) or the other object's leaves.
+static int64_t query_count(const DevState& d, int32_t n) { return d.inter ? d.nq : n - 1; }
+
 void launch_pure_cd(DevState& d, int32_t n, hipStream_t s) {
     reset_step_counters(d, s);
-    if (n >= 2)
-        hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, null_sink(),
-                           null_sink());
+    const int64_t nq = query_count(d, n);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(nq)), dim3(kBlock), 0, s, n, d, null_sink(), null_sink());
 }
 
 // ---------------------------------------------------------------- batched scenes
@@ -596,7 +601,8 @@ void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order) {
     reset_step_counters(d, s);
     front_begin(d, s);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
-    if (n >= 2) hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d, fi, fe);
+    const int64_t nq = query_count(d, n);
+    if (nq > 0) hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(nq)), dim3(kBlock), 0, s, n, d, fi, fe);
     front_finish(d, 0, false, s);
     front_finish(d, 1, false, s);
     if (order && n >= 2) launch_order_fronts(d, n, s);
@@ -614,7 +620,7 @@ __global__ void __launch_bounds__(kBlock) k_update(DevState d, FrontIn in, Front
         if (k < ni) {
             const int32_t q = in.query[0][k];
             int32_t node = (int32_t)in.node[0][k];
-            const float4 qlo = d.leaf[q].lo, qhi = d.leaf[q].hi;
+            const float4 qlo = d.qleaf[q].lo, qhi = d.qleaf[q].hi;
             const Box qb = rec_box(qlo, qhi);
             const float4 lo = d.inode[node].lo, hi = d.inode[node].hi;
             ++visits;
@@ -637,7 +643,7 @@ __global__ void __launch_bounds__(kBlock) k_update(DevState d, FrontIn in, Front
         } else {
             const int32_t q = in.query[1][k - ni];
             const int32_t j = (int32_t)in.node[1][k - ni];
-            const float4 qlo = d.leaf[q].lo, qhi = d.leaf[q].hi;
+            const float4 qlo = d.qleaf[q].lo, qhi = d.qleaf[q].hi;
             const Box qb = rec_box(qlo, qhi);
             const float4 llo = d.leaf[j].lo, lhi = d.leaf[j].hi;
             ++visits;
@@ -775,8 +781,8 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int imp
             qlo[u] = qhi[u] = lo[u] = hi[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (q[u] >= 0) {
                 const NodeRec* r = k0 + u * stride < ni ? &d.inode[nd[u]] : &d.leaf[nd[u]];
-                qlo[u] = d.leaf[q[u]].lo;
-                qhi[u] = d.leaf[q[u]].hi;
+                qlo[u] = d.qleaf[q[u]].lo;
+                qhi[u] = d.qleaf[q[u]].hi;
                 lo[u] = r->lo;
                 hi[u] = r->hi;
             }
@@ -884,13 +890,15 @@ void launch_adj_partition(DevState& d, int32_t n, hipStream_t s) {
 }
 
 void launch_keep(DevState& d, int32_t n, hipStream_t s, bool counters_zeroed) {
-    if (d.adj_version != d.front_version) launch_adj_partition(d, n, s);
+    // inter-object fronts pair triangles of two objects: no shared vertices, no partition
+    const int implicit = d.inter ? 0 : 1;
+    if (implicit && d.adj_version != d.front_version) launch_adj_partition(d, n, s);
     if (!counters_zeroed) reset_step_counters(d, s);
     const unsigned sh = tuned_shape(TK_KEEP, n), g = shape_grid(sh);
     switch (shape_unroll(sh)) {
-        case 4: hipLaunchKernelGGL(k_keep<4>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), 1); break;
-        case 2: hipLaunchKernelGGL(k_keep<2>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), 1); break;
-        default: hipLaunchKernelGGL(k_keep<1>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), 1); break;
+        case 4: hipLaunchKernelGGL(k_keep<4>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), implicit); break;
+        case 2: hipLaunchKernelGGL(k_keep<2>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), implicit); break;
+        default: hipLaunchKernelGGL(k_keep<1>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), implicit); break;
     }
 }
 
@@ -1009,7 +1017,7 @@ __global__ void __launch_bounds__(kBlock) k_reorder_keys(DevState d, FrontIn in,
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ni; k += gridDim.x * blockDim.x) {
         const uint32_t node = in.node[0][k];
         const int32_t q = in.query[0][k];
-        const float4 qlo = d.leaf[q].lo, qhi = d.leaf[q].hi;
+        const float4 qlo = d.qleaf[q].lo, qhi = d.qleaf[q].hi;
         const bool ov = node_overlap(d, (int32_t)node, rec_box(qlo, qhi));
         keys[k] = node * 2u + (ov ? 1u : 0u);
     }
@@ -1124,6 +1132,15 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
     const uint32_t npairs = seg_prefix(d.pcnt, (uint32_t)d.pair_seg_cap, pre);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += gridDim.x * blockDim.x) {
         const int2 lv = d.bp[seg_slot(pre, k, d.pair_seg_cap)];  // leaves
+        if (d.inter) {  // (query leaf of the other object, leaf of this tree): this tree's triangle is P
+            const int32_t* fa = d.lfaces + 3 * (int64_t)lv.y;
+            const int32_t* fb = d.qlfaces + 3 * (int64_t)lv.x;
+            const bool hit = sat_tri_tri(ld3(d.verts, fa[0]), ld3(d.verts, fa[1]), ld3(d.verts, fa[2]),
+                                         ld3(d.qverts, fb[0]), ld3(d.qverts, fb[1]), ld3(d.qverts, fb[2]));
+            stage_push(sink, &d.ctr[C_OVF], sbuf[w], sfill[w], kStage, hit,
+                       make_int2(d.prim_idx[lv.y], d.qprim[lv.x]));
+            continue;
+        }
         // canonical order: the triangle with the lower id is P
         int32_t ta = d.prim_idx[lv.x], tb = d.prim_idx[lv.y], la = lv.x, lb = lv.y;
         if (ta > tb) {
@@ -1192,6 +1209,7 @@ __device__ inline int2 list_pair(const DevState& d, int kind, const uint32_t* pr
         const int c = d.fcur[1];
         p = make_int2(d.fquery[1][c][k - listed], (int32_t)d.fnode[1][c][k - listed]);
     }
+    if (d.inter) return make_int2(d.prim_idx[p.y], d.qprim[p.x]);  // (this tree's triangle, query's)
     const int32_t a = d.prim_idx[p.x], b = d.prim_idx[p.y];
     return a < b ? make_int2(a, b) : make_int2(b, a);
 }

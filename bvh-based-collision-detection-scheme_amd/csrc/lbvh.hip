@@ -1090,6 +1090,7 @@ __global__ void __launch_bounds__(kTopBlock) k_rp_top_segments(DevState d, int32
 int32_t refit_chunks(int32_t n) { return (n + kTreelet - 1) / kTreelet; }
 
 void launch_refit_prep(DevState& d, int32_t n, hipStream_t s) {
+    d.topo_version++;  // every topology change passes here
     if (n < 1) return;
     const int32_t nc = refit_chunks(n);
     fill_i32(d.rp_cstart, nc + 1, n, s);

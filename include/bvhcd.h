@@ -250,6 +250,40 @@ int  bvhcd_mlp_set_repair_cost(bvhcd_ctx* ctx, float repair_cost);
    scheduler. */
 int  bvhcd_restructure(bvhcd_ctx* ctx, const int32_t* roots, int64_t k);
 
+/* ---- inter-object front (BvttFrontLooseInter + LBvhRigid,
+        collision/bvtt_front/BvttFrontLooseInter.{h,cu}, collision/lbvh/LBvhRigid.{h,cu}) ----
+   A second object's triangles queried against a context's tree: the second object (the
+   reference's rigid model: fixed topology; BUILD orders its triangles by Morton code, REFIT
+   updates their boxes) gets a tree of its own whose sorted leaves are the queries, each walking
+   the whole tree of the queried context, with a loose front of (query, node) entries maintained
+   by the same operations as the self-collision front. Every triangle of the second object is a
+   query (the reference's generate kernel skips the last one, BvttFrontGenerateKernels.cu:95).
+   The queried context must have been stepped for the frame first; its tree is read, not
+   changed, and a change of its topology makes the next inter step regenerate the front. Work is
+   ordered on the queried context's stream. */
+typedef struct bvhcd_inter bvhcd_inter;
+/* cfg: max_tris / max_verts of the second object, int / ext front and pair capacities (0 =
+   48 T + 2^20, 24 T + 2^20, 24 T + 2^20 of its T), narrow_phase; other fields are ignored. */
+int  bvhcd_inter_create(bvhcd_ctx* queried, const bvhcd_config* cfg, bvhcd_inter** out);
+int  bvhcd_inter_destroy(bvhcd_inter* x);
+const char* bvhcd_inter_last_error(const bvhcd_inter* x);
+int  bvhcd_inter_set_topology(bvhcd_inter* x, const int32_t* faces, int64_t num_tris, int64_t num_verts, int where);
+int  bvhcd_inter_update_vertices(bvhcd_inter* x, const float* xyz, int64_t num_verts, int where);
+/* rigid_op: BVHCD_BVH_BUILD or BVHCD_BVH_REFIT of the second object's tree; front_op:
+   PURE_BVH_CD, GENERATE, UPDATE, REORDER or KEEP (BvttFrontLooseInter::maintain, :69-104). */
+int  bvhcd_inter_step(bvhcd_inter* x, int rigid_op, int front_op);
+/* Pairs (triangle of the queried context, triangle of the second object); filtered = after the
+   triangle test, which takes the queried context's triangle as P. */
+int  bvhcd_inter_get_pairs(bvhcd_inter* x, int32_t* out, int64_t cap_pairs, int64_t* num_pairs, int where,
+                           int filtered);
+/* Front entries (query = sorted leaf of the second object, node of the queried tree). */
+int  bvhcd_inter_export_front(bvhcd_inter* x, int which, int32_t* out, int64_t cap, int64_t* n);
+/* out[6]: internal front, external front, broad-phase pairs, narrow-phase pairs, node tests,
+   last front op. */
+int  bvhcd_inter_stats(bvhcd_inter* x, int64_t* out);
+/* The second object's sorted codes and sorted slot -> triangle map. */
+int  bvhcd_inter_export_rigid(bvhcd_inter* x, uint32_t* codes, int32_t* prim_idx);
+
 /* ---- batched scenes (many scenes per launch; one hipGraph per frame) ----
    No counterpart object in the reference, which runs one scene per LBvh / BvttFront pair
    (Frameworks/CudaProject/CudaProjectApp.cpp:28-34); each scene here gets exactly the pairs a single-scene context

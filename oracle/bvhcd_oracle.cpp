@@ -376,6 +376,7 @@ struct Ctx {
     int bvh_cycle = 12, front_cycle = 6;
     bool enable_restr = true;
     int scheme = 0;              // scheme_ops id used by orc_step_scheduled
+    int64_t topo_version = 0;    // bumped whenever the BVH topology changes (inter fronts rely on it)
     int last_bvh_op = -1, last_front_op = -1;
     int64_t visits = 0, restr_subtrees = 0, restr_leaves = 0;
     int narrow = 1;
@@ -897,7 +898,10 @@ int step(Ctx& c, int bvh_op, int front_op, int threads) {
             return -1;
     }
     c.last_bvh_op = bvh_op == 2 ? (tag == 2 ? 0 : tag == 0 ? 1 : 2) : bvh_op;
-    if (topo_changed) c.front_valid = false;
+    if (topo_changed) {
+        c.front_valid = false;
+        c.topo_version++;
+    }
     if (c.repair_pending && (!c.front_valid || front_op == 0)) {
         c.repair_pending = false;  // a pure traversal frame or a new topology drops the repair
         c.front_valid = false;
@@ -1257,6 +1261,7 @@ int orc_restructure_roots(void* p, const int32_t* roots, int64_t k, int threads)
     for (size_t i = 1; i < rg.size(); ++i)
         if (rg[i].first <= rg[i - 1].second) return -1;
     if (k == 0) return 0;
+    c->topo_version++;
     if (c->repair_pending) c->front_valid = c->repair_pending = false;  // one repair at a time
     if (c->front_valid) front_restr_prepare(*c, std::vector<int32_t>(roots, roots + k));
     Box scene = scene_box(c->T, c->faces.data(), c->verts.data());
@@ -1265,5 +1270,267 @@ int orc_restructure_roots(void* p, const int32_t* roots, int64_t k, int threads)
     c->restr_pending = false;
     return 0;
 }
+
+
+}  // extern "C"
+
+// ====================================================================== inter-object front
+// A rigid object's triangles queried against the deformable BVH (BvttFrontLooseInter,
+// collision/bvtt_front/BvttFrontLooseInter.cu:16-260). The rigid object gets its own LBVH
+// (LBvhRigid, collision/lbvh/LBvhRigid.cu: build = codes, sort, emission; refit = boxes), whose
+// sorted primitives are the queries: query q is the rigid leaf slot q, with that leaf's box.
+// Each query walks the whole deformable tree from its root (genLooseInterFrontsWithLog,
+// BvttFrontGenerateKernels.cu:92-125, which stops one query short: `idx >= travPrimSize - 1`;
+// here every rigid primitive is a query). Pairs are (deformable triangle, rigid triangle).
+namespace {
+
+struct InterCtx {
+    Ctx* def = nullptr;
+    int64_t T = 0, V = 0;
+    std::vector<int32_t> faces;
+    std::vector<float> verts;
+    Bvh rb;
+    bool built = false, front_valid = false;
+    int64_t def_version = -1;
+    std::vector<Entry> fi, fe;  // (rigid slot, deformable node)
+    std::vector<Pair> bp, np;   // (deformable triangle, rigid triangle)
+    int64_t visits = 0;
+    int last_front_op = -1;
+};
+
+struct InterSink {
+    std::vector<Entry> fi, fe;
+    std::vector<std::pair<int32_t, int32_t>> pairs;  // (deformable leaf, rigid slot)
+    int64_t visits = 0;
+};
+
+// Stackless walk of the deformable tree for one query box (A.6), from encoded `st` to `stop`.
+template <bool FRONT>
+void inter_walk(const Bvh& b, const Box& bv, int32_t q, int32_t st, int32_t stop, InterSink& s) {
+    while (st != stop) {
+        int32_t t = st & 1;
+        st >>= 1;
+        int32_t lbd;
+        if (!t) {
+            lbd = b.rx[st];
+            t = b.lpar[lbd];
+            for (; st <= t; ++st) {
+                ++s.visits;
+                if (!overlaps(b.ibox[st], bv)) break;
+            }
+        } else {
+            t = st - 1;
+            lbd = st;
+        }
+        if (st == t + 1) {
+            ++s.visits;
+            if (overlaps(b.lbox[lbd], bv)) s.pairs.push_back({lbd, q});
+            if (FRONT) s.fe.push_back({q, lbd});
+            st = b.llca[lbd + 1];
+        } else {
+            if (FRONT) s.fi.push_back({q, st});
+            st = b.llca[b.ry[st] + 1];
+        }
+    }
+}
+
+template <class F>
+void inter_run(InterCtx& x, int64_t n, int threads, F&& fn, bool keep_front) {
+    const int nt = threads <= 0 ? (int)std::thread::hardware_concurrency() : threads;
+    std::vector<InterSink> sinks(nt < 1 ? 1 : nt);
+    parallel_for(n, (int)sinks.size(), [&](int64_t a, int64_t e, int t) {
+        for (int64_t k = a; k < e; ++k) fn(k, sinks[t]);
+    });
+    const Bvh& b = x.def->bvh;
+    if (keep_front) {
+        x.fi.clear();
+        x.fe.clear();
+    }
+    x.bp.clear();
+    x.visits = 0;
+    for (auto& s : sinks) {
+        if (keep_front) {
+            x.fi.insert(x.fi.end(), s.fi.begin(), s.fi.end());
+            x.fe.insert(x.fe.end(), s.fe.begin(), s.fe.end());
+        }
+        for (auto& pr : s.pairs) x.bp.push_back({b.prim_idx[pr.first], x.rb.prim_idx[pr.second]});
+        x.visits += s.visits;
+    }
+}
+
+// genLooseInterFrontsWithLog / pureBvhInterCD (BvhCDKernels.cu:51-75).
+template <bool FRONT>
+void inter_traverse(InterCtx& x, int threads) {
+    const Bvh& b = x.def->bvh;
+    const int32_t root = b.llca[0];
+    inter_run(x, x.rb.n, threads, [&](int64_t q, InterSink& s) {
+        inter_walk<FRONT>(b, x.rb.lbox[q], (int32_t)q, root, -1, s);
+    }, FRONT);
+}
+
+// maintainInt/ExtLooseInterFrontsWithLog (BvttFrontPruneSproutKernels.cu:203-275).
+void inter_update(InterCtx& x, int threads) {
+    const Bvh& b = x.def->bvh;
+    const std::vector<Entry> fi = x.fi, fe = x.fe;
+    const int64_t ni = (int64_t)fi.size(), ne = (int64_t)fe.size();
+    inter_run(x, ni + ne, threads, [&](int64_t k, InterSink& s) {
+        if (k < ni) {
+            const Entry e = fi[k];
+            int32_t st = e.node;
+            const Box& bv = x.rb.lbox[e.q];
+            ++s.visits;
+            if (!overlaps(b.ibox[st], bv)) {  // prune: climb the left spine
+                const int32_t t = b.llca[b.rx[st]] >> 1;
+                for (st--; st >= t; st--) {
+                    ++s.visits;
+                    if (overlaps(b.ibox[st], bv)) break;
+                }
+                if (st < t && st + 1 > 0) {
+                    ++s.visits;
+                    if (!overlaps(b.ibox[b.par[st + 1]], bv)) return;
+                }
+                s.fi.push_back({e.q, st + 1});
+                return;
+            }
+            inter_walk<true>(b, bv, e.q, (b.lc[st] << 1) | (int32_t)(b.mark[st] & 1u), b.llca[b.ry[st] + 1], s);
+        } else {
+            const Entry e = fe[k - ni];
+            int32_t st = e.node;
+            const Box& bv = x.rb.lbox[e.q];
+            ++s.visits;
+            if (overlaps(b.lbox[st], bv)) {
+                s.fe.push_back(e);
+                s.pairs.push_back({st, e.q});
+                return;
+            }
+            const int32_t gfa = b.lpar[st];
+            ++s.visits;
+            if (overlaps(b.ibox[gfa], bv)) {
+                s.fe.push_back(e);
+                return;
+            }
+            if ((b.lmark[st] & 4u) == 4u) return;
+            const int32_t top = b.llca[st] >> 1;
+            for (st = gfa - 1; st >= top; st--) {
+                ++s.visits;
+                if (overlaps(b.ibox[st], bv)) break;
+            }
+            if (st < top && st + 1 > 0) {
+                ++s.visits;
+                if (!overlaps(b.ibox[b.par[st + 1]], bv)) return;
+            }
+            s.fi.push_back({e.q, st + 1});
+        }
+    }, true);
+}
+
+// keepInt/ExtLooseInterFronts (BvttFrontKeepKernels.cu:60-96).
+void inter_keep(InterCtx& x, int threads) {
+    const Bvh& b = x.def->bvh;
+    const int64_t ni = (int64_t)x.fi.size(), ne = (int64_t)x.fe.size();
+    inter_run(x, ni + ne, threads, [&](int64_t k, InterSink& s) {
+        if (k < ni) {
+            const Entry e = x.fi[k];
+            const Box& bv = x.rb.lbox[e.q];
+            ++s.visits;
+            if (!overlaps(b.ibox[e.node], bv)) return;
+            inter_walk<false>(b, bv, e.q, (b.lc[e.node] << 1) | (int32_t)(b.mark[e.node] & 1u),
+                              b.llca[b.ry[e.node] + 1], s);
+        } else {
+            const Entry e = x.fe[k - ni];
+            ++s.visits;
+            if (overlaps(b.lbox[e.node], x.rb.lbox[e.q])) s.pairs.push_back({e.node, e.q});
+        }
+    }, false);
+}
+
+// Narrow phase of inter pairs: the deformable triangle is P, the rigid one Q.
+void inter_narrow(InterCtx& x, int threads) {
+    x.np.clear();
+    if (!x.def->narrow) return;
+    std::vector<uint8_t> hit(x.bp.size(), 0);
+    parallel_for((int64_t)x.bp.size(), threads, [&](int64_t a, int64_t e, int) {
+        for (int64_t k = a; k < e; ++k) {
+            float P[9], Q[9];
+            gather_tri(x.def->faces.data(), x.def->verts.data(), x.bp[k].a, P);
+            gather_tri(x.faces.data(), x.verts.data(), x.bp[k].b, Q);
+            hit[k] = sat_tt<false>(P, Q) ? 1 : 0;
+        }
+    });
+    for (size_t k = 0; k < x.bp.size(); ++k)
+        if (hit[k]) x.np.push_back(x.bp[k]);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_inter_create(void* def, int64_t T, int64_t V, const int32_t* faces) {
+    InterCtx* x = new InterCtx();
+    x->def = static_cast<Ctx*>(def);
+    x->T = T;
+    x->V = V;
+    x->faces.assign(faces, faces + 3 * T);
+    x->verts.assign(3 * V, 0.f);
+    return x;
+}
+void orc_inter_destroy(void* p) { delete static_cast<InterCtx*>(p); }
+void orc_inter_update_vertices(void* p, const float* xyz) {
+    InterCtx* x = static_cast<InterCtx*>(p);
+    std::memcpy(x->verts.data(), xyz, sizeof(float) * 3 * x->V);
+}
+// rigid_op: 0 BUILD, 1 REFIT; front_op: 0 PURE_BVH_CD, 1 GENERATE, 2 UPDATE, 3 REORDER, 4 KEEP.
+// The front is regenerated whenever the deformable tree's topology changed since it was made.
+int orc_inter_step(void* p, int rigid_op, int front_op, int threads) {
+    InterCtx* x = static_cast<InterCtx*>(p);
+    if (!x->def->built || x->T <= 0) return -5;
+    if (rigid_op == 0) {
+        full_build(x->rb, x->T, x->faces.data(), x->verts.data(), threads);
+        x->built = true;
+    } else if (rigid_op == 1) {
+        if (!x->built) return -5;
+        refit_bvh(x->rb, x->faces.data(), x->verts.data(), threads);
+    } else {
+        return -1;
+    }
+    if (x->def_version != x->def->topo_version || rigid_op == 0) x->front_valid = false;
+    if (front_op != 0 && !x->front_valid) front_op = 1;
+    switch (front_op) {
+        case 0: inter_traverse<false>(*x, threads); break;
+        case 1:
+            inter_traverse<true>(*x, threads);
+            x->front_valid = true;
+            x->def_version = x->def->topo_version;
+            break;
+        case 2: inter_update(*x, threads); break;
+        case 3:  // balance: pairs from the unchanged cut (the order of entries is not observable)
+        case 4: inter_keep(*x, threads); break;
+        default: return -1;
+    }
+    x->last_front_op = front_op;
+    inter_narrow(*x, threads);
+    return 0;
+}
+int64_t orc_inter_get_pairs(void* p, int filtered, int32_t* out, int64_t cap) {
+    InterCtx* x = static_cast<InterCtx*>(p);
+    const auto& v = filtered ? x->np : x->bp;
+    const int64_t n = (int64_t)v.size();
+    for (int64_t k = 0; k < n && k < cap && out; ++k) { out[2 * k] = v[k].a; out[2 * k + 1] = v[k].b; }
+    return n;
+}
+int64_t orc_inter_export_front(void* p, int which, int32_t* out, int64_t cap) {
+    InterCtx* x = static_cast<InterCtx*>(p);
+    const auto& v = which ? x->fe : x->fi;
+    const int64_t n = (int64_t)v.size();
+    for (int64_t k = 0; k < n && k < cap && out; ++k) { out[2 * k] = v[k].q; out[2 * k + 1] = v[k].node; }
+    return n;
+}
+// The rigid object's sorted codes and slot -> triangle map.
+void orc_inter_export_rigid(void* p, uint32_t* codes, int32_t* prim_idx) {
+    InterCtx* x = static_cast<InterCtx*>(p);
+    if (codes) std::memcpy(codes, x->rb.codes.data(), 4 * x->rb.codes.size());
+    if (prim_idx) std::memcpy(prim_idx, x->rb.prim_idx.data(), 4 * x->rb.prim_idx.size());
+}
+int64_t orc_inter_last_front_op(void* p) { return static_cast<InterCtx*>(p)->last_front_op; }
 
 }  // extern "C"

@@ -57,6 +57,14 @@ def lib():
         "orc_stats": (None, [P, P]),
         "orc_export_restr_log": (None, [P, P, P]),
         "orc_quality_eval": (None, [I64, P, P, P, P, P, I32, P, P]),
+        "orc_inter_create": (P, [P, I64, I64, P]),
+        "orc_inter_destroy": (None, [P]),
+        "orc_inter_update_vertices": (None, [P, P]),
+        "orc_inter_step": (I32, [P, I32, I32, I32]),
+        "orc_inter_get_pairs": (I64, [P, I32, P, I64]),
+        "orc_inter_export_front": (I64, [P, I32, P, I64]),
+        "orc_inter_export_rigid": (None, [P, P, P]),
+        "orc_inter_last_front_op": (I64, [P]),
         "orc_restr_decision": (I32, [I64, P, P, P, P, P, C.POINTER(I64), C.POINTER(I64)]),
         "orc_restructure_roots": (I32, [P, P, I64, I32]),
     }
@@ -236,6 +244,74 @@ class OracleContext:
         keys = ["frame", "last_bvh_op", "last_front_op", "int_front", "ext_front", "bp_pairs", "np_pairs",
                 "node_visits", "restructured_subtrees", "restructured_leaves", "restr_pending"]
         return dict(zip(keys, (int(x) for x in s)))
+
+
+class OracleInter:
+    """Host restatement of the inter-object front: a rigid object's triangles against the
+    deformable BVH of an OracleContext (which must be stepped first each frame)."""
+
+    def __init__(self, deformable, faces, num_verts):
+        f = np.ascontiguousarray(faces, np.int32)
+        self.T, self.V = f.shape[0], int(num_verts)
+        self._def = deformable  # keeps the deformable context alive
+        self._h = lib().orc_inter_create(deformable._h, self.T, self.V, _p(f))
+
+    def close(self):
+        if self._h:
+            lib().orc_inter_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def update_vertices(self, xyz):
+        v = np.ascontiguousarray(xyz, np.float32)
+        lib().orc_inter_update_vertices(self._h, _p(v))
+
+    def step(self, rigid_op, front_op, threads=0):
+        st = lib().orc_inter_step(self._h, rigid_op, front_op, threads)
+        if st:
+            raise RuntimeError(f"oracle inter step failed: {st}")
+
+    @property
+    def last_front_op(self):
+        return int(lib().orc_inter_last_front_op(self._h))
+
+    def pairs(self, filtered=True):
+        n = lib().orc_inter_get_pairs(self._h, 1 if filtered else 0, None, 0)
+        out = np.empty((n, 2), np.int32)
+        if n:
+            lib().orc_inter_get_pairs(self._h, 1 if filtered else 0, _p(out), n)
+        return out
+
+    def front(self, which):
+        n = lib().orc_inter_export_front(self._h, which, None, 0)
+        out = np.empty((n, 2), np.int32)
+        if n:
+            lib().orc_inter_export_front(self._h, which, _p(out), n)
+        return out
+
+    def rigid_order(self):
+        codes, idx = np.empty(self.T, np.uint32), np.empty(self.T, np.int32)
+        lib().orc_inter_export_rigid(self._h, _p(codes), _p(idx))
+        return codes, idx
+
+
+def inter_bruteforce(faces_a, verts_a, faces_b, verts_b, narrow=True):
+    """All (a, b) with overlapping boxes (and, narrow, an SAT hit with a's triangle as P)."""
+    ba, bb = tri_boxes(faces_a, verts_a), tri_boxes(faces_b, verts_b)
+    ov = np.all((ba[:, None, :3] <= bb[None, :, 3:]) & (ba[:, None, 3:] >= bb[None, :, :3]), axis=2)
+    i, j = np.nonzero(ov)
+    if narrow:
+        keep = [sat(verts_a[faces_a[x]], verts_b[faces_b[y]]) for x, y in zip(i, j)]
+        i, j = i[np.array(keep, bool)], j[np.array(keep, bool)]
+    return np.stack([i, j], 1).astype(np.int32)
+
+
+def ordered_pair_set(pairs) -> np.ndarray:
+    """Sorted uint64 keys (a << 32 | b) of an ordered pair list (no swapping)."""
+    p = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    return np.sort((p[:, 0].astype(np.uint64) << np.uint64(32)) | p[:, 1].astype(np.uint64))
 
 
 def quality_eval(rx, ry, intbeg, extbeg, snapshot=None):

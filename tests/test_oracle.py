@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import load_package
+from conftest import check_cut, load_package, sheet_sphere
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -661,3 +661,37 @@ def test_restructure_decision_tiers(oracle):
         ir, rt = _marks(topo, set(chosen), n)
         tag, _, m = oracle.restr_decision(rx, ry, ir, rt)
         assert m == target and tag == lo_tag
+
+
+# ---------------------------------------------------------------- inter-object front
+def test_oracle_inter_front(oracle):
+    """Inter-object front (rigid sphere against the folding sheet's BVH) under GENERATE, KEEP,
+    UPDATE and REORDER, with the sheet's tree rebuilt on its own cycle: every frame's pairs equal
+    brute force over all (sheet, sphere) triangle pairs, both before and after the narrow phase,
+    and each query's front is a complete cut of the sheet's tree."""
+    pkg = load_package()
+    sc, fa, fb, nv = sheet_sphere(pkg)
+    dctx = oracle.OracleContext(fa, nv)
+    inter = oracle.OracleInter(dctx, fb, sc.num_verts - nv)
+    ops = [1, 4, 4, 2, 4, 3]
+    hits = 0
+    for fr in range(sc.frames):
+        v = sc.frame(fr)
+        va, vb = v[:nv], v[nv:]
+        dctx.update_vertices(va)
+        dctx.step_scheduled()
+        inter.update_vertices(vb)
+        inter.step(0 if fr == 0 else 1, ops[fr % len(ops)])
+        for narrow in (False, True):
+            got = oracle.ordered_pair_set(inter.pairs(narrow))
+            want = oracle.ordered_pair_set(oracle.inter_bruteforce(fa, va, fb, vb, narrow))
+            np.testing.assert_array_equal(got, want, err_msg=f"frame {fr} narrow {narrow}")
+        check_cut(inter.front(0), inter.front(1), dctx.export_bvh(), len(fb))
+        hits += len(inter.pairs(True))
+    assert hits > 0
+    codes, idx = inter.rigid_order()
+    # the rigid tree is refit, not rebuilt: its order is that of the first frame
+    v0 = sc.frame(0)[nv:]
+    s0, idx0, _ = oracle.sort_codes(oracle.morton_codes(fb, v0, oracle.scene_box(fb, v0)))
+    np.testing.assert_array_equal(idx, idx0)
+    np.testing.assert_array_equal(codes, s0)
