@@ -24,13 +24,8 @@ namespace {
 enum Stage { S_BUILD = 0, S_REFIT = 1, S_RESTR = 2, S_FRONT = 3, S_ORDER = 4, S_NARROW = 5, S_SCHED = 6, S_TOTAL = 7 };
 constexpr int kEv = 16;   // begin/end event per stage
 constexpr int kLog = 8;   // device log words per frame
-// Learned schedule: the tree and front are rebuilt (BUILD + GENERATE) once the KEEP node tests
-// accumulated above those of the first KEEP after the last rebuild reach this many of those
-// first KEEPs: a rebuild with its front generation costs about as much as eight KEEP frames of
-// a fresh front (measured on the 2M-triangle folding cloth: 6.2 ms against 0.7 ms), so the tree
-// is rebuilt when keeping the degraded one has cost as much as rebuilding it would.
-constexpr double kLearnedRebuildCost = 8.0;
-// A front this close to its capacity is replaced by a fresh one at the next check.
+// Learned schedule: A front this close to its capacity is replaced by a fresh one at the next
+// check.
 constexpr double kLearnedCapacityShare = 0.75;
 constexpr double kLearnedNodeOrderRatio = 1.05;
 constexpr int kLearnedMaxBackoff = 3;
@@ -68,8 +63,13 @@ struct bvhcd_ctx {
     hipStream_t copy_stream = nullptr;  // host uploads, overlapping the steps still running
     hipEvent_t copied[2] = {nullptr, nullptr};
     hipEvent_t consumed[2] = {nullptr, nullptr};  // the last step that read each buffer is done
-    double keep_base = -1.0;     // learned schedule: node tests of the first KEEP after the last rebuild
-    double keep_excess = 0.0;    // node tests above keep_base accumulated since that rebuild
+    // learned schedule: the time of every step is taken with two events; at front-cycle points
+    // the last KEEP frame's time is compared with the first one after the last rebuild
+    hipEvent_t lev[2] = {nullptr, nullptr};
+    double keep_base = -1.0;     // ms of the first KEEP frame after the last rebuild
+    double keep_excess = 0.0;    // ms above keep_base accumulated since that rebuild
+    double rebuild_ms = -1.0;    // ms of the last rebuild frame (BUILD + GENERATE)
+    bool rebuild_timed = false;  // the last step was a rebuild whose time is still to be read
     bool totals_pending = false; // pair totals of the last step not summed yet (done on demand)
     bool step_zeroed = false;    // this step's refit zeroed the step counters for its KEEP
 };
@@ -720,6 +720,8 @@ int bvhcd_destroy(bvhcd_ctx* c) {
     if (!c) return BVHCD_E_INVALID;
     (void)hipStreamSynchronize(c->stream);
     release_upload(c);
+    for (hipEvent_t& e : c->lev)
+        if (e) (void)hipEventDestroy(e);
     learned_destroy(c->learned);
     timing_free(c);
     free_all(c);
@@ -758,6 +760,8 @@ int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V,
     c->built = c->front_valid = c->restr_pending = c->learned_frame = c->repair_pending = false;
     c->keep_base = -1.0;
     c->keep_excess = 0.0;
+    c->rebuild_ms = -1.0;
+    c->rebuild_timed = false;
     c->d.front_version++;
     c->frame = 0;
     c->d.fcur[0] = c->d.fcur[1] = 0;
@@ -831,24 +835,34 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
     const int sched = c->cfg.scheduler;
     plain_schedule(sched, c->frame, c->cfg.bvh_cycle, c->cfg.front_cycle, bvh_op, front_op);
     const bool net = sched == BVHCD_SCHED_LEARNED && learned_ready(c->learned);
-    if (c->built && (net || sched == BVHCD_SCHED_LEARNED_NO_NET)) {
+    if (net || sched == BVHCD_SCHED_LEARNED_NO_NET) {
         // Learned schedule: the periodic global maintenance of the plain schedule (rebuild +
         // regenerate every bvh_cycle frames, UPDATE / REORDER every front_cycle) is replaced by
         // decisions. Every frame refits and keeps the front; every bvh_cycle frames (and on the
         // heuristic's restructure flag) the MLP scores the candidate subtrees, and the chosen
         // ones are rebuilt with an incremental front restructure. At front-cycle points the
-        // KEEP work (node tests) is compared with that of the first KEEP after the last
-        // rebuild: once the excess accumulated since then reaches kLearnedRebuildCost such
-        // KEEPs (or a front nears its capacity), the whole tree is rebuilt and its front
-        // generated anew. The pair set is the same for every schedule.
-        bvh_op = BVHCD_BVH_REFIT;
-        front_op = BVHCD_FRONT_KEEP;
+        // whole tree is rebuilt and its front generated anew when keeping them has cost as much
+        // as a rebuild (below), or a front nears its capacity. The pair set is the same for
+        // every schedule.
+        bvh_op = c->built ? BVHCD_BVH_REFIT : BVHCD_BVH_BUILD;
+        front_op = c->built ? BVHCD_FRONT_KEEP : BVHCD_FRONT_GENERATE;
         const int bc = c->cfg.bvh_cycle > 0 ? c->cfg.bvh_cycle : 12;
         const int fc = c->cfg.front_cycle > 0 ? c->cfg.front_cycle : 6;
-        if (net && (c->frame % ((int64_t)bc << c->learned_backoff) == 0 ||
-                    (c->cfg.enable_restructure && c->restr_pending)))
+        if (!c->lev[0]) {
+            HIPCHK(c, hipEventCreate(&c->lev[0]));
+            HIPCHK(c, hipEventCreate(&c->lev[1]));
+        }
+        if (c->rebuild_timed) {  // the rebuild frame just stepped: its time is the rebuild cost
+            float ms = 0.f;
+            HIPCHK(c, hipEventSynchronize(c->lev[1]));
+            HIPCHK(c, hipEventElapsedTime(&ms, c->lev[0], c->lev[1]));
+            c->rebuild_ms = ms;
+            c->rebuild_timed = false;
+        }
+        if (net && c->built &&
+            (c->frame % ((int64_t)bc << c->learned_backoff) == 0 || (c->cfg.enable_restructure && c->restr_pending)))
             c->learned_frame = true;
-        if (c->frame % fc == 0 && c->last_front == BVHCD_FRONT_KEEP) {
+        if (c->built && c->frame % fc == 0 && c->last_front == BVHCD_FRONT_KEEP) {
             uint64_t visits = 0;
             uint32_t sz[4] = {0, 0, 0, 0}, implicit = 0;
             HIPCHK(c, hipMemcpyAsync(&visits, ctr64(c->d.ctr) + C64_VISITS, sizeof(visits), hipMemcpyDeviceToHost,
@@ -860,12 +874,19 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
             const double fi = (double)sz[c->d.fcur[0]], fe = (double)sz[2 + c->d.fcur[1]];
             // entries KEEP tested (the implicit shared-vertex head takes no node test)
             const double entries = fi + fe - (double)implicit;
-            const double work = std::max((double)visits, 1.0);
-            if (c->keep_base < 0) c->keep_base = work;
-            c->keep_excess += std::max(0.0, work - c->keep_base) * fc;
+            // The previous frame was a KEEP frame: its time, against the first such frame after
+            // the last rebuild, is what keeping the aging tree and front costs per frame. Once
+            // that excess, summed over the frames since the rebuild, reaches what a rebuild frame
+            // costs beyond a KEEP frame, keeping has cost as much as rebuilding would have: the
+            // tree is rebuilt and its front generated anew (self-calibrating, times measured on
+            // the device with the context's events).
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->lev[0], c->lev[1]));
+            if (c->keep_base < 0) c->keep_base = ms;
+            c->keep_excess += std::max(0.0, (double)ms - c->keep_base) * fc;
             const bool full = fi > kLearnedCapacityShare * (double)c->d.front_cap[0] ||
                               fe > kLearnedCapacityShare * (double)c->d.front_cap[1];
-            if (full || c->keep_excess >= kLearnedRebuildCost * c->keep_base) {
+            if (full || (c->rebuild_ms > 0 && c->keep_excess >= c->rebuild_ms - c->keep_base)) {
                 bvh_op = BVHCD_BVH_BUILD;
                 front_op = BVHCD_FRONT_GENERATE;
                 c->learned_frame = false;  // the rebuild replaces this frame's subtree decision
@@ -875,8 +896,15 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
             // wave's records local. The next produced front takes the order that fits.
             c->learned_node_order = (double)visits > kLearnedNodeOrderRatio * entries + 1024.0;
         }
-        if (bvh_op == BVHCD_BVH_BUILD) c->keep_base = -1.0, c->keep_excess = 0.0;
-        return do_step(c, bvh_op, front_op);
+        if (bvh_op == BVHCD_BVH_BUILD) {
+            c->keep_base = -1.0;
+            c->keep_excess = 0.0;
+            c->rebuild_timed = true;
+        }
+        HIPCHK(c, hipEventRecord(c->lev[0], c->stream));
+        const int st = do_step(c, bvh_op, front_op);
+        HIPCHK(c, hipEventRecord(c->lev[1], c->stream));
+        return st;
     }
     if (bvh_op == BVHCD_BVH_REFIT && c->cfg.enable_restructure && c->restr_pending) bvh_op = BVHCD_BVH_RESTRUCTURE;
     return do_step(c, bvh_op, front_op);

@@ -84,7 +84,8 @@ extern "C" {
 #define BVHCD_SCHED_PLAIN   0   /* STATIC_MANDATORY with divergent mark (the reference's active setting):
                                    BUILD every bvh_cycle frames; front GENERATE / UPDATE at front-cycle
                                    starts, REORDER at their ends, KEEP otherwise */
-#define BVHCD_SCHED_LEARNED 1   /* MLP per-subtree restructure choice, heuristic fallback */
+#define BVHCD_SCHED_LEARNED 1   /* MLP per-subtree restructure choice, heuristic fallback; the
+                                   whole tree rebuilt as in BVHCD_SCHED_LEARNED_NO_NET */
 #define BVHCD_SCHED_GENERATE 2  /* GENERATE scheme: BUILD + PURE_BVH_CD each frame */
 #define BVHCD_SCHED_FRONT_GENERATE 3  /* FRONT_GENERATE scheme: BUILD + front GENERATE each frame */
 #define BVHCD_SCHED_STATIC_NODIV 4    /* STATIC_MANDATORY without divergent mark (no REORDER) */
@@ -96,9 +97,10 @@ extern "C" {
 #define BVHCD_SCHED_ARCSIM_PERIODIC 8 /* getSchemeOpt scheme 1: BUILD every bvh_cycle frames, front
                                    GENERATE every front_cycle frames, KEEP otherwise */
 #define BVHCD_SCHED_LEARNED_NO_NET 9  /* the learned schedule with the network off (no reference
-                                   counterpart: the control that isolates what the MLP adds): one
-                                   BUILD, then REFIT + KEEP, UPDATE when KEEP's descents outgrow the
-                                   front; no subtree is ever restructured */
+                                   counterpart: the control that isolates what the MLP adds): REFIT +
+                                   KEEP, and BUILD + GENERATE once the time KEEP frames have lost to
+                                   the aging tree since the last rebuild reaches the measured time of
+                                   a rebuild frame; no subtree is ever restructured */
 
 typedef struct bvhcd_config {
     int64_t max_tris;        /* capacity in triangles (>= 1) */
