@@ -111,6 +111,7 @@ def lib():
         "bvhcd_mlp_get_weights": (C.c_int, [P, P, I64]),
         "bvhcd_mlp_features": (C.c_int, [P, P, P, I64, C.POINTER(I64)]),
         "bvhcd_mlp_forward": (C.c_int, [P, P, I64, P, C.c_int]),
+        "bvhcd_mlp_forward_noalloc": (C.c_int, [P, P, I64, P]),
         "bvhcd_mlp_train_step": (C.c_int, [P, P, P, I64, C.c_float, C.POINTER(C.c_float), C.c_int]),
         "bvhcd_mlp_grad": (C.c_int, [P, P, P, I64, P, C.POINTER(C.c_float), C.c_int]),
         "bvhcd_mlp_set_threshold": (C.c_int, [P, C.c_float]),

@@ -685,7 +685,7 @@ int bvhcd_create(const bvhcd_config* cfg, bvhcd_ctx** out) {
     c->d.skip_adj = cfg->skip_adjacent ? 1 : 0;
     if (c->cfg.bvh_cycle <= 0) c->cfg.bvh_cycle = 12;
     if (c->cfg.front_cycle <= 0) c->cfg.front_cycle = 6;
-    if (hipSetDevice(cfg->device) != hipSuccess) {
+    if (hipSetDevice(cfg->device) != hipSuccess || noalloc_scratch_init(cfg->device) != BVHCD_OK) {
         delete c;
         return BVHCD_E_DEVICE;
     }
@@ -1214,6 +1214,11 @@ int bvhcd_mlp_forward(bvhcd_ctx* c, const float* feats, int64_t rows, float* out
     if (!c || !feats || !out || rows < 0) return BVHCD_E_INVALID;
     if (!c->learned) return fail(c, BVHCD_E_INVALID, "no weights");
     return learned_forward(c->learned, feats, rows, out, where, c->stream, &c->err);
+}
+int bvhcd_mlp_forward_noalloc(bvhcd_ctx* c, const float* feats, int64_t rows, float* out) {
+    if (!c || !feats || !out || rows < 0) return BVHCD_E_INVALID;
+    if (!c->learned) return fail(c, BVHCD_E_INVALID, "no weights");
+    return learned_forward_noalloc(c->learned, c->cfg.device, feats, rows, out, c->stream, &c->err);
 }
 int bvhcd_mlp_train_step(bvhcd_ctx* c, const float* feats, const float* targets, int64_t rows, float lr, float* loss,
                          int where) {

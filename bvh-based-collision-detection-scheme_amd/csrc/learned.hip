@@ -23,6 +23,7 @@
 // and output layer. Training: per-sample fp32 forward/backward of the MSE loss through the
 // same rounding (straight-through), block-reduced gradients, Adam on fp32 master weights.
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "kernels.hpp"
@@ -230,6 +231,65 @@ int learned_forward(LearnedState* L, const float* feats, int64_t rows, float* ou
     if (hipMemcpyAsync(out, L->y, sizeof(float) * rows, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         if (err) *err = "mlp_forward: device error";
+        return BVHCD_E_DEVICE;
+    }
+    return BVHCD_OK;
+}
+
+// ---------------------------------------------------------------- allocation-free inference
+// Staging for learned_forward_noalloc: kNoAllocRows feature rows in, then kNoAllocRows scores
+// out. The array lives in the code object: each GPU holds its own copy from module load until
+// the process ends, at the address hipGetSymbolAddress reports with that GPU current (host code
+// never uses the array's name as a device address). Every call on a GPU shares that GPU's copy,
+// so calls that overlap on one GPU race. The forward kernel writes every score it produces and
+// the rows are copied in whole, so no part of the staging needs clearing between calls.
+constexpr int64_t kNoAllocRows = BVHCD_MLP_NOALLOC_MAX_ROWS;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocRows * (kIn + 1)];
+
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxDevices] = {};
+
+// Looks up the device address of `scratch` on `device` (made current by the caller), once per
+// device.
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return BVHCD_E_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return BVHCD_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BVHCD_E_DEVICE;
+    g_scratch_ptr[device] = static_cast<float*>(p);
+    return BVHCD_OK;
+}
+
+int learned_forward_noalloc(LearnedState* L, int device, const float* feats, int64_t rows, float* out,
+                            hipStream_t s, std::string* err) {
+    if (!L || !L->ready) {
+        if (err) *err = "no weights";
+        return BVHCD_E_INVALID;
+    }
+    if (rows > kNoAllocRows) {
+        if (err) *err = "mlp_forward_noalloc: more rows than BVHCD_MLP_NOALLOC_MAX_ROWS";
+        return BVHCD_E_INVALID;
+    }
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || (cur != device && hipSetDevice(device) != hipSuccess)) {
+        if (err) *err = "mlp_forward_noalloc: cannot select the context's device";
+        return BVHCD_E_DEVICE;
+    }
+    if (noalloc_scratch_init(device) != BVHCD_OK) {
+        if (err) *err = "mlp_forward_noalloc: scratch address lookup failed";
+        return BVHCD_E_DEVICE;
+    }
+    if (rows == 0) return BVHCD_OK;
+    float* x = g_scratch_ptr[device];
+    float* y = x + kNoAllocRows * kIn;
+    if (hipMemcpyAsync(x, feats, sizeof(float) * kIn * rows, hipMemcpyHostToDevice, s) != hipSuccess)
+        return BVHCD_E_DEVICE;
+    launch_forward(L, x, rows, y, s);
+    if (hipMemcpyAsync(out, y, sizeof(float) * rows, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        if (err) *err = "mlp_forward_noalloc: device error";
         return BVHCD_E_DEVICE;
     }
     return BVHCD_OK;

@@ -32,6 +32,12 @@ void update_sa_ref(DevState& d, int32_t n, int all, int32_t frame, hipStream_t s
 // Marks host-given subtree roots for restructure_subtrees (validated: disjoint ranges).
 int mark_given_roots(DevState& d, int32_t n, const int32_t* roots, int64_t k, hipStream_t s, int64_t* nsub,
                      int64_t* nint, int64_t* nleaf, std::string* err);
+// Forward pass over host rows staged through the library's static device scratch (no device
+// allocation); rows <= BVHCD_MLP_NOALLOC_MAX_ROWS. noalloc_scratch_init looks the scratch up on
+// a device (current), once per device.
+int noalloc_scratch_init(int device);
+int learned_forward_noalloc(LearnedState* L, int device, const float* feats, int64_t rows, float* out,
+                            hipStream_t s, std::string* err);
 void learned_set_threshold(LearnedState* L, float t);
 void learned_set_repair_cost(LearnedState* L, float cost);
 }  // namespace bvhcd

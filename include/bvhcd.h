@@ -229,6 +229,14 @@ int  bvhcd_mlp_get_weights(bvhcd_ctx* ctx, float* params, int64_t n);
 int  bvhcd_mlp_features(bvhcd_ctx* ctx, float* feats, int32_t* roots, int64_t cap, int64_t* rows);
 /* Forward pass on the device over host/device feature rows (bf16 weights, fp32 accumulate). */
 int  bvhcd_mlp_forward(bvhcd_ctx* ctx, const float* feats, int64_t rows, float* out, int where);
+/* bvhcd_mlp_forward over host rows without any device allocation: no counterpart in the
+   reference; for callers that cannot allocate device memory (tight serving loops, harnesses
+   that forbid it). Rows are staged through a static device buffer of the library (one per GPU,
+   held until the process ends) instead of buffers the library grows on demand; at most
+   BVHCD_MLP_NOALLOC_MAX_ROWS rows per call (more: BVHCD_E_INVALID). Calls that overlap on one
+   GPU share that buffer and race. */
+#define BVHCD_MLP_NOALLOC_MAX_ROWS 16384
+int  bvhcd_mlp_forward_noalloc(bvhcd_ctx* ctx, const float* feats, int64_t rows, float* out);
 /* One training step (fp32 master weights, bf16 forward, MSE loss, Adam) on device; returns loss. */
 int  bvhcd_mlp_train_step(bvhcd_ctx* ctx, const float* feats, const float* targets, int64_t rows,
                           float lr, float* loss, int where);
