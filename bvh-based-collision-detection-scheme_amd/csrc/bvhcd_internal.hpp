@@ -268,6 +268,7 @@ struct DevState {
     int32_t nq = 0;
     int inter = 0;
     int64_t topo_version = 0;       // host-side: bumped by every topology change (refit preparation)
+    int front_by_query = 0;         // host-side: the last produced fronts are ordered by query leaf
 };
 
 enum Counter : int {

@@ -607,6 +607,7 @@ void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order) {
     front_finish(d, 1, false, s);
     if (order && n >= 2) launch_order_fronts(d, n, s);
     if (!order) launch_query_order(d, n, s);
+    d.front_by_query = order ? 0 : 1;
 }
 
 // ---------------------------------------------------------------- UPDATE (prune / sprout)
@@ -721,6 +722,7 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool orde
     }
     if (order && n >= 2) launch_order_fronts(d, n, s);
     if (!order) launch_query_order(d, n, s);
+    d.front_by_query = order ? 0 : 1;
 }
 
 // ---------------------------------------------------------------- KEEP

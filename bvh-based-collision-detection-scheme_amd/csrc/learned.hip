@@ -70,8 +70,11 @@ struct LearnedState {
     uint32_t* flags = nullptr;
     uint32_t* pos = nullptr;
     uint32_t* total = nullptr;
-    uint32_t* acnt = nullptr;     // front entries per leaf (by the node's leftmost leaf)
+    uint32_t* cand_cnt = nullptr; // front entries per candidate (by the node's leftmost leaf)
+    int32_t* cand_of_leaf = nullptr; // candidate holding each leaf, -1 none
     uint32_t* lcnt = nullptr;     // external-front entries per leaf
+    bool by_query = true;         // the current fronts are ordered by query (else by node)
+    int64_t counted_topo = -1;    // topology version the counts were made on
     int64_t counted_version = -1; // front version the per-leaf counts belong to
     int64_t node_cap = 0;
     float* feats = nullptr;
@@ -108,7 +111,8 @@ void learned_destroy(LearnedState* L) {
     if (!L) return;
     for (void* p : {(void*)L->params, (void*)L->wbf, (void*)L->grad, (void*)L->m, (void*)L->v, (void*)L->loss,
                     (void*)L->x, (void*)L->y, (void*)L->t, (void*)L->roots, (void*)L->flags, (void*)L->pos,
-                    (void*)L->total, (void*)L->feats, (void*)L->acnt, (void*)L->lcnt, (void*)L->gate})
+                    (void*)L->total, (void*)L->feats, (void*)L->cand_cnt, (void*)L->cand_of_leaf, (void*)L->lcnt,
+                    (void*)L->gate})
         if (p) (void)hipFree(p);
     delete L;
 }
@@ -472,54 +476,77 @@ __global__ void k_treelet_compact(int32_t I, const uint32_t* __restrict__ flags,
     if (i < I && flags[i]) roots[pos[i]] = i;
 }
 
-// Adds, for every run of equal keys over consecutive lanes with pred set, the run's length to
-// cnt[key] with one atomic (node-sorted fronts repeat a key over neighbouring entries; on
-// query-sorted ones keys rarely repeat and each lane adds its own). Called by all 64 lanes.
-__device__ inline void run_count(uint32_t* __restrict__ cnt, int32_t key, bool pred) {
-    const int lane = lane_id();
-    const int32_t prev = __shfl(key, lane > 0 ? lane - 1 : 0);
-    const uint64_t on = __ballot(pred);
-    const bool head = pred && (lane == 0 || !((on >> (lane - 1)) & 1ull) || prev != key);
-    const uint64_t stops = __ballot(head) | ~on;  // a run ends before the next head or inactive lane
-    if (head) {
-        const uint64_t after = lane == 63 ? 0ull : stops & (~0ull << (lane + 1));
-        const int end = after ? __ffsll((long long)after) - 1 : 64;
-        atomicAdd(&cnt[key], (uint32_t)(end - lane));
-    }
+// Candidate of every leaf: -1, then candidate c (roots in preorder, compacted) over its leaves.
+__global__ void __launch_bounds__(kBlock) k_cand_of_leaf(const int32_t* __restrict__ roots, DevState d,
+                                                         int32_t* __restrict__ cand_of_leaf) {
+    const int32_t r = roots[blockIdx.x];
+    for (int32_t j = d.int_rx[r] + threadIdx.x; j <= d.int_ry[r]; j += kBlock) cand_of_leaf[j] = blockIdx.x;
 }
 
-// Front entries per leaf: every entry counts for its node's leftmost leaf (acnt), external
-// entries also for their leaf (lcnt); recomputed only when the front changed. The loop runs
-// per wave (all lanes together) so that run_count sees whole waves.
-__global__ void __launch_bounds__(kBlock) k_count_front(DevState d, uint32_t* __restrict__ acnt,
+// Front counts for the features, recomputed only when the front changed:
+//  * cand_cnt[c]: entries whose key leaf (an internal entry's node's leftmost leaf, an external
+//    entry's leaf) lies in candidate c;
+//  * lcnt[j]: external entries at leaf j.
+// Each workgroup takes one contiguous piece of the fronts and counts in LDS: candidates in a
+// table of all of them (when they fit), leaves in a window starting at the piece's first query
+// (query-ordered fronts: a query's external leaves follow it closely in Morton order) or first
+// leaf (node-ordered fronts); keys outside go to global atomics. The tables are then added to
+// the global counts once per non-zero bin.
+constexpr int kCandLds = 4096, kLeafWin = 4096;
+__global__ void __launch_bounds__(kBlock) k_count_front(DevState d, const int32_t* __restrict__ cand_of_leaf,
+                                                        int32_t ncand, int by_query, uint32_t* __restrict__ cand_cnt,
                                                         uint32_t* __restrict__ lcnt) {
+    __shared__ uint32_t hc[kCandLds];
+    __shared__ uint32_t hl[kLeafWin];
+    __shared__ int32_t wbase;
+    const bool lds_cand = ncand <= kCandLds;
+    for (int i = threadIdx.x; i < kCandLds; i += kBlock) hc[i] = 0u;
+    for (int i = threadIdx.x; i < kLeafWin; i += kBlock) hl[i] = 0u;
     const int c0 = d.fcur[0], c1 = d.fcur[1];
     const uint32_t ni = (uint32_t)min((int64_t)d.fsize[c0], d.front_cap[0]);
     const uint32_t ne = (uint32_t)min((int64_t)d.fsize[2 + c1], d.front_cap[1]);
     const uint32_t total = ni + ne;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < total; base += stride) {
-        const uint32_t k = base + (uint32_t)lane_id();
-        const bool valid = k < total;
-        const bool ext = valid && k >= ni;
-        int32_t leaf = -1;
-        if (valid) leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
-        run_count(acnt, leaf, valid);
-        run_count(lcnt, leaf, ext);
+    const uint32_t piece = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t k0 = min(blockIdx.x * piece, total), k1 = min(k0 + piece, total);
+    if (threadIdx.x == 0) {
+        const uint32_t e0 = max(k0, ni);  // first external entry of the piece
+        wbase = e0 < k1 ? (by_query ? d.fquery[1][c1][e0 - ni] + 1 : (int32_t)d.fnode[1][c1][e0 - ni]) : 0;
     }
+    __syncthreads();
+    const int32_t wb = wbase;
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
+        const bool ext = k >= ni;
+        const int32_t leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
+        const int32_t c = cand_of_leaf[leaf];
+        if (c >= 0) {
+            if (lds_cand) atomicAdd(&hc[c], 1u);
+            else atomicAdd(&cand_cnt[c], 1u);
+        }
+        if (ext) {
+            const uint32_t o = (uint32_t)(leaf - wb);
+            if (o < (uint32_t)kLeafWin) atomicAdd(&hl[o], 1u);
+            else atomicAdd(&lcnt[leaf], 1u);
+        }
+    }
+    __syncthreads();
+    if (lds_cand)
+        for (int i = threadIdx.x; i < ncand; i += kBlock)
+            if (hc[i]) atomicAdd(&cand_cnt[i], hc[i]);
+    for (int i = threadIdx.x; i < kLeafWin; i += kBlock)
+        if (hl[i]) atomicAdd(&lcnt[wb + i], hl[i]);
 }
 
 // One workgroup per treelet.
 __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__ roots, DevState d, int32_t frame,
-                                                     float frame_scale, const uint32_t* __restrict__ acnt,
+                                                     float frame_scale, const uint32_t* __restrict__ cand_cnt,
                                                      const uint32_t* __restrict__ lcnt, float* __restrict__ feats) {
-    __shared__ float red[5][kBlock];
+    __shared__ float red[4][kBlock];
     __shared__ uint32_t hist[10];
     const int32_t r = roots[blockIdx.x];
     const int32_t a = d.int_rx[r], b = d.int_ry[r], m = b - a + 1;
     if (threadIdx.x < 10) hist[threadIdx.x] = 0;
     __syncthreads();
-    float sa_now = 0.f, sa_ref = 0.f, leaf_exp = 0.f, age = 0.f, entries = 0.f;
+    float sa_now = 0.f, sa_ref = 0.f, leaf_exp = 0.f, age = 0.f;
     for (int32_t i = r + threadIdx.x; i < r + m - 1; i += kBlock) {
         sa_now += rec_area(d.inode[i]);
         sa_ref += d.sa_ref[i];
@@ -528,7 +555,6 @@ __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__
         const float now = rec_area(d.leaf[j]), ref = d.sa_ref_leaf[j];
         leaf_exp += logf((now + 1e-12f) / (ref + 1e-12f));
         age += (float)(frame - d.built_frame[j]);
-        entries += (float)acnt[j];
         const uint32_t c = lcnt[j];
         const int bin = c >= 511u ? 9 : 31 - __clz(c + 1u);  // floor(log2(1 + c)), c + 1 in [1, 511]
         atomicAdd(&hist[bin], 1u);
@@ -537,18 +563,17 @@ __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__
     red[1][threadIdx.x] = sa_ref;
     red[2][threadIdx.x] = leaf_exp;
     red[3][threadIdx.x] = age;
-    red[4][threadIdx.x] = entries;
     __syncthreads();
     for (int off = kBlock / 2; off > 0; off >>= 1) {
         if (threadIdx.x < off) {
-            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         float* f = feats + (int64_t)blockIdx.x * kIn;
         f[0] = logf((red[0][0] + 1e-12f) / (red[1][0] + 1e-12f));
-        f[1] = red[4][0] / (float)m;
+        f[1] = (float)cand_cnt[blockIdx.x] / (float)m;
         f[2] = red[2][0] / (float)m;
         int depth = 0;
         for (int32_t p = d.int_par[r]; p >= 0 && depth < 4096; p = d.int_par[p]) ++depth;
@@ -566,11 +591,12 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
     if (I < 1) return BVHCD_OK;
     if (L->node_cap < n) {
         for (void* p : {(void*)L->roots, (void*)L->flags, (void*)L->pos, (void*)L->total, (void*)L->feats,
-                        (void*)L->acnt, (void*)L->lcnt})
+                        (void*)L->cand_cnt, (void*)L->cand_of_leaf, (void*)L->lcnt})
             if (p) (void)hipFree(p);
         L->roots = nullptr; L->flags = nullptr; L->pos = nullptr; L->total = nullptr; L->feats = nullptr;
-        L->acnt = nullptr; L->lcnt = nullptr;
-        if (hipMalloc(&L->acnt, sizeof(uint32_t) * n) != hipSuccess ||
+        L->cand_cnt = nullptr; L->cand_of_leaf = nullptr; L->lcnt = nullptr;
+        if (hipMalloc(&L->cand_cnt, sizeof(uint32_t) * n) != hipSuccess ||
+            hipMalloc(&L->cand_of_leaf, sizeof(int32_t) * n) != hipSuccess ||
             hipMalloc(&L->lcnt, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->roots, sizeof(int32_t) * n) != hipSuccess ||
             hipMalloc(&L->flags, sizeof(uint32_t) * n) != hipSuccess ||
@@ -584,19 +610,25 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
     hipLaunchKernelGGL(k_treelet_flags, dim3(grid_for(I)), dim3(kBlock), 0, s, I, d, L->flags);
     launch_exscan(d, L->flags, L->pos, I, L->total, s);
     hipLaunchKernelGGL(k_treelet_compact, dim3(grid_for(I)), dim3(kBlock), 0, s, I, L->flags, L->pos, L->roots);
-    if (L->counted_version != d.front_version) {  // per-leaf front counts of this front
-        hipMemsetAsync(L->acnt, 0, sizeof(uint32_t) * (size_t)n, s);
-        hipMemsetAsync(L->lcnt, 0, sizeof(uint32_t) * (size_t)n, s);
-        hipLaunchKernelGGL(k_count_front, dim3(4096), dim3(kBlock), 0, s, d, L->acnt, L->lcnt);
-        L->counted_version = d.front_version;
-    }
     uint32_t cnt = 0;
     if (hipMemcpyAsync(&cnt, L->total, sizeof(cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return BVHCD_E_DEVICE;
+    // front counts of this front on these candidates (the candidates change with the topology)
+    if (cnt > 0 && (L->counted_version != d.front_version || L->counted_topo != d.topo_version)) {
+        L->by_query = d.front_by_query != 0;
+        fill_i32(L->cand_of_leaf, n, -1, s);
+        hipLaunchKernelGGL(k_cand_of_leaf, dim3(cnt), dim3(kBlock), 0, s, L->roots, d, L->cand_of_leaf);
+        hipMemsetAsync(L->cand_cnt, 0, sizeof(uint32_t) * (size_t)cnt, s);
+        hipMemsetAsync(L->lcnt, 0, sizeof(uint32_t) * (size_t)n, s);
+        hipLaunchKernelGGL(k_count_front, dim3(1024), dim3(kBlock), 0, s, d, L->cand_of_leaf, (int32_t)cnt,
+                           L->by_query ? 1 : 0, L->cand_cnt, L->lcnt);
+        L->counted_version = d.front_version;
+        L->counted_topo = d.topo_version;
+    }
     if (cnt > 0)
         hipLaunchKernelGGL(k_features, dim3(cnt), dim3(kBlock), 0, s, L->roots, d, (int32_t)frame, 1.0f / 12.0f,
-                           L->acnt, L->lcnt, L->feats);
+                           L->cand_cnt, L->lcnt, L->feats);
     *rows = cnt;
     return hipGetLastError() == hipSuccess ? BVHCD_OK : BVHCD_E_DEVICE;
 }

@@ -89,7 +89,11 @@ def trace_sequence(pkg, seed, frames, fork_every=6):
     verts = [sc.frame(f) for f in range(frames)]
     X, Y = [], []
     for fork in range(fork_every, frames, fork_every):
-        ctxs = [pkg.CollisionContext(sc.num_tris, sc.num_verts) for _ in range(2)]
+        # a front generated on a tree up to a rebuild cycle old can be several times larger than
+        # a fresh one on the folding cloth: room for it
+        cap = 128 * sc.num_tris
+        ctxs = [pkg.CollisionContext(sc.num_tris, sc.num_verts, int_front_cap=cap, ext_front_cap=cap, pair_cap=cap // 2)
+                for _ in range(2)]
         for c in ctxs:
             c.set_topology(faces, sc.num_verts)
         for f in range(fork):  # both contexts on the plain trajectory up to frame fork - 1

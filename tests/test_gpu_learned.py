@@ -248,9 +248,10 @@ def test_front_restructure_matches_oracle(pkg, oracle, max_leaves, every):
     ctx.close()
 
 
-def _timed_run(pkg, sc, frames, weights=None, threshold=0.0, repair_cost=None):
-    ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts,
-                               scheduler=pkg.SCHED_LEARNED if weights is not None else pkg.SCHED_PLAIN)
+def _timed_run(pkg, sc, frames, weights=None, threshold=0.0, repair_cost=None, scheduler=None):
+    if scheduler is None:
+        scheduler = pkg.SCHED_LEARNED if weights is not None else pkg.SCHED_PLAIN
+    ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts, scheduler=scheduler)
     ctx.set_topology(sc.faces(), sc.num_verts)
     if weights is not None:
         ctx.mlp_set_weights(weights)
@@ -269,9 +270,12 @@ def _timed_run(pkg, sc, frames, weights=None, threshold=0.0, repair_cost=None):
 
 
 def test_trained_scheduler_gains_over_plain(pkg):
-    """The committed trained network on a sequence it never saw (256^2 cloth, seed 977): the
-    learned schedule is faster than the plain one with identical pair sets on every frame, and
-    faster than the same schedule told to rebuild every candidate (what its decisions avoid)."""
+    """The committed trained network on a folding-cloth sequence it never saw (256^2 cloth, seed
+    977): the learned schedule is faster than the plain one with identical pair sets on every
+    frame, and faster than the same schedule told to rebuild every candidate. What the network
+    itself adds is measured against the schedule with the network off (SCHED_LEARNED_NO_NET):
+    on this workload it finds no subtree worth rebuilding, so the two must be within the cost of
+    its decisions (10 %) of each other, and the gain over plain comes from the schedule."""
     import os
     z = np.load(os.path.join(os.path.dirname(pkg.__file__), "mlp_scheduler.npz"), allow_pickle=False)
     params, thr = z["params"], float(z["threshold"])
@@ -279,13 +283,15 @@ def test_trained_scheduler_gains_over_plain(pkg):
     frames = [sc.frame(f) for f in range(sc.frames)]
     t_plain, d_plain, _ = _timed_run(pkg, sc, frames)
     t_learn, d_learn, ops = _timed_run(pkg, sc, frames, params, thr)
+    t_none, d_none, _ = _timed_run(pkg, sc, frames, scheduler=pkg.SCHED_LEARNED_NO_NET)
     t_all, d_all, _ = _timed_run(pkg, sc, frames, params, -1e30, repair_cost=0.0)
-    print(f"[gpu] plain {t_plain:.1f} ms, learned {t_learn:.1f} ms ({t_plain / t_learn:.3f}x), "
-          f"rebuild-all {t_all:.1f} ms, learned restructure frames {(ops[:, 0] == pkg.BVH_RESTRUCTURE).sum()}",
-          flush=True)
-    assert d_learn == d_plain and d_all == d_plain
+    print(f"[gpu] plain {t_plain:.1f} ms, learned {t_learn:.1f} ms ({t_plain / t_learn:.3f}x), network off "
+          f"{t_none:.1f} ms, rebuild-all {t_all:.1f} ms, learned restructure frames "
+          f"{(ops[:, 0] == pkg.BVH_RESTRUCTURE).sum()}", flush=True)
+    assert d_learn == d_plain and d_all == d_plain and d_none == d_plain
     assert t_learn < 0.9 * t_plain
     assert t_learn < t_all
+    assert t_learn < 1.1 * t_none
 
 
 def test_learned_schedule_keeps_pairs(pkg, oracle):
