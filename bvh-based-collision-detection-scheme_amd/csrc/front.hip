@@ -726,6 +726,110 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool orde
 }
 
 // ---------------------------------------------------------------- KEEP
+// Descents below internal front entries that now overlap their query are shared by the wave:
+// instead of one lane walking a subtree while the others wait, the (query, node) pairs still to
+// be tested go on a per-wave LDS stack and every lane takes one per round (a round is one
+// dependent load: the node's record, and for an overlapping internal node its children). The
+// nodes tested and the pairs found are those of the lane's own walk; a stack that is full
+// leaves the lane to walk that subtree itself.
+constexpr uint32_t kDescStack = 256;
+struct DescStack {
+    int2 item[kWaves][kDescStack];  // (query, node << 1 | leaf)
+    uint32_t top[kWaves];
+};
+
+// Pushes (q, node enc) for the lanes with pred set; returns false for a lane whose item did not
+// fit (it must then test that node itself). Called by all active lanes of the wave.
+__device__ inline bool desc_push(volatile int2* stk, volatile uint32_t* top, bool pred, int32_t q, int32_t enc) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return true;
+    const uint32_t t = *top;
+    const uint32_t rank = (uint32_t)__popcll(m & lanes_below());
+    const bool fits = t + rank < kDescStack;
+    if (pred && fits) {
+        stk[t + rank].x = q;
+        stk[t + rank].y = enc;
+    }
+    const uint32_t pushed = (uint32_t)__popcll(__ballot(pred && fits));
+    if (lane_id() == __ffsll((long long)__ballot(1)) - 1) *top = t + pushed;
+    return fits || !pred;
+}
+
+// Tests node `enc` against q's box: an overlapping leaf is a pair, an overlapping internal node
+// pushes its two children (or, without room, is walked by this lane). Returns the node tests.
+__device__ inline uint32_t desc_test(const DevState& d, const Out& out, volatile int2* stk, volatile uint32_t* top,
+                                     bool active, int32_t q, int32_t enc) {
+    uint32_t visits = 0;
+    bool leaf_hit = false, int_hit = false;
+    int32_t nd = enc >> 1;
+    Box qb{};
+    float4 hi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active) {
+        const float4 qlo = d.qleaf[q].lo, qhi = d.qleaf[q].hi;
+        qb = rec_box(qlo, qhi);
+        const NodeRec* r = (enc & 1) ? &d.leaf[nd] : &d.inode[nd];
+        const float4 lo = r->lo;
+        hi = r->hi;
+        ++visits;
+        const bool hit = box_overlap(qb, lo, hi);
+        leaf_hit = hit && (enc & 1);
+        int_hit = hit && !(enc & 1);
+    }
+    bool emit = leaf_hit;
+    if (out.skip_adj && emit) emit = !faces_share_vertex(out.lfaces, q, nd);
+    stage_push(out.bp, out.ovf, out.sb[2], out.sf[2], out.scap, emit, make_int2(q, nd));
+    int4 tp = make_int4(0, 0, 0, 0);
+    if (int_hit) tp = d.itopo[nd];
+    const int32_t lc = (tp.x << 1) | (tp.w & 1), rc = (tp.y << 1) | ((tp.w >> 1) & 1);
+    const bool ok_l = desc_push(stk, top, int_hit, q, lc);
+    const bool ok_r = desc_push(stk, top, int_hit, q, rc);
+    if (int_hit && !(ok_l && ok_r)) {  // no room: walk what did not fit here
+        if (!ok_l && !ok_r) {
+            visits += stackless<false>(d, out, q, qb, lc, f2i(hi.w));
+        } else if (!ok_r) {
+            visits += stackless<false>(d, out, q, qb, rc, f2i(hi.w));
+        } else {  // the left child alone: from it to where its subtree ends
+            const int32_t stop = (lc & 1) ? d.leaf_lca[(lc >> 1) + 1] : f2i(d.inode[lc >> 1].hi.w);
+            visits += stackless<false>(d, out, q, qb, lc, stop);
+        }
+    }
+    return visits;
+}
+
+// Empties the wave's stack, one item per lane and round. Called by all active lanes.
+__device__ inline uint32_t desc_drain(const DevState& d, const Out& out, volatile int2* stk, volatile uint32_t* top) {
+    uint32_t visits = 0;
+    const int lane = lane_id();
+    const uint32_t nact = (uint32_t)__popcll(__ballot(1));
+    const uint32_t rank = (uint32_t)__popcll(__ballot(1) & lanes_below());
+    // every node is tested at most once per query walk; the bound only guards against a
+    // corrupted tree (flag 16, reported as a device error, instead of a hang)
+    const int64_t limit = 2 * d.maxT + 16;
+    for (int64_t rounds = 0;; ++rounds) {
+        const uint32_t t = *top;
+        if (t == 0) break;
+        if (rounds > limit) {
+            if (lane == __ffsll((long long)__ballot(1)) - 1) {
+                atomicOr(&d.ctr[C_OVF], 16u);
+                *top = 0;
+            }
+            break;
+        }
+        const uint32_t take = min(t, nact);
+        const bool active = rank < take;
+        int32_t q = 0, enc = 0;
+        if (active) {
+            q = stk[t - 1 - rank].x;
+            enc = stk[t - 1 - rank].y;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane == __ffsll((long long)__ballot(1)) - 1) *top = t - take;
+        __builtin_amdgcn_wave_barrier();
+        visits += desc_test(d, out, stk, top, active, q, enc);
+    }
+    return visits;
+}
+
 // KEEP over the current front, U entries per thread per loop iteration with all their loads
 // issued together: the kernel is bound by the latency of dependent loads (entry, then the two
 // node records), so more loads in flight per wave is what speeds it up.
@@ -735,7 +839,12 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool orde
 template <int U>
 __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int implicit) {
     __shared__ StagesBP st;
+    __shared__ DescStack ds;
     stages_init(st);
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) ds.top[w] = 0;
+    volatile int2* stk = ds.item[w];
+    volatile uint32_t* stop = &ds.top[w];
     const Out out = make_out(st, d);
     uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]);
     // with skip_adj the head's pairs are filtered out anyway: skipped and not counted
@@ -796,14 +905,22 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int imp
             const Box qb = rec_box(qlo[u], qhi[u]);
             const bool hit = q[u] >= 0 && box_overlap(qb, lo[u], hi[u]);
             visits += q[u] >= 0 ? 1u : 0u;
-            if (internal && hit) {  // the node now overlaps: descend below it
-                const int32_t start = (d.int_lc[nd[u]] << 1) | (int32_t)(d.int_mark[nd[u]] & 1u);
+            // the node now overlaps: its children go on the wave's stack
+            const bool desc = internal && hit;
+            int4 tp = make_int4(0, 0, 0, 0);
+            if (desc) tp = d.itopo[nd[u]];
+            const int32_t lc = (tp.x << 1) | (tp.w & 1), rc = (tp.y << 1) | ((tp.w >> 1) & 1);
+            const bool ok_l = desc_push(stk, stop, desc, q[u], lc);
+            const bool ok_r = desc_push(stk, stop, desc, q[u], rc);
+            if (desc && !(ok_l && ok_r)) {  // no room on the stack: this lane walks the subtree
+                const int32_t start = ok_l ? rc : lc;
                 visits += stackless<false>(d, out, q[u], qb, start, f2i(hi[u].w));
             }
             bool emit = hit && !internal;
             if (out.skip_adj && emit) emit = !faces_share_vertex(out.lfaces, q[u], nd[u]);
             stage_push(out.bp, out.ovf, out.sb[2], out.sf[2], out.scap, emit, make_int2(q[u], nd[u]));
         }
+        visits += desc_drain(d, out, stk, stop);
     }
     stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
     add_visits(d, visits);
