@@ -155,10 +155,14 @@ STAGE_KERNEL = {0: "build kernels", 1: "k_refit+k_refit_top", 2: "restructure ke
 def compulsory_bytes(stage, op, T, V, by, cnt):
     """Bytes a frame's stage must move at least once: KEEP reads each tested front entry (8 B)
     and at most every node and leaf record once (32 B each) and writes its listed pairs (8 B);
-    the other stages' byte formulas already count every access once."""
+    the narrow phase reads each listed pair (8 B), at most every face (12 B) and vertex (12 B)
+    once, and writes its listed hits (8 B); the other stages' byte formulas already count every
+    access once."""
     if stage == 3 and op == 4:
         tested = cnt[3] + cnt[4] - cnt[6]
         return 8.0 * tested + 32.0 * (2 * T - 1) + 8.0 * (cnt[1] - cnt[6])
+    if stage == 5:
+        return min(by, 8.0 * (cnt[1] - cnt[6]) + 12.0 * T + 12.0 * V + 8.0 * (cnt[2] - cnt[7]))
     return by
 
 

@@ -41,6 +41,14 @@ constexpr uint32_t kStage3 = 256;
 
 __device__ inline uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
 
+// Per-wave LDS words shared between lanes are reached through pointers typed as LDS: their
+// accesses are then ds_read / ds_write. Through generic pointers they would be flat accesses,
+// which also count against the vector-memory counter, so waiting for one of them would wait for
+// every global load in flight as well.
+#define BVHCD_LDS __attribute__((address_space(3)))
+typedef volatile BVHCD_LDS int2 lds_int2;
+typedef volatile BVHCD_LDS uint32_t lds_u32;
+
 // ---------------------------------------------------------------- output sinks
 // Front output (SoA node / query), segmented like the pair lists: kPairSegs counters
 // kCntStride words apart, segment g at [g * seg_cap, g * seg_cap + count). The producing kernel
@@ -69,7 +77,7 @@ struct SegPairSink {  // segmented pair list: kPairSegs counters kCntStride word
 // The staging word pair of a wave and list: sfill[0] = staged entries, sfill[1] = rotor that
 // picks the segment of the next flush of a segmented list.
 template <class Sink>
-__device__ void stage_flush(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t*, uint32_t f,
+__device__ void stage_flush(const Sink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u32*, uint32_t f,
                             uint64_t active) {
     const int lane = lane_id();
     const int leader = __ffsll((long long)active) - 1;
@@ -95,7 +103,7 @@ __device__ inline uint32_t rotor_start() {
     const uint32_t b = blockIdx.x, w = threadIdx.x >> 6;
     return (b / kXcds) * kWaves + w + (b % kXcds) * kXcds;
 }
-__device__ inline uint32_t seg_reserve(uint32_t* cnt, volatile uint32_t* sfill, uint32_t f, uint64_t active,
+__device__ inline uint32_t seg_reserve(uint32_t* cnt, lds_u32* sfill, uint32_t f, uint64_t active,
                                        uint32_t& g) {
     const int leader = __ffsll((long long)active) - 1;
     uint32_t base = 0;
@@ -108,7 +116,7 @@ __device__ inline uint32_t seg_reserve(uint32_t* cnt, volatile uint32_t* sfill, 
     g = __shfl(g, leader);
     return __shfl(base, leader);
 }
-__device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
+__device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u32* sfill,
                                    uint32_t f, uint64_t active) {
     uint32_t g;
     const uint32_t base = seg_reserve(sk.cnt, sfill, f, active, g);
@@ -123,7 +131,7 @@ __device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, volatil
     }
     if (over) atomicOr(ovf, sk.ovf_bit);
 }
-__device__ inline void stage_flush(const FrontSink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
+__device__ inline void stage_flush(const FrontSink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u32* sfill,
                                    uint32_t f, uint64_t active) {
     uint32_t g;
     const uint32_t base = seg_reserve(sk.cnt, sfill, f, active, g);
@@ -145,7 +153,7 @@ __device__ inline void stage_flush(const FrontSink& sk, uint32_t* ovf, volatile 
 
 // Called by the currently active lanes of a wave; `pred` selects the lanes that emit `v`.
 template <class Sink>
-__device__ inline void stage_push(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill,
+__device__ inline void stage_push(const Sink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u32* sfill,
                                   uint32_t cap, bool pred, int2 v) {
     const uint64_t active = __ballot(1);
     const uint64_t m = __ballot(pred);
@@ -165,7 +173,7 @@ __device__ inline void stage_push(const Sink& sk, uint32_t* ovf, volatile int2* 
 }
 
 template <class Sink>
-__device__ inline void stage_final(const Sink& sk, uint32_t* ovf, volatile int2* sbuf, volatile uint32_t* sfill) {
+__device__ inline void stage_final(const Sink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u32* sfill) {
     const uint32_t f = *sfill;
     if (f) stage_flush(sk, ovf, sbuf, sfill, f, __ballot(1));
     if (lane_id() == 0) *sfill = 0;
@@ -189,8 +197,8 @@ struct Out {
     FrontSink fi, fe;
     SegPairSink bp;
     uint32_t* ovf;
-    volatile int2* sb[3];
-    volatile uint32_t* sf[3];
+    lds_int2* sb[3];
+    lds_u32* sf[3];
     uint32_t scap;  // staging entries per wave and list
     const int32_t* lfaces;
     int skip_adj;   // drop pairs of triangles sharing a vertex
@@ -240,8 +248,8 @@ __device__ inline Out make_out(StagesBP& st, const DevState& d) {
     const int w = threadIdx.x >> 6;
     o.sb[0] = o.sb[1] = nullptr;
     o.sf[0] = o.sf[1] = nullptr;
-    o.sb[2] = st.buf[w];
-    o.sf[2] = st.fill[w];
+    o.sb[2] = (lds_int2*)st.buf[w];
+    o.sf[2] = (lds_u32*)st.fill[w];
     o.scap = kStage;
     o.lfaces = d.lfaces;
     o.skip_adj = d.skip_adj;
@@ -256,8 +264,8 @@ __device__ inline Out make_out(Stages& st, const FrontSink& fi, const FrontSink&
     o.ovf = &d.ctr[C_OVF];
     const int w = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
-        o.sb[k] = st.buf[k][w];
-        o.sf[k] = st.fill[k][w];
+        o.sb[k] = (lds_int2*)st.buf[k][w];
+        o.sf[k] = (lds_u32*)st.fill[k][w];
     }
     o.scap = kStage3;
     o.lfaces = d.lfaces;
@@ -740,7 +748,7 @@ struct DescStack {
 
 // Pushes (q, node enc) for the lanes with pred set; returns false for a lane whose item did not
 // fit (it must then test that node itself). Called by all active lanes of the wave.
-__device__ inline bool desc_push(volatile int2* stk, volatile uint32_t* top, bool pred, int32_t q, int32_t enc) {
+__device__ inline bool desc_push(lds_int2* stk, lds_u32* top, bool pred, int32_t q, int32_t enc) {
     const uint64_t m = __ballot(pred);
     if (m == 0) return true;
     const uint32_t t = *top;
@@ -757,7 +765,7 @@ __device__ inline bool desc_push(volatile int2* stk, volatile uint32_t* top, boo
 
 // Tests node `enc` against q's box: an overlapping leaf is a pair, an overlapping internal node
 // pushes its two children (or, without room, is walked by this lane). Returns the node tests.
-__device__ inline uint32_t desc_test(const DevState& d, const Out& out, volatile int2* stk, volatile uint32_t* top,
+__device__ inline uint32_t desc_test(const DevState& d, const Out& out, lds_int2* stk, lds_u32* top,
                                      bool active, int32_t q, int32_t enc) {
     uint32_t visits = 0;
     bool leaf_hit = false, int_hit = false;
@@ -797,7 +805,7 @@ __device__ inline uint32_t desc_test(const DevState& d, const Out& out, volatile
 }
 
 // Empties the wave's stack, one item per lane and round. Called by all active lanes.
-__device__ inline uint32_t desc_drain(const DevState& d, const Out& out, volatile int2* stk, volatile uint32_t* top) {
+__device__ inline uint32_t desc_drain(const DevState& d, const Out& out, lds_int2* stk, lds_u32* top) {
     uint32_t visits = 0;
     const int lane = lane_id();
     const uint32_t nact = (uint32_t)__popcll(__ballot(1));
@@ -843,8 +851,8 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int imp
     stages_init(st);
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) ds.top[w] = 0;
-    volatile int2* stk = ds.item[w];
-    volatile uint32_t* stop = &ds.top[w];
+    lds_int2* stk = (lds_int2*)ds.item[w];
+    lds_u32* stop = (lds_u32*)&ds.top[w];
     const Out out = make_out(st, d);
     uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]);
     // with skip_adj the head's pairs are filtered out anyway: skipped and not counted
@@ -1248,6 +1256,8 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
         sfill[w][1] = rotor_start();
     }
     const SegPairSink sink = pair_segments(d, 1, 8u);
+    lds_int2* const sb = (lds_int2*)sbuf[w];
+    lds_u32* const sf = (lds_u32*)sfill[w];
     const uint32_t npairs = seg_prefix(d.pcnt, (uint32_t)d.pair_seg_cap, pre);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += gridDim.x * blockDim.x) {
         const int2 lv = d.bp[seg_slot(pre, k, d.pair_seg_cap)];  // leaves
@@ -1256,7 +1266,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
             const int32_t* fb = d.qlfaces + 3 * (int64_t)lv.x;
             const bool hit = sat_tri_tri(ld3(d.verts, fa[0]), ld3(d.verts, fa[1]), ld3(d.verts, fa[2]),
                                          ld3(d.qverts, fb[0]), ld3(d.qverts, fb[1]), ld3(d.qverts, fb[2]));
-            stage_push(sink, &d.ctr[C_OVF], sbuf[w], sfill[w], kStage, hit,
+            stage_push(sink, &d.ctr[C_OVF], sb, sf, kStage, hit,
                        make_int2(d.prim_idx[lv.y], d.qprim[lv.x]));
             continue;
         }
@@ -1282,9 +1292,9 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
             hit = sat_tri_tri(ld3(d.verts, a0), ld3(d.verts, a1), ld3(d.verts, a2), ld3(d.verts, b0),
                               ld3(d.verts, b1), ld3(d.verts, b2));
         }
-        stage_push(sink, &d.ctr[C_OVF], sbuf[w], sfill[w], kStage, hit, pr);
+        stage_push(sink, &d.ctr[C_OVF], sb, sf, kStage, hit, pr);
     }
-    stage_final(sink, &d.ctr[C_OVF], sbuf[w], sfill[w]);
+    stage_final(sink, &d.ctr[C_OVF], sb, sf);
 }
 
 void launch_narrow(DevState& d, hipStream_t s, bool counters_zeroed) {
