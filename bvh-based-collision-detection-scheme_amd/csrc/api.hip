@@ -27,7 +27,6 @@ constexpr int kLog = 8;   // device log words per frame
 // Learned schedule: A front this close to its capacity is replaced by a fresh one at the next
 // check.
 constexpr double kLearnedCapacityShare = 0.75;
-constexpr double kLearnedNodeOrderRatio = 1.05;
 constexpr int kLearnedMaxBackoff = 3;
 }  // namespace
 
@@ -41,7 +40,6 @@ struct bvhcd_ctx {
     bool built = false, front_valid = false, restr_pending = false;
     bool learned_frame = false;  // set by the learned scheduler for its decision frames
     int learned_backoff = 0;     // decision interval = bvh_cycle << backoff (rejected selections)
-    bool learned_node_order = false;  // produce node-ordered fronts (KEEP descends often)
     bool repair_pending = false; // front classified for the incremental front restructure
     uint32_t repair_tasks = 0;   // its (query, subtree) walks
     int64_t frame = 0;
@@ -232,6 +230,15 @@ int allocate(bvhcd_ctx* c) {
     ok &= dalloc(c, d.fcnt, 2 * kPairSegs * kCntStride);
     ok &= dalloc(c, d.adj_cnt, (size_t)(d.front_cap[1] / kAdjTile + 2));
     ok &= dalloc(c, d.adj_off, (size_t)(d.front_cap[1] / kAdjTile + 2));
+    // wave-ordered GENERATE: one chunk record per staging flush (every flush but a wave's last
+    // holds more than 128 entries; the last ones may all land in one segment), per-wave totals
+    d.wave_cap = (int64_t)grid_for(std::max<int64_t>(T - 1, 1)) * (kBlock / 64);
+    d.chunk_seg_cap = std::max(d.front_cap[0], d.front_cap[1]) / kPairSegs / 128 + d.wave_cap + 16;
+    for (int k = 0; k < 2; ++k) {
+        ok &= dalloc(c, d.fchunk[k], (size_t)(kPairSegs * d.chunk_seg_cap));
+        ok &= dalloc(c, d.wave_tot[k], (size_t)d.wave_cap);
+    }
+    ok &= dalloc(c, d.wave_base, (size_t)d.wave_cap);
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
     HIPCHK(c, hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(d.pcnt, 0, 2 * kPairSegs * kCntStride * sizeof(uint32_t), c->stream));
@@ -493,12 +500,13 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     if (front_op != BVHCD_FRONT_PURE_BVH_CD && !c->front_valid) front_op = BVHCD_FRONT_GENERATE;
     c->restr_pending = false;  // the quality flag lives for one frame
     // The learned schedule reads neither the node-id ordering's bucket offsets nor the
-    // front-quality heuristic: its fronts are sorted by query leaf (neighbouring queries
-    // together, the best locality for KEEP without descents), or by node id once KEEP has been
-    // seen descending often (lanes then descend the same subtrees).
-    const bool plain_order = !is_learned(c->cfg.scheduler);
-    const bool quality = c->cfg.enable_restructure != 0 && plain_order;
-    const bool order = plain_order || c->learned_node_order;  // node order, else query order
+    // front-quality heuristic: its fronts are ordered by query leaf (neighbouring queries
+    // together, the best locality for KEEP and for the narrow phase over KEEP's pairs), which
+    // its GENERATE produces without a global sort. (Node-ordered fronts made KEEP's descents
+    // about 4 % faster on the folded 2M-triangle cloth but cost a 3-pass radix sort, 2.8 ms
+    // more per rebuild.)
+    const bool order = !is_learned(c->cfg.scheduler);  // node order, else query order
+    const bool quality = c->cfg.enable_restructure != 0 && order;
     if (front_op != BVHCD_FRONT_KEEP && front_op != BVHCD_FRONT_PURE_BVH_CD) d.front_version++;
     switch (front_op) {
         case BVHCD_FRONT_PURE_BVH_CD:
@@ -863,17 +871,10 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
             (c->frame % ((int64_t)bc << c->learned_backoff) == 0 || (c->cfg.enable_restructure && c->restr_pending)))
             c->learned_frame = true;
         if (c->built && c->frame % fc == 0 && c->last_front == BVHCD_FRONT_KEEP) {
-            uint64_t visits = 0;
-            uint32_t sz[4] = {0, 0, 0, 0}, implicit = 0;
-            HIPCHK(c, hipMemcpyAsync(&visits, ctr64(c->d.ctr) + C64_VISITS, sizeof(visits), hipMemcpyDeviceToHost,
-                                     c->stream));
+            uint32_t sz[4] = {0, 0, 0, 0};
             HIPCHK(c, hipMemcpyAsync(sz, c->d.fsize, sizeof(sz), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipMemcpyAsync(&implicit, c->d.ctr + C_IMPLICIT, sizeof(implicit), hipMemcpyDeviceToHost,
-                                     c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             const double fi = (double)sz[c->d.fcur[0]], fe = (double)sz[2 + c->d.fcur[1]];
-            // entries KEEP tested (the implicit shared-vertex head takes no node test)
-            const double entries = fi + fe - (double)implicit;
             // The previous frame was a KEEP frame: its time, against the first such frame after
             // the last rebuild, is what keeping the aging tree and front costs per frame. Once
             // that excess, summed over the frames since the rebuild, reaches what a rebuild frame
@@ -891,10 +892,6 @@ int bvhcd_step_scheduled(bvhcd_ctx* c) {
                 front_op = BVHCD_FRONT_GENERATE;
                 c->learned_frame = false;  // the rebuild replaces this frame's subtree decision
             }
-            // KEEP's descents below now-overlapping internal entries are cheaper on node-ordered
-            // fronts (lanes descend the same subtree); without them, query order keeps each
-            // wave's records local. The next produced front takes the order that fits.
-            c->learned_node_order = (double)visits > kLearnedNodeOrderRatio * entries + 1024.0;
         }
         if (bvh_op == BVHCD_BVH_BUILD) {
             c->keep_base = -1.0;
@@ -1409,6 +1406,10 @@ DevState inter_view(bvhcd_inter* x) {
     v.ctr = o.ctr;
     v.skip_adj = 0;
     v.inter = 1;
+    v.fchunk[0] = v.fchunk[1] = nullptr;  // inter fronts are sorted by query after production
+    v.wave_tot[0] = v.wave_tot[1] = nullptr;
+    v.wave_base = nullptr;
+    v.wave_cap = 0;
     v.qleaf = r.leaf;
     v.qprim = r.prim_idx;
     v.qlfaces = r.lfaces;

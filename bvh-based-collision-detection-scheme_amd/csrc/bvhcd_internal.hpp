@@ -221,6 +221,16 @@ struct DevState {
     uint32_t* fsize = nullptr;      // [kind*2 + buf] sizes (4 words)
     uint32_t* fcnt = nullptr;       // [kind][kPairSegs * kCntStride] segment counters of a front
                                     // being produced (front_cap is kPairSegs equal segments)
+    // Wave-ordered production (self-CD GENERATE without node ordering): every staging flush of a
+    // front list also records one chunk (offset in its segment, producing wave, the wave's
+    // entries before it, count) at [segment][chunk seq], the seq being the high word of the
+    // segment's 64-bit counter; each wave leaves its total in wave_tot. A scan of the totals
+    // then places every chunk: the front comes out in wave order with no sort.
+    uint4* fchunk[2] = {nullptr, nullptr};      // [kind][kPairSegs * chunk_seg_cap]
+    int64_t chunk_seg_cap = 0;
+    uint32_t* wave_tot[2] = {nullptr, nullptr};  // [kind][wave_cap]
+    uint32_t* wave_base = nullptr;              // [wave_cap] exclusive scan of one kind's totals
+    int64_t wave_cap = 0;
     uint32_t* intcnt = nullptr;     // T
     uint32_t* extcnt = nullptr;     // T+1
     uint32_t* intbeg = nullptr;     // T+1
@@ -293,6 +303,7 @@ enum Counter : int {
     C_NO_IMPLICIT = 25,// the last refit cannot vouch for shared-vertex pairs: a vertex coordinate
                        // above 1e8 in magnitude (or NaN / inf), or boxes given by the caller
     C_BADFACE = 26,    // set_topology: a face vertex id outside its vertex range
+    C_WAVE_TOTAL = 27, // wave-ordered production: entries of the list being gathered
     C_NUM = 32
 };
 enum Counter64 : int {

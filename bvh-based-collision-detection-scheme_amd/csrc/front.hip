@@ -59,6 +59,10 @@ struct FrontSink {
     int32_t* query;
     int64_t seg_cap;
     uint32_t ovf_bit;
+    // wave-ordered production (null: off): chunk records and per-wave totals
+    uint4* chunks = nullptr;
+    uint32_t* wave_tot = nullptr;
+    uint32_t chunk_seg_cap = 0;
 };
 struct PairSink {  // int2 list with one counter
     uint32_t* count;
@@ -131,10 +135,40 @@ __device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, lds_int
     }
     if (over) atomicOr(ovf, sk.ovf_bit);
 }
+// Wave-ordered production: one 64-bit add on the segment counter takes both the entry slots
+// (low word) and the chunk's seq in the segment (high word). The chunk record keeps only what
+// its segment holds; a record that does not fit drops its chunk (overflow bit) so that the
+// gathered front stays dense. sfill[2] counts the wave's recorded entries.
+__device__ inline uint32_t seg_reserve_chunk(const FrontSink& sk, uint32_t* ovf, lds_u32* sfill, uint32_t f,
+                                             uint64_t active, uint32_t& g) {
+    const int leader = __ffsll((long long)active) - 1;
+    uint32_t base = 0;
+    g = 0;
+    if (lane_id() == leader) {
+        g = sfill[1] % kPairSegs;
+        sfill[1] = sfill[1] + 1;
+        const unsigned long long old =
+            atomicAdd(reinterpret_cast<unsigned long long*>(sk.cnt + g * kCntStride), (1ull << 32) | f);
+        base = (uint32_t)old;
+        const uint32_t seq = (uint32_t)(old >> 32);
+        uint32_t stored = base < (uint64_t)sk.seg_cap ? (uint32_t)min<int64_t>(f, sk.seg_cap - base) : 0u;
+        if (seq < sk.chunk_seg_cap) {
+            const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+            sk.chunks[(int64_t)g * sk.chunk_seg_cap + seq] = make_uint4(base, wave, sfill[2], stored);
+        } else {
+            stored = 0u;
+            atomicOr(ovf, sk.ovf_bit);
+        }
+        sfill[2] = sfill[2] + stored;
+    }
+    g = __shfl(g, leader);
+    return __shfl(base, leader);
+}
 __device__ inline void stage_flush(const FrontSink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u32* sfill,
                                    uint32_t f, uint64_t active) {
     uint32_t g;
-    const uint32_t base = seg_reserve(sk.cnt, sfill, f, active, g);
+    const uint32_t base = sk.chunks ? seg_reserve_chunk(sk, ovf, sfill, f, active, g)
+                                    : seg_reserve(sk.cnt, sfill, f, active, g);
     const uint32_t rank = (uint32_t)__popcll(active & lanes_below());
     const uint32_t nact = (uint32_t)__popcll(active);
     const int64_t seg0 = (int64_t)g * sk.seg_cap;
@@ -179,17 +213,24 @@ __device__ inline void stage_final(const Sink& sk, uint32_t* ovf, lds_int2* sbuf
     if (lane_id() == 0) *sfill = 0;
 }
 
-// Per-block staging storage for up to three lists; fill[list][wave] = {staged, rotor}.
+// Per-block staging storage for up to three lists; fill[list][wave] = {staged, rotor, recorded
+// entries of a wave-ordered production}.
 struct Stages {
     int2 buf[3][kWaves][kStage3];
-    uint32_t fill[3][kWaves][2];
+    uint32_t fill[3][kWaves][3];
 };
 __device__ inline void stages_init(Stages& st) {
     const int w = threadIdx.x >> 6;
     if (lane_id() < 3) {
         st.fill[lane_id()][w][0] = 0;
         st.fill[lane_id()][w][1] = rotor_start();
+        st.fill[lane_id()][w][2] = 0;
     }
+}
+// End of a wave's production: its recorded total, every wave of the launch writing its own.
+__device__ inline void wave_total(const FrontSink& sk, lds_u32* sfill) {
+    if (sk.chunks && lane_id() == __ffsll((long long)__ballot(1)) - 1)
+        sk.wave_tot[blockIdx.x * kWaves + (threadIdx.x >> 6)] = sfill[2];
 }
 
 // Output context of a traversal thread.
@@ -218,6 +259,8 @@ struct Out {
         stage_final(fi, ovf, sb[0], sf[0]);
         stage_final(fe, ovf, sb[1], sf[1]);
         stage_final(bp, ovf, sb[2], sf[2]);
+        wave_total(fi, sf[0]);
+        wave_total(fe, sf[1]);
     }
 };
 
@@ -402,6 +445,131 @@ static void front_finish(DevState& d, int kind, bool append, hipStream_t s) {
     hipLaunchKernelGGL(k_front_compact, dim3(2048), dim3(kBlock), 0, s, d, kind, d.fnode[kind][nb],
                        d.fquery[kind][nb], d.fnode[kind][c], d.fquery[kind][c], base_word, size);
 }
+// Wave-ordered production: the sinks record chunks, `waves` producing waves (the launch's).
+static bool wave_ordered_ok(const DevState& d, int64_t waves) {
+    return d.fchunk[0] && d.fchunk[1] && waves <= d.wave_cap;
+}
+static void record_chunks(DevState& d, FrontSink& sk, int kind) {
+    sk.chunks = d.fchunk[kind];
+    sk.wave_tot = d.wave_tot[kind];
+    sk.chunk_seg_cap = (uint32_t)d.chunk_seg_cap;
+}
+
+// Gathers a wave-ordered production of `kind` into its current buffer: chunk c (segment g, seq)
+// of wave w goes to wave_base[w] + its wave offset, one wave per chunk. The size becomes the
+// wave totals' sum, clamped to the capacity (overflow bit otherwise; the clamped front is a
+// dense prefix).
+__global__ void __launch_bounds__(kBlock) k_chunk_gather(DevState d, int kind, const uint32_t* __restrict__ src_node,
+                                                         const int32_t* __restrict__ src_query,
+                                                         uint32_t* __restrict__ dst_node,
+                                                         int32_t* __restrict__ dst_query, uint32_t* dst_size) {
+    __shared__ uint32_t pre[kPairSegs + 1];
+    const int64_t seg_cap = d.front_cap[kind] / kPairSegs;
+    const uint32_t* cnt = d.fcnt + kind * kPairSegs * kCntStride;
+    const uint32_t nchunks = seg_prefix(cnt + 1, (uint32_t)d.chunk_seg_cap, pre);  // high words: chunk seqs
+    const uint32_t total = d.ctr[C_WAVE_TOTAL];
+    const uint32_t cap = (uint32_t)min(d.front_cap[kind], (int64_t)0xFFFFFFFF);
+    const uint4* chunks = d.fchunk[kind];
+    const int lane = lane_id();
+    const uint32_t nw = gridDim.x * kWaves;
+    for (uint32_t c = blockIdx.x * kWaves + (threadIdx.x >> 6); c < nchunks; c += nw) {
+        const int64_t slot = seg_slot(pre, c, d.chunk_seg_cap);
+        const uint4 ch = chunks[slot];
+        const int64_t src = (slot / d.chunk_seg_cap) * seg_cap + ch.x;
+        const uint32_t dst0 = d.wave_base[ch.y] + ch.z;
+        for (uint32_t i = lane; i < ch.w; i += 64) {
+            const uint32_t o = dst0 + i;
+            if (o < cap) {
+                dst_node[o] = src_node[src + i];
+                dst_query[o] = src_query[src + i];
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *dst_size = min(total, cap);
+        if (total > cap) atomicOr(&d.ctr[C_OVF], kind ? 2u : 1u);
+        uint32_t sum = 0, mx = 0;
+        for (int g = 0; g < kPairSegs; ++g) {
+            const uint32_t v = cnt[g * kCntStride];
+            sum += v;
+            mx = max(mx, v);
+        }
+        d.ctr[kind ? C_FREQ_E : C_FREQ_I] = max(sum, mx * (uint32_t)kPairSegs);
+    }
+}
+// Then each wave's run of entries (its 64 queries 64 w .. 64 w + 63) is ordered by query, one
+// workgroup per run: a counting sort on q - 64 w through LDS into the spare buffer, which
+// becomes the current one. The order within a query is not kept (fronts are multisets). A run
+// longer than kWaveRunLds entries is copied as it is.
+constexpr uint32_t kWaveRunLds = 4096;
+__global__ void __launch_bounds__(kBlock) k_wave_sort(DevState d, int kind, const uint32_t* __restrict__ src_node,
+                                                      const int32_t* __restrict__ src_query,
+                                                      uint32_t* __restrict__ dst_node, int32_t* __restrict__ dst_query,
+                                                      const uint32_t* src_size, uint32_t* dst_size) {
+    __shared__ uint32_t lnode[kWaveRunLds];
+    __shared__ int32_t lquery[kWaveRunLds];
+    __shared__ uint32_t cnt[64];
+    const uint32_t w = blockIdx.x, size = *src_size;
+    const uint32_t b0 = min(d.wave_base[w], size), b1 = min(d.wave_base[w] + d.wave_tot[kind][w], size);
+    const uint32_t m = b1 - b0;
+    const int32_t q0 = (int32_t)(w * 64u);
+    if (threadIdx.x == 0 && w == 0) *dst_size = size;
+    if (m > kWaveRunLds) {
+        for (uint32_t i = threadIdx.x; i < m; i += kBlock) {
+            dst_node[b0 + i] = src_node[b0 + i];
+            dst_query[b0 + i] = src_query[b0 + i];
+        }
+        return;
+    }
+    if (threadIdx.x < 64) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    constexpr int kPer = kWaveRunLds / kBlock;
+    uint32_t rank[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t i = threadIdx.x + k * kBlock;
+        if (i < m) {
+            const uint32_t nd = src_node[b0 + i];
+            const int32_t q = src_query[b0 + i];
+            lnode[i] = nd;
+            lquery[i] = q;
+            rank[k] = atomicAdd(&cnt[min((uint32_t)(q - q0), 63u)], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the 64 query counts
+        const uint32_t v = cnt[threadIdx.x];
+        uint32_t x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if ((int)threadIdx.x >= off) x += y;
+        }
+        cnt[threadIdx.x] = x - v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t i = threadIdx.x + k * kBlock;
+        if (i < m) {
+            const int32_t q = lquery[i];
+            const uint32_t o = b0 + cnt[min((uint32_t)(q - q0), 63u)] + rank[k];
+            dst_node[o] = lnode[i];
+            dst_query[o] = q;
+        }
+    }
+}
+
+static void front_finish_waves(DevState& d, int kind, int64_t waves, hipStream_t s) {
+    const int c = d.fcur[kind], nb = c ^ 1;
+    launch_exscan(d, d.wave_tot[kind], d.wave_base, waves, d.ctr + C_WAVE_TOTAL, s);
+    hipLaunchKernelGGL(k_chunk_gather, dim3(2048), dim3(kBlock), 0, s, d, kind, d.fnode[kind][nb], d.fquery[kind][nb],
+                       d.fnode[kind][c], d.fquery[kind][c], d.fsize + 2 * kind + c);
+    hipLaunchKernelGGL(k_wave_sort, dim3((unsigned)waves), dim3(kBlock), 0, s, d, kind, d.fnode[kind][c],
+                       d.fquery[kind][c], d.fnode[kind][nb], d.fquery[kind][nb], d.fsize + 2 * kind + c,
+                       d.fsize + 2 * kind + nb);
+    d.fcur[kind] = nb;
+}
+
 static FrontIn cur_front(DevState& d) {
     FrontIn f;
     for (int k = 0; k < 2; ++k) {
@@ -605,16 +773,30 @@ void launch_order_fronts(DevState& d, int32_t n, hipStream_t s) {
                        d.fsize + 2 + d.fcur[1], (uint32_t)std::min<int64_t>(d.front_cap[1], 0xFFFFFFFFll), n, d.extbeg);
 }
 
+// Without node ordering, the front comes out ordered by query without a global sort: the
+// production is gathered in wave order (each wave's 64 consecutive queries keep their entries
+// together, the waves following in query order), then each wave's run is sorted by query.
 void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order) {
     reset_step_counters(d, s);
     front_begin(d, s);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
     const int64_t nq = query_count(d, n);
-    if (nq > 0) hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(nq)), dim3(kBlock), 0, s, n, d, fi, fe);
-    front_finish(d, 0, false, s);
-    front_finish(d, 1, false, s);
-    if (order && n >= 2) launch_order_fronts(d, n, s);
-    if (!order) launch_query_order(d, n, s);
+    const unsigned grid = nq > 0 ? grid_for(nq) : 0u;
+    const bool waves = !order && nq > 0 && wave_ordered_ok(d, (int64_t)grid * kWaves);
+    if (waves) {
+        record_chunks(d, fi, 0);
+        record_chunks(d, fe, 1);
+    }
+    if (nq > 0) hipLaunchKernelGGL(k_traverse<true>, dim3(grid), dim3(kBlock), 0, s, n, d, fi, fe);
+    if (waves) {
+        front_finish_waves(d, 0, (int64_t)grid * kWaves, s);
+        front_finish_waves(d, 1, (int64_t)grid * kWaves, s);
+    } else {
+        front_finish(d, 0, false, s);
+        front_finish(d, 1, false, s);
+        if (order && n >= 2) launch_order_fronts(d, n, s);
+        if (!order) launch_query_order(d, n, s);
+    }
     d.front_by_query = order ? 0 : 1;
 }
 
