@@ -922,7 +922,7 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool orde
 // dependent load: the node's record, and for an overlapping internal node its children). The
 // nodes tested and the pairs found are those of the lane's own walk; a stack that is full
 // leaves the lane to walk that subtree itself.
-constexpr uint32_t kDescStack = 256;
+constexpr uint32_t kDescStack = 512;
 struct DescStack {
     int2 item[kWaves][kDescStack];  // (query, node << 1 | leaf)
     uint32_t top[kWaves];
@@ -986,8 +986,11 @@ __device__ inline uint32_t desc_test(const DevState& d, const Out& out, lds_int2
     return visits;
 }
 
-// Empties the wave's stack, one item per lane and round. Called by all active lanes.
-__device__ inline uint32_t desc_drain(const DevState& d, const Out& out, lds_int2* stk, lds_u32* top) {
+// Works the wave's stack off, one item per lane and round, while it holds at least `least` items
+// (64 between loop iterations: only full rounds, the rest waits for more; 1 at the end). Called
+// by all active lanes.
+__device__ inline uint32_t desc_drain(const DevState& d, const Out& out, lds_int2* stk, lds_u32* top,
+                                      uint32_t least) {
     uint32_t visits = 0;
     const int lane = lane_id();
     const uint32_t nact = (uint32_t)__popcll(__ballot(1));
@@ -997,7 +1000,7 @@ __device__ inline uint32_t desc_drain(const DevState& d, const Out& out, lds_int
     const int64_t limit = 2 * d.maxT + 16;
     for (int64_t rounds = 0;; ++rounds) {
         const uint32_t t = *top;
-        if (t == 0) break;
+        if (t < least || t == 0) break;
         if (rounds > limit) {
             if (lane == __ffsll((long long)__ballot(1)) - 1) {
                 atomicOr(&d.ctr[C_OVF], 16u);
@@ -1110,8 +1113,9 @@ __global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int imp
             if (out.skip_adj && emit) emit = !faces_share_vertex(out.lfaces, q[u], nd[u]);
             stage_push(out.bp, out.ovf, out.sb[2], out.sf[2], out.scap, emit, make_int2(q[u], nd[u]));
         }
-        visits += desc_drain(d, out, stk, stop);
+        visits += desc_drain(d, out, stk, stop, 64u);
     }
+    visits += desc_drain(d, out, stk, stop, 1u);
     stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
     add_visits(d, visits);
 }
