@@ -1082,9 +1082,9 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     out->np_pairs = ctr[C_NP];
     out->required_int_front = std::max<int64_t>(out->int_front, ctr[C_FREQ_I]);
     out->required_ext_front = std::max<int64_t>(out->ext_front, ctr[C_FREQ_E]);
-    // pair lists are kPairSegs equal segments: what fits every segment
-    out->required_pairs = std::max<int64_t>((int64_t)ctr[C_BP],
-                                            (int64_t)std::max(ctr[C_BP_SEGMAX], ctr[C_NP_SEGMAX]) * kPairSegs);
+    // the requested broad-phase count (flushes move on to segments with room, so the pair
+    // capacity is a plain total)
+    out->required_pairs = (int64_t)ctr[C_BP];
     out->node_visits = (int64_t)c64[C64_VISITS];
     out->implicit_pairs = ctr[C_IMPLICIT];
     out->restructured_subtrees = c->restr_sub;
@@ -1105,7 +1105,10 @@ int bvhcd_get_stats(bvhcd_ctx* c, bvhcd_stats* out) {
     if (ctr[C_OVF]) {
         if (ctr[C_OVF] & 16u) return fail(c, BVHCD_E_DEVICE, "traversal step bound exceeded (corrupted tree)");
         out->required_int_front = std::max<int64_t>(out->int_front, c->d.front_cap[0] + 1);
-        return fail(c, BVHCD_E_CAPACITY, "capacity exceeded");
+        char buf[160];
+        snprintf(buf, sizeof(buf), "capacity exceeded (flags 0x%x: 1 int front, 2 ext front, 4 bp pairs, 8 np pairs)",
+                 ctr[C_OVF]);
+        return fail(c, BVHCD_E_CAPACITY, buf);
     }
     return BVHCD_OK;
 }

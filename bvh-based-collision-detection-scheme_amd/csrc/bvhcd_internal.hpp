@@ -14,6 +14,7 @@ namespace bvhcd {
 
 constexpr int kPairSegs = 64;   // pair-list segments (one append counter each)
 constexpr int kCntStride = 32;  // words between two segment counters (128 bytes)
+constexpr int kDroppedWord = 2; // word of a list's counter block: entries no segment had room for
 
 // ------------------------------------------------------------------ node records (HBM layout)
 // One 32-byte record per node, read with two 16-byte loads by the traversal kernels:
@@ -317,6 +318,10 @@ __host__ __device__ inline uint64_t* ctr64(uint32_t* ctr) { return reinterpret_c
 __device__ inline void zero_step_counters(const DevState& d, int g) {
     d.pcnt[g * kCntStride] = 0u;                // broad-phase segment counters
     d.pcnt[(kPairSegs + g) * kCntStride] = 0u;  // narrow-phase segment counters
+    if (g == 0) {
+        d.pcnt[kDroppedWord] = 0u;
+        d.pcnt[kPairSegs * kCntStride + kDroppedWord] = 0u;
+    }
     if (g <= C_IMPL_NP) d.ctr[g] = 0u;           // C_BP, C_NP, C_OVF, C_IMPLICIT, C_IMPL_NP
     if (g == 8) ctr64(d.ctr)[C64_VISITS] = 0ull;
 }

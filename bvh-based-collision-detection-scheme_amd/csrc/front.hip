@@ -107,82 +107,82 @@ __device__ inline uint32_t rotor_start() {
     const uint32_t b = blockIdx.x, w = threadIdx.x >> 6;
     return (b / kXcds) * kWaves + w + (b % kXcds) * kXcds;
 }
-__device__ inline uint32_t seg_reserve(uint32_t* cnt, lds_u32* sfill, uint32_t f, uint64_t active,
-                                       uint32_t& g) {
+// One flush takes its slots in the rotor's next segment; what does not fit there (the segment
+// filled up) is given back to the counter and tries the following segments, so a list overflows
+// only once all its segments are full, and every segment counter ends at the entries its segment
+// holds. Entries no segment took are counted in word 2 of the list's counter block (the list's
+// requested length is the segment counts plus that). Wave-ordered production (chunk records):
+// one 64-bit add on the segment counter takes both the slots (low word) and the chunk's seq in
+// the segment (high word), and every seq gets its record, which keeps only what its segment
+// holds; a record that does not fit drops its chunk (overflow bit) so that the gathered front
+// stays dense. sfill[2] counts the wave's recorded entries.
+struct SegTake {
+    uint32_t g, base, fit;
+};
+__device__ inline SegTake seg_take(uint32_t* cnt, int64_t seg_cap, const FrontSink* chunked, uint32_t* ovf,
+                                   lds_u32* sfill, uint32_t want, uint64_t active) {
     const int leader = __ffsll((long long)active) - 1;
-    uint32_t base = 0;
-    g = 0;
+    uint32_t g = 0, base = 0, fit = 0;
     if (lane_id() == leader) {
         g = sfill[1] % kPairSegs;
         sfill[1] = sfill[1] + 1;
-        base = atomicAdd(cnt + g * kCntStride, f);
+        if (chunked) {
+            const unsigned long long old =
+                atomicAdd(reinterpret_cast<unsigned long long*>(cnt + g * kCntStride), (1ull << 32) | want);
+            base = (uint32_t)old;
+            const uint32_t seq = (uint32_t)(old >> 32);
+            fit = base < (uint64_t)seg_cap ? (uint32_t)min<int64_t>(want, seg_cap - base) : 0u;
+            if (seq < chunked->chunk_seg_cap) {
+                const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+                chunked->chunks[(int64_t)g * chunked->chunk_seg_cap + seq] = make_uint4(base, wave, sfill[2], fit);
+            } else {
+                fit = 0u;
+                atomicOr(ovf, chunked->ovf_bit);
+            }
+            sfill[2] = sfill[2] + fit;
+        } else {
+            base = atomicAdd(cnt + g * kCntStride, want);
+            fit = base < (uint64_t)seg_cap ? (uint32_t)min<int64_t>(want, seg_cap - base) : 0u;
+        }
+        if (fit < want) atomicSub(cnt + g * kCntStride, want - fit);  // slots past the segment's end
     }
-    g = __shfl(g, leader);
-    return __shfl(base, leader);
+    return SegTake{(uint32_t)__shfl((int)g, leader), (uint32_t)__shfl((int)base, leader),
+                   (uint32_t)__shfl((int)fit, leader)};
 }
 __device__ inline void stage_flush(const SegPairSink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u32* sfill,
                                    uint32_t f, uint64_t active) {
-    uint32_t g;
-    const uint32_t base = seg_reserve(sk.cnt, sfill, f, active, g);
     const uint32_t rank = (uint32_t)__popcll(active & lanes_below());
     const uint32_t nact = (uint32_t)__popcll(active);
-    int2* out = sk.base + (int64_t)g * sk.seg_cap;
-    bool over = false;
-    for (uint32_t i = rank; i < f; i += nact) {
-        const uint32_t o = base + i;
-        if (o < sk.seg_cap) out[o] = make_int2(sbuf[i].x, sbuf[i].y);
-        else over = true;
+    uint32_t done = 0;
+    for (int a = 0; a < kPairSegs && done < f; ++a) {
+        const SegTake t = seg_take(sk.cnt, sk.seg_cap, nullptr, ovf, sfill, f - done, active);
+        int2* out = sk.base + (int64_t)t.g * sk.seg_cap + t.base;
+        for (uint32_t i = rank; i < t.fit; i += nact) out[i] = make_int2(sbuf[done + i].x, sbuf[done + i].y);
+        done += t.fit;
     }
-    if (over) atomicOr(ovf, sk.ovf_bit);
-}
-// Wave-ordered production: one 64-bit add on the segment counter takes both the entry slots
-// (low word) and the chunk's seq in the segment (high word). The chunk record keeps only what
-// its segment holds; a record that does not fit drops its chunk (overflow bit) so that the
-// gathered front stays dense. sfill[2] counts the wave's recorded entries.
-__device__ inline uint32_t seg_reserve_chunk(const FrontSink& sk, uint32_t* ovf, lds_u32* sfill, uint32_t f,
-                                             uint64_t active, uint32_t& g) {
-    const int leader = __ffsll((long long)active) - 1;
-    uint32_t base = 0;
-    g = 0;
-    if (lane_id() == leader) {
-        g = sfill[1] % kPairSegs;
-        sfill[1] = sfill[1] + 1;
-        const unsigned long long old =
-            atomicAdd(reinterpret_cast<unsigned long long*>(sk.cnt + g * kCntStride), (1ull << 32) | f);
-        base = (uint32_t)old;
-        const uint32_t seq = (uint32_t)(old >> 32);
-        uint32_t stored = base < (uint64_t)sk.seg_cap ? (uint32_t)min<int64_t>(f, sk.seg_cap - base) : 0u;
-        if (seq < sk.chunk_seg_cap) {
-            const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
-            sk.chunks[(int64_t)g * sk.chunk_seg_cap + seq] = make_uint4(base, wave, sfill[2], stored);
-        } else {
-            stored = 0u;
-            atomicOr(ovf, sk.ovf_bit);
-        }
-        sfill[2] = sfill[2] + stored;
+    if (done < f && lane_id() == __ffsll((long long)active) - 1) {
+        atomicOr(ovf, sk.ovf_bit);
+        atomicAdd(sk.cnt + kDroppedWord, f - done);
     }
-    g = __shfl(g, leader);
-    return __shfl(base, leader);
 }
 __device__ inline void stage_flush(const FrontSink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u32* sfill,
                                    uint32_t f, uint64_t active) {
-    uint32_t g;
-    const uint32_t base = sk.chunks ? seg_reserve_chunk(sk, ovf, sfill, f, active, g)
-                                    : seg_reserve(sk.cnt, sfill, f, active, g);
     const uint32_t rank = (uint32_t)__popcll(active & lanes_below());
     const uint32_t nact = (uint32_t)__popcll(active);
-    const int64_t seg0 = (int64_t)g * sk.seg_cap;
-    bool over = false;
-    for (uint32_t i = rank; i < f; i += nact) {
-        const uint32_t o = base + i;
-        if (o < sk.seg_cap) {
-            sk.node[seg0 + o] = (uint32_t)sbuf[i].y;
-            sk.query[seg0 + o] = sbuf[i].x;
-        } else {
-            over = true;
+    uint32_t done = 0;
+    for (int a = 0; a < kPairSegs && done < f; ++a) {
+        const SegTake t = seg_take(sk.cnt, sk.seg_cap, sk.chunks ? &sk : nullptr, ovf, sfill, f - done, active);
+        const int64_t o = (int64_t)t.g * sk.seg_cap + t.base;
+        for (uint32_t i = rank; i < t.fit; i += nact) {
+            sk.node[o + i] = (uint32_t)sbuf[done + i].y;
+            sk.query[o + i] = sbuf[done + i].x;
         }
+        done += t.fit;
     }
-    if (over) atomicOr(ovf, sk.ovf_bit);
+    if (done < f && lane_id() == __ffsll((long long)active) - 1) {
+        atomicOr(ovf, sk.ovf_bit);
+        atomicAdd(sk.cnt + kDroppedWord, f - done);
+    }
 }
 
 // Called by the currently active lanes of a wave; `pred` selects the lanes that emit `v`.
@@ -423,14 +423,12 @@ __global__ void __launch_bounds__(kBlock) k_front_compact(DevState d, int kind, 
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *dst_size = base + m;
         if (m < total) atomicOr(&d.ctr[C_OVF], kind ? 2u : 1u);
-        // capacity that would have held every requested entry (segments fill unevenly)
-        uint32_t sum = 0, mx = 0;
-        for (int g = 0; g < kPairSegs; ++g) {
-            const uint32_t v = d.fcnt[(kind * kPairSegs + g) * kCntStride];
-            sum += v;
-            mx = max(mx, v);
-        }
-        d.ctr[kind ? C_FREQ_E : C_FREQ_I] = max(base + sum, mx * (uint32_t)kPairSegs);
+        // capacity that would have held every requested entry (an over-estimate after an
+        // overflow: slots a full segment refused were asked for again elsewhere)
+        const uint32_t* cnt = d.fcnt + kind * kPairSegs * kCntStride;
+        uint32_t sum = cnt[kDroppedWord];
+        for (int g = 0; g < kPairSegs; ++g) sum += cnt[g * kCntStride];
+        d.ctr[kind ? C_FREQ_E : C_FREQ_I] = base + sum;
     }
 }
 // append: keep the current buffer's entries and add the new ones after them.
@@ -488,13 +486,9 @@ __global__ void __launch_bounds__(kBlock) k_chunk_gather(DevState d, int kind, c
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *dst_size = min(total, cap);
         if (total > cap) atomicOr(&d.ctr[C_OVF], kind ? 2u : 1u);
-        uint32_t sum = 0, mx = 0;
-        for (int g = 0; g < kPairSegs; ++g) {
-            const uint32_t v = cnt[g * kCntStride];
-            sum += v;
-            mx = max(mx, v);
-        }
-        d.ctr[kind ? C_FREQ_E : C_FREQ_I] = max(sum, mx * (uint32_t)kPairSegs);
+        uint32_t sum = cnt[kDroppedWord];
+        for (int g = 0; g < kPairSegs; ++g) sum += cnt[g * kCntStride];
+        d.ctr[kind ? C_FREQ_E : C_FREQ_I] = sum;
     }
 }
 // Then each wave's run of entries (its 64 queries 64 w .. 64 w + 63) is ordered by query, one
@@ -730,17 +724,6 @@ static int bits_for(int64_t v) {
     return b;
 }
 
-// Sort one front kind by node id (query as payload) and flip to the sorted buffer.
-static void sort_front(DevState& d, int kind, int64_t max_id, hipStream_t s) {
-    const int c = d.fcur[kind], o = c ^ 1;
-    const bool in_tmp = radix_sort_u32_dev(d, d.fnode[kind][c], d.fquery[kind][c], d.fnode[kind][o], d.fquery[kind][o],
-                                           d.front_cap[kind], d.fsize + 2 * kind + c, bits_for(max_id), s);
-    if (in_tmp) {
-        hipMemcpyAsync(d.fsize + 2 * kind + o, d.fsize + 2 * kind + c, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
-        d.fcur[kind] = o;
-    }
-}
-
 // Fronts of the learned schedule are not node-ordered; they are sorted by query leaf instead
 // (stable, so each query keeps its emission order): neighbouring queries read neighbouring
 // nodes, which KEEP's per-XCD split then keeps inside one L2.
@@ -762,15 +745,168 @@ void launch_query_order(DevState& d, int32_t n, hipStream_t s) {
     sort_front_by_query(d, 1, n, s);
 }
 
-// The reference's FlOrderLog::prepare + counting-sort scatter: here a radix sort per kind,
-// then intbeg[0..n-1] (internal ids) and extbeg[0..n] (leaves) from the sorted node ids.
-void launch_order_fronts(DevState& d, int32_t n, hipStream_t s) {
-    sort_front(d, 0, n - 1, s);
-    sort_front(d, 1, n, s);
-    hipLaunchKernelGGL(k_bucket_begin, dim3(grid_for(n)), dim3(kBlock), 0, s, d.fnode[0][d.fcur[0]],
-                       d.fsize + d.fcur[0], (uint32_t)std::min<int64_t>(d.front_cap[0], 0xFFFFFFFFll), n - 1, d.intbeg);
-    hipLaunchKernelGGL(k_bucket_begin, dim3(grid_for(n + 1)), dim3(kBlock), 0, s, d.fnode[1][d.fcur[1]],
-                       d.fsize + 2 + d.fcur[1], (uint32_t)std::min<int64_t>(d.front_cap[1], 0xFFFFFFFFll), n, d.extbeg);
+// Counting sort by node id (the reference's FlOrderLog::prepare + pureReorderLooseFrontsWithLog,
+// FlOrderLog.cu:7-26, BvttFrontReorderKernels.cu:12-24): per-node counts, an exclusive scan
+// into the bucket starts intbeg[0..n-1] (internal ids) / extbeg[0..n] (leaves), one scatter
+// into the spare buffer. Lanes holding runs of one node id (fronts produced from node-ordered
+// fronts keep them together) take their slots with one atomic per run. The order inside a
+// bucket is that of the atomics, as in the reference.
+__device__ inline uint32_t run_atomic(uint32_t* ctr, bool valid, uint32_t key) {
+    const int lane = lane_id();
+    const uint64_t vmask = __ballot(valid);
+    const uint32_t prev = __shfl_up(key, 1);
+    const bool prev_valid = lane > 0 ? ((vmask >> (lane - 1)) & 1ull) != 0 : false;
+    const bool start = valid && !(prev_valid && prev == key);
+    const uint64_t starts = __ballot(start);
+    const uint64_t le = (2ull << lane) - 1ull;  // lanes at or below this one
+    const uint64_t upto = starts & le;
+    const int s0 = upto ? 63 - __clzll((long long)upto) : lane;  // this lane's run start
+    uint32_t base = 0;
+    if (start) {  // the run ends at the next start or the next invalid lane
+        const uint64_t stop = (starts | ~vmask) & ~le;
+        const int e = stop ? __ffsll((long long)stop) - 1 : 64;
+        base = atomicAdd(ctr + key, (uint32_t)(e - lane));
+    }
+    base = __shfl(base, s0);
+    return base + (uint32_t)(lane - s0);
+}
+
+// The same with every set of equal keys in the wave taken together, one round per distinct key:
+// internal fronts put nodes with large subtrees into the fronts of many queries, and their
+// counters would otherwise take one contended atomic per entry.
+__device__ inline uint32_t match_atomic(uint32_t* ctr, bool valid, uint32_t key) {
+    const int lane = lane_id();
+    uint64_t todo = __ballot(valid);
+    uint32_t r = 0;
+    while (todo) {  // wave-uniform
+        const int first = __ffsll((long long)todo) - 1;
+        const uint32_t k = (uint32_t)__shfl((int)key, first);
+        const uint64_t grp = __ballot(key == k) & todo;
+        uint32_t base = 0;
+        if (lane == first) base = atomicAdd(ctr + k, (uint32_t)__popcll(grp));
+        base = (uint32_t)__shfl((int)base, first);
+        if ((grp >> lane) & 1ull) r = base + (uint32_t)__popcll(grp & lanes_below());
+        todo &= ~grp;
+    }
+    return r;
+}
+template <bool MATCH>
+__device__ inline uint32_t slot_atomic(uint32_t* ctr, bool valid, uint32_t key) {
+    return MATCH ? match_atomic(ctr, valid, key) : run_atomic(ctr, valid, key);
+}
+
+template <bool MATCH>
+__global__ void __launch_bounds__(kBlock) k_order_count(const uint32_t* __restrict__ node, const uint32_t* size,
+                                                        uint32_t cap, int32_t nb, uint32_t* __restrict__ cnt) {
+    const uint32_t m = min(*size, cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < m; k0 += stride) {  // wave-uniform trip count
+        const uint32_t k = k0 + threadIdx.x;
+        const bool valid = k < m;
+        const uint32_t key = valid ? min(node[k], (uint32_t)(nb - 1)) : 0u;
+        slot_atomic<MATCH>(cnt, valid, key);
+    }
+}
+
+// BALANCE (REORDER, internal front): inside each bucket the entries whose node does not overlap
+// its query go first (slots from the bucket start up), the overlapping ones last (from the bucket
+// end down), as the reference's reorderIntLooseIntraFrontsWithLog does with its back counts
+// (BvttFrontReorderKernels.cu:27-63).
+template <bool BALANCE, bool MATCH>
+__global__ void __launch_bounds__(kBlock) k_order_scatter(DevState d, const uint32_t* __restrict__ node,
+                                                          const int32_t* __restrict__ query, const uint32_t* size,
+                                                          uint32_t cap, int32_t nb, const uint32_t* __restrict__ beg,
+                                                          uint32_t* __restrict__ cur, uint32_t* __restrict__ back,
+                                                          uint32_t* __restrict__ dst_node,
+                                                          int32_t* __restrict__ dst_query, uint32_t* dst_size) {
+    const uint32_t m = min(*size, cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst_size = m;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < m; k0 += stride) {  // wave-uniform trip count
+        const uint32_t k = k0 + threadIdx.x;
+        const bool valid = k < m;
+        uint32_t nd = 0;
+        int32_t q = 0;
+        if (valid) {
+            nd = node[k];
+            q = query[k];
+        }
+        const uint32_t key = min(nd, (uint32_t)(nb - 1));
+        bool ov = false;
+        if (BALANCE && valid) {
+            const float4 qlo = d.qleaf[q].lo, qhi = d.qleaf[q].hi;
+            ov = node_overlap(d, (int32_t)key, rec_box(qlo, qhi));
+        }
+        const uint32_t r = slot_atomic<MATCH>(cur, valid && !ov, key);
+        uint32_t o = beg[key] + r;
+        if (BALANCE) {
+            const uint32_t rb = slot_atomic<MATCH>(back, valid && ov, key);
+            if (ov) o = beg[key + 1] - 1u - rb;
+        }
+        if (valid) {
+            dst_node[o] = nd;
+            dst_query[o] = q;
+        }
+    }
+}
+
+// One counting sort of a front kind by node id into its spare buffer, which becomes current;
+// beg[0..nb] = bucket starts (beg[nb] = size).
+static void order_front(DevState& d, int kind, int32_t nb, uint32_t* cnt, uint32_t* beg, bool balance,
+                        hipStream_t s) {
+    const int c = d.fcur[kind], o = c ^ 1;
+    const uint32_t cap = (uint32_t)std::min<int64_t>(d.front_cap[kind], 0xFFFFFFFFll);
+    const unsigned g = std::min<unsigned>(grid_for(d.front_cap[kind]), 8192u);
+    fill_u32(cnt, nb, 0u, s);
+    // internal fronts take equal keys per wave together, external ones (leaves near each query,
+    // few repeats) per run of equal neighbours
+    if (kind == 0)
+        hipLaunchKernelGGL(k_order_count<true>, dim3(g), dim3(kBlock), 0, s, d.fnode[kind][c], d.fsize + 2 * kind + c,
+                           cap, nb, cnt);
+    else
+        hipLaunchKernelGGL(k_order_count<false>, dim3(g), dim3(kBlock), 0, s, d.fnode[kind][c],
+                           d.fsize + 2 * kind + c, cap, nb, cnt);
+    launch_exscan(d, cnt, beg, nb, beg + nb, s);
+    fill_u32(cnt, nb, 0u, s);
+    if (balance) {
+        fill_u32(d.intback, nb, 0u, s);
+        hipLaunchKernelGGL((k_order_scatter<true, true>), dim3(g), dim3(kBlock), 0, s, d, d.fnode[kind][c],
+                           d.fquery[kind][c], d.fsize + 2 * kind + c, cap, nb, beg, cnt, d.intback, d.fnode[kind][o],
+                           d.fquery[kind][o], d.fsize + 2 * kind + o);
+    } else if (kind == 0) {
+        hipLaunchKernelGGL((k_order_scatter<false, true>), dim3(g), dim3(kBlock), 0, s, d, d.fnode[kind][c],
+                           d.fquery[kind][c], d.fsize + 2 * kind + c, cap, nb, beg, cnt, nullptr, d.fnode[kind][o],
+                           d.fquery[kind][o], d.fsize + 2 * kind + o);
+    } else {
+        hipLaunchKernelGGL((k_order_scatter<false, false>), dim3(g), dim3(kBlock), 0, s, d, d.fnode[kind][c],
+                           d.fquery[kind][c], d.fsize + 2 * kind + c, cap, nb, beg, cnt, nullptr, d.fnode[kind][o],
+                           d.fquery[kind][o], d.fsize + 2 * kind + o);
+    }
+    d.fcur[kind] = o;
+}
+
+// A freshly generated internal front is radix-sorted instead (3 passes of 8 bits, then the bucket
+// starts by binary search): there every query's front holds the few nodes with the largest
+// subtrees, and the counting sort's atomics on those nodes' counters contend even when each wave
+// takes its equal keys together (measured on the folded 2M-triangle cloth, 45 M entries: counting
+// 4.7 ms, radix 1.9 ms; the other fronts: counting 1.0-1.4 ms, radix 1.2-1.9 ms).
+static void radix_order_front(DevState& d, int kind, int32_t nb, uint32_t* beg, hipStream_t s) {
+    const int c = d.fcur[kind], o = c ^ 1;
+    const bool in_tmp = radix_sort_u32_dev(d, d.fnode[kind][c], d.fquery[kind][c], d.fnode[kind][o], d.fquery[kind][o],
+                                           d.front_cap[kind], d.fsize + 2 * kind + c, bits_for(nb - 1), s);
+    if (in_tmp) {
+        hipMemcpyAsync(d.fsize + 2 * kind + o, d.fsize + 2 * kind + c, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        d.fcur[kind] = o;
+    }
+    const int cc = d.fcur[kind];
+    hipLaunchKernelGGL(k_bucket_begin, dim3(grid_for(nb + 1)), dim3(kBlock), 0, s, d.fnode[kind][cc],
+                       d.fsize + 2 * kind + cc, (uint32_t)std::min<int64_t>(d.front_cap[kind], 0xFFFFFFFFll), nb, beg);
+}
+
+void launch_order_fronts(DevState& d, int32_t n, hipStream_t s, bool generated) {
+    if (generated) radix_order_front(d, 0, n - 1, d.intbeg, s);
+    else order_front(d, 0, n - 1, d.intcnt, d.intbeg, false, s);
+    order_front(d, 1, n, d.extcnt, d.extbeg, false, s);
 }
 
 // Without node ordering, the front comes out ordered by query without a global sort: the
@@ -794,7 +930,7 @@ void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order) {
     } else {
         front_finish(d, 0, false, s);
         front_finish(d, 1, false, s);
-        if (order && n >= 2) launch_order_fronts(d, n, s);
+        if (order && n >= 2) launch_order_fronts(d, n, s, true);
         if (!order) launch_query_order(d, n, s);
     }
     d.front_by_query = order ? 0 : 1;
@@ -910,7 +1046,7 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool orde
         fill_i32(d.rtroot, n, INT32_MAX, s);
         hipLaunchKernelGGL(k_quality, dim3(grid_for(n - 1)), dim3(kBlock), 0, s, n, d);
     }
-    if (order && n >= 2) launch_order_fronts(d, n, s);
+    if (order && n >= 2) launch_order_fronts(d, n, s, false);
     if (!order) launch_query_order(d, n, s);
     d.front_by_query = order ? 0 : 1;
 }
@@ -1323,39 +1459,14 @@ void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream
 }
 
 // ---------------------------------------------------------------- REORDER (balance)
-// Pairs as in KEEP from the unchanged cut; then the internal entries of each node bucket are
-// re-ordered with the non-overlapping ones first (sort key node * 2 + overlap).
-__global__ void __launch_bounds__(kBlock) k_reorder_keys(DevState d, FrontIn in, uint32_t* __restrict__ keys) {
-    const uint32_t ni = min(*in.size[0], in.cap[0]);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ni; k += gridDim.x * blockDim.x) {
-        const uint32_t node = in.node[0][k];
-        const int32_t q = in.query[0][k];
-        const float4 qlo = d.qleaf[q].lo, qhi = d.qleaf[q].hi;
-        const bool ov = node_overlap(d, (int32_t)node, rec_box(qlo, qhi));
-        keys[k] = node * 2u + (ov ? 1u : 0u);
-    }
-}
-__global__ void __launch_bounds__(kBlock) k_halve(uint32_t* __restrict__ keys, const uint32_t* dn, uint32_t cap) {
-    const uint32_t m = min(*dn, cap);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) keys[k] >>= 1;
-}
-
+// Pairs as in KEEP from the unchanged cut; then the internal front is counting-sorted by node
+// with each bucket's non-overlapping entries first. Its bucket starts go to scratch (`cursor`):
+// intbeg keeps the last ordering's offsets, which the quality check reads (the front need not
+// have been node-ordered since: an incremental restructure appends entries).
 void launch_reorder(DevState& d, int32_t n, hipStream_t s) {
     launch_keep(d, n, s);
     if (n < 2) return;
-    const int c = d.fcur[0], o = c ^ 1;
-    const FrontIn in = cur_front(d);
-    // keys (node * 2 + overlap) into the spare node array; the query ids stay as payload
-    hipLaunchKernelGGL(k_reorder_keys, dim3(stream_grid()), dim3(kBlock), 0, s, d, in, d.fnode[0][o]);
-    // sort (fnode[o], fquery[c]) with temporaries (fnode[c], fquery[o])
-    const bool in_tmp = radix_sort_u32_dev(d, d.fnode[0][o], d.fquery[0][c], d.fnode[0][c], d.fquery[0][o],
-                                           d.front_cap[0], d.fsize + c, bits_for(2 * (int64_t)n), s);
-    const uint32_t cap = (uint32_t)std::min<int64_t>(d.front_cap[0], 0xFFFFFFFFll);
-    hipLaunchKernelGGL(k_halve, dim3(stream_grid()), dim3(kBlock), 0, s, in_tmp ? d.fnode[0][c] : d.fnode[0][o],
-                       d.fsize + c, cap);
-    // re-pair the arrays so that buffer c holds the sorted (node, query) again
-    if (in_tmp) std::swap(d.fquery[0][c], d.fquery[0][o]);  // result: (fnode[c], fquery[o])
-    else std::swap(d.fnode[0][c], d.fnode[0][o]);            // result: (fnode[o], fquery[c])
+    order_front(d, 0, n - 1, d.intcnt, d.cursor, true, s);
 }
 
 // ---------------------------------------------------------------- restructure decision
@@ -1489,12 +1600,14 @@ void launch_narrow(DevState& d, hipStream_t s, bool counters_zeroed) {
     hipLaunchKernelGGL(k_narrow, dim3(shape_grid(tuned_shape(TK_NARROW, d.maxT))), dim3(kBlock), 0, s, d);
 }
 
-// Totals of both segmented pair lists (requested counts, so an overflowing list reports what it
-// needed) and their largest segment: ctr[C_BP], ctr[C_NP], ctr[C_BP_SEGMAX], ctr[C_NP_SEGMAX].
-// Implicit (shared-vertex) pairs are added to both totals; with a narrow phase they are hits.
+// Totals of both segmented pair lists (requested counts: what the segments hold plus what none
+// took, so an overflowing list reports what it needed) and their largest segment: ctr[C_BP],
+// ctr[C_NP], ctr[C_BP_SEGMAX], ctr[C_NP_SEGMAX]. Implicit (shared-vertex) pairs are added to
+// both totals; with a narrow phase they are hits.
 __global__ void k_pair_totals(DevState d, int narrow) {
     const int kind = threadIdx.x >> 6, g = lane_id();
-    uint32_t v = d.pcnt[(kind * kPairSegs + g) * kCntStride], m = v;
+    const uint32_t* cnt = d.pcnt + kind * kPairSegs * kCntStride;
+    uint32_t v = cnt[g * kCntStride] + (g == 0 ? cnt[kDroppedWord] : 0u), m = cnt[g * kCntStride];
     for (int off = 32; off > 0; off >>= 1) {
         v += __shfl_down(v, off);
         m = max(m, __shfl_down(m, off));

@@ -71,7 +71,7 @@ void launch_reorder(DevState& d, int32_t n, hipStream_t s);
 // ctr[C_TASKS]), then, after the rebuild, pairs of the kept entries plus the walks.
 void launch_front_restr_prepare(DevState& d, int32_t n, hipStream_t s);
 void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream_t s);
-void launch_order_fronts(DevState& d, int32_t n, hipStream_t s);      // radix sort by node + bucket starts
+void launch_order_fronts(DevState& d, int32_t n, hipStream_t s, bool generated);  // by node + bucket starts
 void launch_query_order(DevState& d, int32_t n, hipStream_t s);       // stable radix sort by query leaf
 void launch_snapshot(DevState& d, int32_t n, hipStream_t s);
 void launch_narrow(DevState& d, hipStream_t s, bool counters_zeroed = false);  // true: right after a step's reset

@@ -148,7 +148,7 @@ def test_quality_heuristic_flags_parity(pkg, oracle):
     tier (I/8 <= flagged internal nodes <= I/2) and the front is regenerated. Ops, restructure
     counts, topology after each restructure, fronts and pair sets equal the oracle's every
     frame."""
-    print("[gpu] quality heuristic flags", flush=True)
+    print("[gpu] quality heuristic flags; launch shapes:\n" + pkg.autotune_report(), flush=True)
     sc = pkg.Scene.preset("cloth256", n=48, frames=40)
     f = sc.faces()
     frames = _scrambled_frames(sc, 36)
@@ -160,7 +160,12 @@ def test_quality_heuristic_flags_parity(pkg, oracle):
         orc.update_vertices(v)
         ctx.step_scheduled()
         orc.step_scheduled()
-        gs, os_ = ctx.stats(), orc.stats()
+        try:
+            gs = ctx.stats()
+        except pkg.BvhcdError as e:
+            raise AssertionError(f"frame {fr} (oracle ops {orc.stats()['last_bvh_op']}, "
+                                 f"{orc.stats()['last_front_op']}): {e}") from e
+        os_ = orc.stats()
         assert (gs["last_bvh_op"], gs["last_front_op"]) == (os_["last_bvh_op"], os_["last_front_op"]), fr
         assert (gs["restructured_subtrees"], gs["restructured_leaves"]) == \
             (os_["restructured_subtrees"], os_["restructured_leaves"]), fr
@@ -196,7 +201,29 @@ def test_explicit_ops_parity(pkg, oracle):
                                       err_msg=str(fr))
         for w in (0, 1):
             np.testing.assert_array_equal(_rows(ctx.export_front(w)), _rows(orc.front(w)), err_msg=f"{fr} {w}")
+        _check_front_order(ctx, fo, fr)
     ctx.close()
+
+
+def _check_front_order(ctx, fo, fr):
+    """The plain schedule's fronts are counting-sorted by node after GENERATE / UPDATE; after
+    REORDER the internal front is too, each node's non-overlapping entries before its overlapping
+    ones (boxes from the exported tree)."""
+    if fo not in (1, 2, 3):
+        return
+    fi = ctx.export_front(0)
+    assert np.all(np.diff(fi[:, 1]) >= 0), f"frame {fr}: internal front not ordered by node"
+    if fo in (1, 2):
+        fe = ctx.export_front(1)
+        assert np.all(np.diff(fe[:, 1]) >= 0), f"frame {fr}: external front not ordered by node"
+        return
+    b = ctx.export_bvh()
+    qb = b["leaf_box"].reshape(-1, 6)[fi[:, 0]]
+    nb = b["int_box"].reshape(-1, 6)[fi[:, 1]]
+    ov = np.all((qb[:, :3] <= nb[:, 3:]) & (nb[:, :3] <= qb[:, 3:]), axis=1)
+    same = fi[1:, 1] == fi[:-1, 1]
+    assert ov.any() and (~ov).any(), f"frame {fr}: REORDER case without both kinds of entries"
+    assert not np.any(same & ov[:-1] & ~ov[1:]), f"frame {fr}: overlapping entry before a non-overlapping one"
 
 
 @pytest.mark.parametrize("name,frames", [
@@ -512,6 +539,42 @@ def test_capacity_overflow_reported(pkg):
     ctx.close()
     ctx = _ctx(pkg, sc, int_front_cap=10, ext_front_cap=10)
     ctx.update_vertices(sc.frame(0))
+    ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
+    with pytest.raises(pkg.BvhcdError) as e:
+        ctx.stats()
+    assert e.value.code == pkg.E_CAPACITY
+    ctx.close()
+
+
+def test_capacities_are_totals(pkg, oracle):
+    """Fronts and pair lists are written in segments, but a flush that does not fit its segment
+    moves on to others: lists sized 1 % above what a frame needs hold it exactly (pairs equal
+    brute force, required sizes equal the lengths), and a capacity below the length is reported as an
+    overflow."""
+    print("[gpu] capacities are totals", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=96)
+    f, v = sc.faces(), sc.frame(150)
+    probe = _ctx(pkg, sc)
+    probe.update_vertices(v)
+    probe.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
+    st = probe.stats()
+    fi, fe, bp = st["int_front"], st["ext_front"], len(probe.pairs(False))
+    probe.close()
+    assert min(fi, fe, bp) > 64 * 256  # several flushes per segment
+    ref_bp = oracle.pair_set(oracle.bruteforce_pairs(f, v, False))
+    ref_np = oracle.pair_set(oracle.bruteforce_pairs(f, v, True))
+    ctx = _ctx(pkg, sc, int_front_cap=fi + fi // 100, ext_front_cap=fe + fe // 100, pair_cap=bp + bp // 100)
+    ctx.update_vertices(v)
+    for op in (pkg.FRONT_GENERATE, pkg.FRONT_KEEP):
+        ctx.step(pkg.BVH_BUILD if op == pkg.FRONT_GENERATE else pkg.BVH_REFIT, op)
+        st = ctx.stats()
+        assert (st["int_front"], st["ext_front"]) == (fi, fe)
+        assert (st["required_int_front"], st["required_ext_front"]) == (fi, fe)
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(False)), ref_bp)
+        np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(True)), ref_np)
+    ctx.close()
+    ctx = _ctx(pkg, sc, ext_front_cap=fe - 64)  # rounded up to a multiple of 64: still below fe
+    ctx.update_vertices(v)
     ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
     with pytest.raises(pkg.BvhcdError) as e:
         ctx.stats()
