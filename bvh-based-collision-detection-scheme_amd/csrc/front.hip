@@ -631,18 +631,109 @@ std::string tuning_report() {
 }
 
 // ---------------------------------------------------------------- pure traversal / generate
+// Lane of the wave holding the k-th (0-based) set bit of m (k < popcount(m)).
+__device__ inline int kth_set_lane(uint64_t m, uint32_t k) {
+    int lo = 0, hi = 63;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((uint32_t)__popcll(m & ((2ull << mid) - 1ull)) > k) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// The stackless walk of stackless<FRONT>, one node test per lane and step, with the wave's
+// lanes sharing the work: a query's walk from `cur` to `stop` is, at an overlapping internal
+// node nd, nd's subtree followed by the walk from nd's escape to `stop`. So a lane at such a node
+// can hand the second part to an idle lane (its own walk then stops at nd's escape) — the walks
+// of 64 queries end at very different times (the longest is ~1.8x the mean on the folded 2M
+// cloth), and without this the wave waits for its longest one. The nodes tested, the entries
+// and the pairs are those of the 64 queries' own walks; only their order changes.
 template <bool FRONT>
 __global__ void __launch_bounds__(kBlock) k_traverse(int32_t n, DevState d, FrontSink fi, FrontSink fe) {
     __shared__ Stages st;
     stages_init(st);
     const Out out = make_out(st, fi, fe, d);
+    const int lane = lane_id();
+    const int32_t nq = d.inter ? d.nq : n - 1;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t v = 0;
     // self-CD: query leaf i walks the leaves after it (from lca[i + 1], its record's escape);
     // inter-object: every query walks the whole tree from its root lca[0]
-    if (i < (d.inter ? d.nq : n - 1)) {
+    bool act = i < nq;
+    int32_t qi = i, cur = -1, stop = -1, spine = -1;  // spine: bottom of the spine being descended
+    Box qb{};
+    if (act) {
         const float4 lo = d.qleaf[i].lo, hi = d.qleaf[i].hi;
-        v = stackless<FRONT>(d, out, i, rec_box(lo, hi), d.inter ? d.leaf_lca[0] : f2i(hi.w), -1);
+        qb = rec_box(lo, hi);
+        cur = d.inter ? d.leaf_lca[0] : f2i(hi.w);
+        act = cur != stop;
+    }
+    uint32_t v = 0;
+    // every node is tested at most once per query walk; the bound only guards against a
+    // corrupted tree (flag 16, reported as a device error, instead of a hang)
+    const int64_t limit = 64 * (2 * d.maxT + 16);
+    for (int64_t round = 0; __ballot(act) != 0ull; ++round) {
+        if (round > limit) {
+            if (act) atomicOr(&d.ctr[C_OVF], 16u);
+            break;
+        }
+        bool emit_int = false, emit_ext = false, emit_pair = false, can_give = false;
+        int32_t node = 0, give = -1;
+        if (act) {
+            ++v;
+            if (!(cur & 1)) {
+                const int32_t nd = cur >> 1;
+                const float4 lo = d.inode[nd].lo, hi = d.inode[nd].hi;
+                if (spine < 0) spine = d.leaf_par[f2i(lo.w)];  // a new subtree: its left spine's bottom
+                node = nd;
+                if (!box_overlap(qb, lo, hi)) {
+                    emit_int = FRONT;
+                    cur = f2i(hi.w);
+                    spine = -1;
+                } else {
+                    give = f2i(hi.w);  // what follows nd's subtree
+                    can_give = give != stop;
+                    cur = nd == spine ? ((f2i(lo.w) << 1) | 1) : cur + 2;  // leftmost leaf, or nd + 1
+                    if (nd == spine) spine = -1;
+                }
+            } else {
+                const int32_t j = cur >> 1;
+                const float4 llo = d.leaf[j].lo, lhi = d.leaf[j].hi;
+                node = j;
+                emit_pair = box_overlap(qb, llo, lhi);
+                emit_ext = FRONT;
+                cur = f2i(lhi.w);
+                spine = -1;
+            }
+            act = cur != stop;
+        }
+        if (FRONT) {
+            stage_push(out.fi, out.ovf, out.sb[0], out.sf[0], out.scap, emit_int, make_int2(qi, node));
+            stage_push(out.fe, out.ovf, out.sb[1], out.sf[1], out.scap, emit_ext, make_int2(qi, node));
+        }
+        bool pair = emit_pair;
+        if (out.skip_adj && pair) pair = !faces_share_vertex(out.lfaces, qi, node);
+        stage_push(out.bp, out.ovf, out.sb[2], out.sf[2], out.scap, pair, make_int2(qi, node));
+        // idle lanes take the rest of other lanes' walks
+        const uint64_t idle = __ballot(!act), donors = __ballot(can_give);
+        const uint32_t pairs = (uint32_t)min(__popcll(idle), __popcll(donors));
+        if (pairs >= 8u || (pairs > 0u && __popcll(idle) >= 32)) {
+            const uint32_t ri = (uint32_t)__popcll(idle & lanes_below());
+            const uint32_t rd = (uint32_t)__popcll(donors & lanes_below());
+            const int src = (!act && ri < pairs) ? kth_set_lane(donors, ri) : lane;
+            const int32_t g_q = __shfl(qi, src), g_cur = __shfl(give, src), g_stop = __shfl(stop, src);
+            const float g0 = __shfl(qb.mnx, src), g1 = __shfl(qb.mny, src), g2 = __shfl(qb.mnz, src);
+            const float g3 = __shfl(qb.mxx, src), g4 = __shfl(qb.mxy, src), g5 = __shfl(qb.mxz, src);
+            if (can_give && rd < pairs) stop = give;  // the donor ends at nd's escape
+            if (!act && ri < pairs) {
+                act = true;
+                qi = g_q;
+                cur = g_cur;
+                stop = g_stop;
+                spine = -1;
+                qb = Box{g0, g1, g2, g3, g4, g5};
+            }
+        }
     }
     if (FRONT) {
         out.final_flush();
