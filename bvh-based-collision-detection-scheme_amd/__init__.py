@@ -128,6 +128,7 @@ def lib():
         "bvhcd_batch_autotune": (C.c_int, [P]),
         "bvhcd_autotune_report": (C.c_int, [P, I64]),
         "bvhcd_autotune_set": (C.c_int, [C.c_int, I64, C.c_uint32, C.c_uint32]),
+        "bvhcd_autotune_set_block": (C.c_int, [C.c_int, I64, C.c_uint32, C.c_uint32, C.c_uint32]),
         "bvhcd_update_boxes": (C.c_int, [P, P, I64, C.c_int]),
         "bvhcd_mlp_set_repair_cost": (C.c_int, [P, C.c_float]),
         "bvhcd_schedule_ops": (C.c_int, [C.c_int, I64, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -493,9 +494,9 @@ class CollisionContext:
 TUNE_KEEP, TUNE_NARROW, TUNE_BATCH_CD = 0, 1, 2
 
 
-def autotune_set(kernel: int, n: int, grid: int, unroll: int = 1) -> None:
-    """Fix one kernel's launch shape for the size class of n (bvhcd_autotune_set)."""
-    st = lib().bvhcd_autotune_set(int(kernel), int(n), int(grid), int(unroll))
+def autotune_set(kernel: int, n: int, grid: int, unroll: int = 1, block: int = 256) -> None:
+    """Fix one kernel's launch shape for the size class of n (bvhcd_autotune_set_block)."""
+    st = lib().bvhcd_autotune_set_block(int(kernel), int(n), int(grid), int(unroll), int(block))
     if st != OK:
         raise BvhcdError(st, "autotune_set")
 

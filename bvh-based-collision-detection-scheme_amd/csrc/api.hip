@@ -1244,8 +1244,10 @@ int bvhcd_autotune(bvhcd_ctx* c) {
     HIPCHK(c, hipEventCreate(&e1));
     std::vector<unsigned> cand[2];
     for (unsigned g : {1024u, 2048u, 4096u, 8192u, 16384u, 32768u})
-        for (unsigned u : {1u, 2u, 4u}) cand[TK_KEEP].push_back(make_shape(g, u));
-    for (unsigned g : {1024u, 2048u, 4096u, 8192u, 16384u}) cand[TK_NARROW].push_back(g);
+        for (unsigned u : {1u, 2u, 4u})
+            for (unsigned w : {2u, 4u, 8u}) cand[TK_KEEP].push_back(make_shape(g, u, w));
+    for (unsigned g : {1024u, 2048u, 4096u, 8192u, 16384u})
+        for (unsigned w : {2u, 4u, 8u}) cand[TK_NARROW].push_back(make_shape(g, 1u, w));
     for (int k : {(int)TK_KEEP, (int)TK_NARROW}) {
         float best_ms = 1e30f;
         unsigned best = tuned_shape(k, k == TK_KEEP ? c->T : d.maxT);
@@ -1285,12 +1287,16 @@ int bvhcd_autotune(bvhcd_ctx* c) {
     return check_overflow(c);
 }
 
-int bvhcd_autotune_set(int kernel, int64_t n, uint32_t grid, uint32_t unroll) {
+int bvhcd_autotune_set_block(int kernel, int64_t n, uint32_t grid, uint32_t unroll, uint32_t block) {
     if (kernel < 0 || kernel >= TK_COUNT || n < 1 || grid < 1 || grid >= (1u << 24)) return BVHCD_E_INVALID;
     const bool ok = unroll == 1 || ((unroll == 2 || unroll == 4) && kernel == TK_KEEP);
-    if (!ok) return BVHCD_E_INVALID;
-    set_tuned_shape(kernel, n, make_shape(grid, unroll));
+    const bool bok = block == 256 || ((block == 128 || block == 512) && kernel != TK_BATCH_CD);
+    if (!ok || !bok) return BVHCD_E_INVALID;
+    set_tuned_shape(kernel, n, make_shape(grid, unroll, block / 64));
     return BVHCD_OK;
+}
+int bvhcd_autotune_set(int kernel, int64_t n, uint32_t grid, uint32_t unroll) {
+    return bvhcd_autotune_set_block(kernel, n, grid, unroll, 256);
 }
 
 int bvhcd_autotune_report(char* buf, int64_t cap) {

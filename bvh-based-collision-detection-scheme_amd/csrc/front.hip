@@ -103,9 +103,10 @@ __device__ void stage_flush(const Sink& sk, uint32_t* ovf, lds_int2* sbuf, lds_u
 // spread over kPairSegs counters and every segment fills evenly whichever workgroups produce
 // the entries. The rotor start spreads the waves of one XCD (workgroups b with equal b % 8,
 // which a per-XCD split makes neighbours in the data) over all segments too.
+template <int W = kWaves>
 __device__ inline uint32_t rotor_start() {
     const uint32_t b = blockIdx.x, w = threadIdx.x >> 6;
-    return (b / kXcds) * kWaves + w + (b % kXcds) * kXcds;
+    return (b / kXcds) * W + w + (b % kXcds) * kXcds;
 }
 // One flush takes its slots in the rotor's next segment; what does not fit there (the segment
 // filled up) is given back to the counter and tries the following segments, so a list overflows
@@ -269,20 +270,24 @@ __device__ inline SegPairSink pair_segments(const DevState& d, int kind, uint32_
     return SegPairSink{d.pcnt + kind * kPairSegs * kCntStride, kind ? d.np : d.bp, d.pair_seg_cap, ovf_bit};
 }
 
-// Pair-only kernels (KEEP, pure traversal) stage one list: a third of the LDS, so 8 waves per
-// SIMD instead of 6 hide more of the dependent node loads.
-struct StagesBP {
-    int2 buf[kWaves][kStage];
-    uint32_t fill[kWaves][2];
+// Pair-only kernels (KEEP, batched traversal) stage one list: a third of the LDS, so 8 waves
+// per SIMD instead of 6 hide more of the dependent node loads. W = waves per workgroup.
+template <int W = kWaves>
+struct StagesBPT {
+    int2 buf[W][kStage];
+    uint32_t fill[W][2];
 };
-__device__ inline void stages_init(StagesBP& st) {
+using StagesBP = StagesBPT<>;
+template <int W>
+__device__ inline void stages_init(StagesBPT<W>& st) {
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) {
         st.fill[w][0] = 0;
-        st.fill[w][1] = rotor_start();
+        st.fill[w][1] = rotor_start<W>();
     }
 }
-__device__ inline Out make_out(StagesBP& st, const DevState& d) {
+template <int W>
+__device__ inline Out make_out(StagesBPT<W>& st, const DevState& d) {
     Out o;
     o.fi = FrontSink{nullptr, nullptr, nullptr, 0u, 0u};
     o.fe = o.fi;
@@ -370,16 +375,16 @@ __device__ uint32_t stackless(const DevState& d, const Out& out, int32_t qi, con
     return visits;
 }
 
-// Block-reduced visit count, one 64-bit atomic per block.
+// Block-reduced visit count, one 64-bit atomic per block (blocks of up to 16 waves).
 __device__ inline void add_visits(const DevState& d, uint32_t v) {
-    __shared__ uint64_t red[kWaves];
+    __shared__ uint64_t red[16];
     uint64_t x = v;
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
     if (lane_id() == 0) red[threadIdx.x >> 6] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t t = 0;
-        for (int i = 0; i < kWaves; ++i) t += red[i];
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
         if (t) atomicAdd((unsigned long long*)&ctr64(d.ctr)[C64_VISITS], (unsigned long long)t);
     }
 }
@@ -589,10 +594,11 @@ static unsigned stream_grid() { return 2048; }
 
 // ---------------------------------------------------------------- launch-shape tuning
 // The reference picks one block size per kernel from the occupancy API
-// (system/CudaDevice/CudaDevice.cu:60-94). Here block sizes are fixed at one wave64 multiple
-// (256, the LDS staging is sized for it) and what is tuned is the grid of the grid-stride and
-// persistent kernels, per kernel and size class ceil(log2 n), by timing candidates on live data
-// (bvhcd_autotune / bvhcd_batch_autotune). The table is process-wide and not locked.
+// (system/CudaDevice/CudaDevice.cu:60-94). Here the grid of the grid-stride and persistent
+// kernels, KEEP's entries per thread and the workgroup size of KEEP and the narrow phase (2, 4
+// or 8 wave64s, their LDS sized to match) are tuned per kernel and size class ceil(log2 n), by
+// timing candidates on live data (bvhcd_autotune / bvhcd_batch_autotune). The table is
+// process-wide and not locked.
 namespace {
 constexpr int kClasses = 40;
 const unsigned kDefaultGrid[TK_COUNT] = {2048u, 4096u, 2048u};
@@ -620,11 +626,11 @@ void record_tuning(int kernel, int64_t n, unsigned sh, float ms) {
 }
 std::string tuning_report() {
     static const char* names[TK_COUNT] = {"k_keep", "k_narrow", "k_batch_cd"};
-    std::string r = "kernel size_class grid unroll ms_per_launch chosen\n";
+    std::string r = "kernel size_class grid unroll block ms_per_launch chosen\n";
     char line[160];
     for (const TuneTrial& t : g_trials) {
-        snprintf(line, sizeof(line), "%s 2^%d %u %u %.4f %s\n", names[t.kernel], t.cls, shape_grid(t.grid),
-                 shape_unroll(t.grid), t.ms, g_grid[t.kernel][t.cls] == t.grid ? "*" : "");
+        snprintf(line, sizeof(line), "%s 2^%d %u %u %u %.4f %s\n", names[t.kernel], t.cls, shape_grid(t.grid),
+                 shape_unroll(t.grid), 64u * shape_waves(t.grid), t.ms, g_grid[t.kernel][t.cls] == t.grid ? "*" : "");
         r += line;
     }
     return r;
@@ -1150,9 +1156,10 @@ void launch_update(DevState& d, int32_t n, int quality, hipStream_t s, bool orde
 // nodes tested and the pairs found are those of the lane's own walk; a stack that is full
 // leaves the lane to walk that subtree itself.
 constexpr uint32_t kDescStack = 512;
+template <int W>
 struct DescStack {
-    int2 item[kWaves][kDescStack];  // (query, node << 1 | leaf)
-    uint32_t top[kWaves];
+    int2 item[W][kDescStack];  // (query, node << 1 | leaf)
+    uint32_t top[W];
 };
 
 // Pushes (q, node enc) for the lanes with pred set; returns false for a lane whose item did not
@@ -1256,10 +1263,11 @@ __device__ inline uint32_t desc_drain(const DevState& d, const Out& out, lds_int
 // implicit: the current front's shared-vertex head (C_ADJ entries after the last partition) is
 // not tested when the last refit found every coordinate finite: its pairs are broad-phase hits
 // whatever the positions (both boxes hold the shared vertex) and are counted, not listed.
-template <int U>
-__global__ void __launch_bounds__(kBlock) k_keep(DevState d, FrontIn in, int implicit) {
-    __shared__ StagesBP st;
-    __shared__ DescStack ds;
+// W = waves per workgroup (the tuner picks it with the grid and U).
+template <int U, int W>
+__global__ void __launch_bounds__(64 * W) k_keep(DevState d, FrontIn in, int implicit) {
+    __shared__ StagesBPT<W> st;
+    __shared__ DescStack<W> ds;
     stages_init(st);
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) ds.top[w] = 0;
@@ -1434,12 +1442,21 @@ void launch_keep(DevState& d, int32_t n, hipStream_t s, bool counters_zeroed) {
     const int implicit = d.inter ? 0 : 1;
     if (implicit && d.adj_version != d.front_version) launch_adj_partition(d, n, s);
     if (!counters_zeroed) reset_step_counters(d, s);
-    const unsigned sh = tuned_shape(TK_KEEP, n), g = shape_grid(sh);
-    switch (shape_unroll(sh)) {
-        case 4: hipLaunchKernelGGL(k_keep<4>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), implicit); break;
-        case 2: hipLaunchKernelGGL(k_keep<2>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), implicit); break;
-        default: hipLaunchKernelGGL(k_keep<1>, dim3(g), dim3(kBlock), 0, s, d, cur_front(d), implicit); break;
+    const unsigned sh = tuned_shape(TK_KEEP, n), g = shape_grid(sh), w = shape_waves(sh);
+    const FrontIn in = cur_front(d);
+#define BVHCD_KEEP(U, W) hipLaunchKernelGGL((k_keep<U, W>), dim3(g), dim3(64 * W), 0, s, d, in, implicit)
+    switch (shape_unroll(sh) * 16 + w) {
+        case 4 * 16 + 8: BVHCD_KEEP(4, 8); break;
+        case 4 * 16 + 2: BVHCD_KEEP(4, 2); break;
+        case 4 * 16 + 4: BVHCD_KEEP(4, 4); break;
+        case 2 * 16 + 8: BVHCD_KEEP(2, 8); break;
+        case 2 * 16 + 2: BVHCD_KEEP(2, 2); break;
+        case 2 * 16 + 4: BVHCD_KEEP(2, 4); break;
+        case 1 * 16 + 8: BVHCD_KEEP(1, 8); break;
+        case 1 * 16 + 2: BVHCD_KEEP(1, 2); break;
+        default: BVHCD_KEEP(1, 4); break;
     }
+#undef BVHCD_KEEP
 }
 
 // ---------------------------------------------------------------- incremental front restructure
@@ -1540,7 +1557,7 @@ void launch_front_restr_apply(DevState& d, int32_t n, uint32_t ntasks, hipStream
     FrontIn kept = cur_front(d);
     kept.size[0] = d.ctr + C_KEPT_I;
     kept.size[1] = d.ctr + C_KEPT_E;
-    hipLaunchKernelGGL(k_keep<1>, dim3(stream_grid()), dim3(kBlock), 0, s, d, kept, 0);
+    hipLaunchKernelGGL((k_keep<1, kWaves>), dim3(stream_grid()), dim3(kBlock), 0, s, d, kept, 0);
     // the walks write segments into the spare buffers, then append after the kept entries
     front_begin(d, s);
     FrontSink fi = next_sink(d, 0), fe = next_sink(d, 1);
@@ -1634,14 +1651,15 @@ __device__ bool sat_tri_tri(float3 P1, float3 P2, float3 P3, float3 Q1, float3 Q
     return true;
 }
 
-__global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
-    __shared__ int2 sbuf[kWaves][kStage];
-    __shared__ uint32_t sfill[kWaves][2];
+template <int W>
+__global__ void __launch_bounds__(64 * W) k_narrow(DevState d) {
+    __shared__ int2 sbuf[W][kStage];
+    __shared__ uint32_t sfill[W][2];
     __shared__ uint32_t pre[kPairSegs + 1];
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) {
         sfill[w][0] = 0;
-        sfill[w][1] = rotor_start();
+        sfill[w][1] = rotor_start<W>();
     }
     const SegPairSink sink = pair_segments(d, 1, 8u);
     lds_int2* const sb = (lds_int2*)sbuf[w];
@@ -1688,7 +1706,12 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevState d) {
 void launch_narrow(DevState& d, hipStream_t s, bool counters_zeroed) {
     // kernels, not memsets: the batched frame graph records this
     if (!counters_zeroed) fill_u32(d.pcnt + kPairSegs * kCntStride, kPairSegs * kCntStride, 0u, s);
-    hipLaunchKernelGGL(k_narrow, dim3(shape_grid(tuned_shape(TK_NARROW, d.maxT))), dim3(kBlock), 0, s, d);
+    const unsigned sh = tuned_shape(TK_NARROW, d.maxT), g = shape_grid(sh);
+    switch (shape_waves(sh)) {
+        case 8: hipLaunchKernelGGL(k_narrow<8>, dim3(g), dim3(512), 0, s, d); break;
+        case 2: hipLaunchKernelGGL(k_narrow<2>, dim3(g), dim3(128), 0, s, d); break;
+        default: hipLaunchKernelGGL(k_narrow<4>, dim3(g), dim3(256), 0, s, d); break;
+    }
 }
 
 // Totals of both segmented pair lists (requested counts: what the segments hold plus what none

@@ -25,9 +25,13 @@ void launch_emit_full(DevState& d, int32_t n, hipStream_t s);
 // class ceil(log2 n), process-wide; 0 entries fall back to the defaults.
 enum TunedKernel { TK_KEEP = 0, TK_NARROW = 1, TK_BATCH_CD = 2, TK_COUNT = 3 };
 // A launch shape: grid size (low 24 bits) and items per thread per loop iteration (1, 2, 4).
-inline unsigned make_shape(unsigned grid, unsigned unroll) { return grid | (unroll << 24); }
+// launch shape: grid (24 bits), entries per thread (4 bits), waves per workgroup (4 bits, 0 = 4)
+inline unsigned make_shape(unsigned grid, unsigned unroll, unsigned waves = 4) {
+    return grid | (unroll << 24) | ((waves & 15u) << 28);
+}
 inline unsigned shape_grid(unsigned sh) { return sh & 0xFFFFFFu; }
-inline unsigned shape_unroll(unsigned sh) { return (sh >> 24) ? (sh >> 24) : 1u; }
+inline unsigned shape_unroll(unsigned sh) { return ((sh >> 24) & 15u) ? ((sh >> 24) & 15u) : 1u; }
+inline unsigned shape_waves(unsigned sh) { return (sh >> 28) ? (sh >> 28) : 4u; }
 unsigned tuned_shape(int kernel, int64_t n);
 void set_tuned_shape(int kernel, int64_t n, unsigned sh);
 void record_tuning(int kernel, int64_t n, unsigned sh, float ms);

@@ -319,9 +319,10 @@ const char* bvhcd_batch_last_error(const bvhcd_batch* b);
 
 /* ---- autotuner (replaces CudaDevice::launchConfig / evalOptimalBlockSize, CudaDevice.cu:60-94) ----
    Block sizes are fixed at 256 (wave64 x 4); the launch shapes of the grid-stride and persistent
-   kernels (grid size, and for KEEP the entries per thread per loop iteration) are tuned by timing
-   candidates on the current data, per kernel and size class ceil(log2 n), in a process-wide
-   table used by every later launch. */
+   kernels (grid size, for KEEP the entries per thread per loop iteration, and for KEEP and the
+   narrow phase the workgroup size: 128, 256 or 512 threads) are tuned by timing candidates on the
+   current data, per kernel and size class ceil(log2 n), in a process-wide table used by every
+   later launch. */
 #define BVHCD_TUNE_KEEP      0
 #define BVHCD_TUNE_NARROW    1
 #define BVHCD_TUNE_BATCH_CD  2
@@ -333,6 +334,10 @@ int  bvhcd_autotune(bvhcd_ctx* ctx);
    unroll 1, 2 or 4 for KEEP, 1 elsewhere. Stands where CudaDevice::launchConfig
    (system/CudaDevice/CudaDevice.cu:85-94) fixes a kernel's configuration by name. */
 int  bvhcd_autotune_set(int kernel, int64_t n, uint32_t grid, uint32_t unroll);
+/* The same with the workgroup size: 128, 256 or 512 threads for KEEP and the narrow phase, 256
+   for the batched traversal (CudaDevice::evalOptimalBlockSize, CudaDevice.cu:60-83, picks a
+   block size per kernel; bvhcd_autotune_set keeps 256). */
+int  bvhcd_autotune_set_block(int kernel, int64_t n, uint32_t grid, uint32_t unroll, uint32_t block);
 /* The persistent batched traversal; drops the captured frame graphs (re-captured on next use). */
 int  bvhcd_batch_autotune(bvhcd_batch* b);
 /* Text table of every timed candidate (kernel, size class, grid, ms, chosen); returns its

@@ -149,20 +149,21 @@ def test_autotune_keeps_results(pkg, oracle):
     ctx.close()
 
 
-@pytest.mark.parametrize("grid,unroll", [(64, 1), (1024, 2), (3000, 4), (1, 4)])
-def test_keep_launch_shapes(pkg, oracle, grid, unroll):
-    """KEEP with every unroll factor and the narrow phase with the same grids (down to one
-    workgroup, and sizes that leave partial iterations) give the brute-force pair sets; the
-    default shapes are restored afterwards."""
-    print(f"[gpu] keep/narrow shape grid {grid} unroll {unroll}", flush=True)
+@pytest.mark.parametrize("grid,unroll,block", [(64, 1, 256), (1024, 2, 128), (3000, 4, 512), (1, 4, 128),
+                                              (5, 2, 512), (777, 1, 128)])
+def test_keep_launch_shapes(pkg, oracle, grid, unroll, block):
+    """KEEP with every unroll factor and workgroup size, and the narrow phase with the same grids
+    and workgroups (down to one workgroup, and sizes that leave partial iterations) give the
+    brute-force pair sets; the default shapes are restored afterwards."""
+    print(f"[gpu] keep/narrow shape grid {grid} unroll {unroll} block {block}", flush=True)
     sc = pkg.Scene.preset("cloth256", n=96)
     ctx = pkg.CollisionContext(sc.num_tris, sc.num_verts)
     ctx.set_topology(sc.faces(), sc.num_verts)
     ctx.update_vertices(sc.frame(100))
     ctx.step(pkg.BVH_BUILD, pkg.FRONT_GENERATE)
     try:
-        pkg.autotune_set(pkg.TUNE_KEEP, sc.num_tris, grid, unroll)
-        pkg.autotune_set(pkg.TUNE_NARROW, sc.num_tris, grid, 1)
+        pkg.autotune_set(pkg.TUNE_KEEP, sc.num_tris, grid, unroll, block)
+        pkg.autotune_set(pkg.TUNE_NARROW, sc.num_tris, grid, 1, block)
         for fr in (101, 102):
             ctx.update_vertices(sc.frame(fr))
             ctx.step(pkg.BVH_REFIT, pkg.FRONT_KEEP)

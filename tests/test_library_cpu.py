@@ -184,14 +184,19 @@ def test_bench_rejects_multi_gpu():
 
 def test_autotune_set_validates_shapes():
     """Launch-shape override (host-side table only): grid in [1, 2^24), unroll 1/2/4 for KEEP
-    and 1 for the other kernels."""
+    and 1 for the other kernels, workgroups of 128 / 256 / 512 threads for KEEP and the narrow
+    phase and 256 for the batched traversal."""
     pkg = load_package()
     for args in [(pkg.TUNE_KEEP, 1000, 0, 1), (pkg.TUNE_KEEP, 1000, 1 << 24, 1), (pkg.TUNE_KEEP, 1000, 64, 3),
                  (pkg.TUNE_NARROW, 1000, 64, 2), (pkg.TUNE_BATCH_CD, 1000, 64, 2), (7, 1000, 64, 1),
-                 (pkg.TUNE_KEEP, 0, 64, 1)]:
+                 (pkg.TUNE_KEEP, 0, 64, 1), (pkg.TUNE_KEEP, 1000, 64, 1, 64), (pkg.TUNE_KEEP, 1000, 64, 1, 1024),
+                 (pkg.TUNE_NARROW, 1000, 64, 1, 384), (pkg.TUNE_BATCH_CD, 1000, 64, 1, 128)]:
         with pytest.raises(pkg.BvhcdError):
             pkg.autotune_set(*args)
     pkg.autotune_set(pkg.TUNE_KEEP, 1000, 512, 4)
+    pkg.autotune_set(pkg.TUNE_KEEP, 1000, 2048, 1, 512)
+    pkg.autotune_set(pkg.TUNE_NARROW, 1000, 2048, 1, 128)
+    pkg.autotune_set(pkg.TUNE_NARROW, 1000, 4096, 1)
     pkg.autotune_set(pkg.TUNE_KEEP, 1000, 2048, 1)
 
 
