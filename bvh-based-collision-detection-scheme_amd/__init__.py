@@ -51,7 +51,7 @@ class Config(C.Structure):
         ("int_front_cap", C.c_int64), ("ext_front_cap", C.c_int64), ("pair_cap", C.c_int64),
         ("narrow_phase", C.c_int32), ("scheduler", C.c_int32), ("enable_restructure", C.c_int32),
         ("bvh_cycle", C.c_int32), ("front_cycle", C.c_int32), ("device", C.c_int32),
-        ("stream", C.c_void_p), ("skip_adjacent", C.c_int32),
+        ("stream", C.c_void_p), ("skip_adjacent", C.c_int32), ("auto_grow", C.c_int32),
     ]
 
 
@@ -273,11 +273,12 @@ class CollisionContext:
 
     def __init__(self, max_tris, max_verts, *, int_front_cap=0, ext_front_cap=0, pair_cap=0,
                  narrow_phase=True, scheduler=SCHED_PLAIN, enable_restructure=True, bvh_cycle=12,
-                 front_cycle=6, device=0, stream=None, skip_adjacent=False):
+                 front_cycle=6, device=0, stream=None, skip_adjacent=False, auto_grow=False):
         L = lib()
         cfg = Config(int(max_tris), int(max_verts), int(int_front_cap), int(ext_front_cap), int(pair_cap),
                      1 if narrow_phase else 0, int(scheduler), 1 if enable_restructure else 0,
-                     int(bvh_cycle), int(front_cycle), int(device), stream, 1 if skip_adjacent else 0)
+                     int(bvh_cycle), int(front_cycle), int(device), stream, 1 if skip_adjacent else 0,
+                     1 if auto_grow else 0)
         h = C.c_void_p()
         st = L.bvhcd_create(C.byref(cfg), C.byref(h))
         if st != OK:

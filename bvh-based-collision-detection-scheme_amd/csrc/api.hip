@@ -53,6 +53,7 @@ struct bvhcd_ctx {
     std::vector<int32_t> tops;
     std::vector<int64_t> trestr;
     std::vector<void*> allocs;
+    std::vector<void*> list_allocs;  // fronts, pair lists and what their capacities size
     LearnedState* learned = nullptr;
     float* boxes = nullptr;      // box mode input (6 floats per triangle), allocated on first use
     float* own_verts = nullptr;  // the context's vertex buffers (d.verts may borrow the caller's);
@@ -100,6 +101,55 @@ bool dalloc(bvhcd_ctx* c, T*& p, size_t n) {
 void free_all(bvhcd_ctx* c) {
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
+    for (void* p : c->list_allocs) (void)hipFree(p);
+    c->list_allocs.clear();
+}
+
+// Fronts, pair lists and what their capacities size (the sort's digit table, the shared-vertex
+// partition tiles, the wave-ordered production's chunk records), in their own allocation list so
+// that a context can grow them (cfg.auto_grow). Their contents are not kept.
+bool alloc_lists(bvhcd_ctx* c, int64_t ci, int64_t ce, int64_t cp) {
+    DevState& d = c->d;
+    const int64_t T = c->d.maxT;
+    for (void* p : c->list_allocs) (void)hipFree(p);
+    c->list_allocs.clear();
+    auto lalloc = [c](auto*& p, size_t n) {
+        void* q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(*p)) != hipSuccess) return false;
+        c->list_allocs.push_back(q);
+        p = static_cast<std::remove_reference_t<decltype(p)>>(q);
+        return true;
+    };
+    // fronts are produced in kPairSegs equal segments, then gathered: capacities are multiples
+    d.front_cap[0] = kPairSegs * ((std::max<int64_t>(ci, 1) + kPairSegs - 1) / kPairSegs);
+    d.front_cap[1] = kPairSegs * ((std::max<int64_t>(ce, 1) + kPairSegs - 1) / kPairSegs);
+    // pair lists: kPairSegs equal segments (an explicit capacity is split evenly)
+    d.pair_seg_cap = std::max<int64_t>((std::max<int64_t>(cp, 1) + kPairSegs - 1) / kPairSegs, 1);
+    d.pair_cap = d.pair_seg_cap * kPairSegs;
+    bool ok = true;
+    for (int k = 0; k < 2; ++k)
+        for (int b = 0; b < 2; ++b) {
+            ok = ok && lalloc(d.fnode[k][b], (size_t)d.front_cap[k]);
+            ok = ok && lalloc(d.fquery[k][b], (size_t)d.front_cap[k]);
+        }
+    // digit table of the largest sort (front ordering) and the scan block sums
+    const int64_t max_sort = std::max<int64_t>(T, std::max(d.front_cap[0], d.front_cap[1]));
+    d.sort_hist_n = (size_t)(256 * ((max_sort + 4095) / 4096));
+    d.scan_tmp_n = (size_t)((std::max<int64_t>((int64_t)d.sort_hist_n, T + 2) + 4095) / 4096 + 1);
+    ok = ok && lalloc(d.sort_hist, d.sort_hist_n) && lalloc(d.scan_tmp, d.scan_tmp_n);
+    ok = ok && lalloc(d.bp, (size_t)d.pair_cap) && lalloc(d.np, (size_t)d.pair_cap) &&
+         lalloc(d.pair_tmp, (size_t)d.pair_cap);
+    ok = ok && lalloc(d.adj_cnt, (size_t)(d.front_cap[1] / kAdjTile + 2)) &&
+         lalloc(d.adj_off, (size_t)(d.front_cap[1] / kAdjTile + 2));
+    // wave-ordered GENERATE: one chunk record per staging flush (every flush but a wave's last
+    // holds more than 128 entries; the last ones may all land in one segment)
+    d.chunk_seg_cap = std::max(d.front_cap[0], d.front_cap[1]) / kPairSegs / 128 + d.wave_cap + 16;
+    for (int k = 0; k < 2; ++k) ok = ok && lalloc(d.fchunk[k], (size_t)(kPairSegs * d.chunk_seg_cap));
+    // the lists start empty
+    d.fcur[0] = d.fcur[1] = 0;
+    d.adj_version = -1;
+    d.front_version++;
+    return ok && hipMemsetAsync(d.fsize, 0, 4 * sizeof(uint32_t), c->stream) == hipSuccess;
 }
 
 int allocate(bvhcd_ctx* c) {
@@ -204,41 +254,15 @@ int allocate(bvhcd_ctx* c) {
         if (c->cfg.ext_front_cap <= 0) ce = (int64_t)(ce * f);
         if (c->cfg.pair_cap <= 0) cp = (int64_t)(cp * f);
     }
-    // fronts are produced in kPairSegs equal segments, then gathered: capacities are multiples
-    d.front_cap[0] = kPairSegs * ((std::max<int64_t>(ci, 1) + kPairSegs - 1) / kPairSegs);
-    d.front_cap[1] = kPairSegs * ((std::max<int64_t>(ce, 1) + kPairSegs - 1) / kPairSegs);
-    // pair lists: kPairSegs equal segments (an explicit capacity is split evenly)
-    d.pair_seg_cap = std::max<int64_t>((std::max<int64_t>(cp, 1) + kPairSegs - 1) / kPairSegs, 1);
-    d.pair_cap = d.pair_seg_cap * kPairSegs;
-    for (int k = 0; k < 2; ++k)
-        for (int b = 0; b < 2; ++b) {
-            ok &= dalloc(c, d.fnode[k][b], d.front_cap[k]);
-            ok &= dalloc(c, d.fquery[k][b], d.front_cap[k]);
-        }
-    // digit table of the largest sort (front ordering) and the scan block sums
-    const int64_t max_sort = std::max<int64_t>(T, std::max(d.front_cap[0], d.front_cap[1]));
-    d.sort_hist_n = (size_t)(256 * ((max_sort + 4095) / 4096));
-    d.scan_tmp_n = (size_t)((std::max<int64_t>((int64_t)d.sort_hist_n, T + 2) + 4095) / 4096 + 1);
-    ok &= dalloc(c, d.sort_hist, d.sort_hist_n);
-    ok &= dalloc(c, d.scan_tmp, d.scan_tmp_n);
     d.rtask_cap = std::min<int64_t>(4 * T + (1 << 20), lim);
     ok &= dalloc(c, d.rtasks, d.rtask_cap);
-    ok &= dalloc(c, d.bp, d.pair_cap);
-    ok &= dalloc(c, d.np, d.pair_cap);
-    ok &= dalloc(c, d.pair_tmp, d.pair_cap);
     ok &= dalloc(c, d.pcnt, 2 * kPairSegs * kCntStride);
     ok &= dalloc(c, d.fcnt, 2 * kPairSegs * kCntStride);
-    ok &= dalloc(c, d.adj_cnt, (size_t)(d.front_cap[1] / kAdjTile + 2));
-    ok &= dalloc(c, d.adj_off, (size_t)(d.front_cap[1] / kAdjTile + 2));
-    // wave-ordered GENERATE: one chunk record per staging flush (every flush but a wave's last
-    // holds more than 128 entries; the last ones may all land in one segment), per-wave totals
     d.wave_cap = (int64_t)grid_for(std::max<int64_t>(T - 1, 1)) * (kBlock / 64);
-    d.chunk_seg_cap = std::max(d.front_cap[0], d.front_cap[1]) / kPairSegs / 128 + d.wave_cap + 16;
-    for (int k = 0; k < 2; ++k) {
-        ok &= dalloc(c, d.fchunk[k], (size_t)(kPairSegs * d.chunk_seg_cap));
-        ok &= dalloc(c, d.wave_tot[k], (size_t)d.wave_cap);
-    }
+    for (int k = 0; k < 2; ++k) ok &= dalloc(c, d.wave_tot[k], (size_t)d.wave_cap);
     ok &= dalloc(c, d.wave_base, (size_t)d.wave_cap);
+    if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fixed arrays)");
+    ok = alloc_lists(c, ci, ce, cp);
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
     HIPCHK(c, hipMemsetAsync(d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(d.pcnt, 0, 2 * kPairSegs * kCntStride * sizeof(uint32_t), c->stream));
@@ -557,6 +581,40 @@ int run_front(bvhcd_ctx* c, int& front_op) {
     return BVHCD_OK;
 }
 
+// cfg.auto_grow: after the step's front maintenance, a list that overflowed is grown to 1.25x
+// what the frame asked for (C_FREQ_I / C_FREQ_E for the fronts, the requested broad-phase count
+// for the pair lists) and the front is generated anew on the same tree, whose pairs are then the
+// frame's; up to four times, while device memory lasts. `front_op` = PURE_BVH_CD is redone as is.
+int grow_and_retry(bvhcd_ctx* c, int& front_op) {
+    DevState& d = c->d;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        uint32_t ctr[C_NUM];
+        launch_pair_totals(d, c->stream, c->cfg.narrow_phase != 0);  // requested pair counts
+        HIPCHK(c, hipMemcpyAsync(ctr, d.ctr, sizeof(ctr), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const uint32_t ovf = ctr[C_OVF];
+        if ((ovf & 15u) == 0u || (ovf & 16u)) return BVHCD_OK;  // fits (or a device error, reported later)
+        auto grown = [](int64_t cap, int64_t need) {
+            return std::min<int64_t>(std::max<int64_t>(cap, need + need / 4 + kPairSegs), 0x7FFFFFFFll);
+        };
+        const int64_t ci = (ovf & 1u) ? grown(d.front_cap[0], ctr[C_FREQ_I]) : d.front_cap[0];
+        const int64_t ce = (ovf & 2u) ? grown(d.front_cap[1], ctr[C_FREQ_E]) : d.front_cap[1];
+        // the pairs of a regenerated front are all listed (no implicit head)
+        const int64_t cp = (ovf & 12u) ? grown(d.pair_cap, std::max<int64_t>(ctr[C_BP], ctr[C_NP])) : d.pair_cap;
+        if (!alloc_lists(c, ci, ce, cp)) {
+            alloc_lists(c, d.front_cap[0], d.front_cap[1], d.pair_cap);  // the sizes that fitted before
+            return fail(c, BVHCD_E_OOM, "auto_grow: device memory exhausted");
+        }
+        c->front_valid = false;
+        c->repair_pending = false;
+        if (front_op != BVHCD_FRONT_PURE_BVH_CD) front_op = BVHCD_FRONT_GENERATE;
+        const int st = run_front(c, front_op);
+        c->step_zeroed = false;
+        if (st) return st;
+    }
+    return BVHCD_OK;  // still overflowing: reported by the read-back calls
+}
+
 int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     if (c->T <= 0) return fail(c, BVHCD_E_INVALID, "set_topology first");
     if (c->d.pbox && c->cfg.narrow_phase)
@@ -577,6 +635,7 @@ int do_step(bvhcd_ctx* c, int bvh_op, int front_op) {
     if (topo) c->front_valid = false;
     st = run_front(c, front_op);
     c->step_zeroed = false;
+    if (!st && c->cfg.auto_grow) st = grow_and_retry(c, front_op);
     mark_consumed();
     if (st) return st;
     end(c, S_TOTAL);

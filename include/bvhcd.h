@@ -119,6 +119,12 @@ typedef struct bvhcd_config {
                                 build's covertex filter at every pair emission, e.g.
                                 BvttFrontKeepKernels.cu:33-35,52-55); 0 = report them (default: the
                                 triangle test reports them as intersecting) */
+    int32_t auto_grow;       /* 1 = a step whose fronts or pair lists overflow grows them (to 1.25x what the
+                                frame asked for, within free HBM) and redoes its front maintenance as a
+                                GENERATE on the same tree, so it never reports BVHCD_E_CAPACITY while
+                                memory lasts; costs one host synchronisation per step. 0 = report
+                                the overflow (no counterpart in the reference, whose fixed-size
+                                buffers overflow silently, BvttFrontLooseIntra.cu:56-58) */
 } bvhcd_config;
 
 typedef struct bvhcd_stats {

@@ -582,6 +582,29 @@ def test_capacities_are_totals(pkg, oracle):
     ctx.close()
 
 
+@pytest.mark.parametrize("scheduler", ["SCHED_PLAIN", "SCHED_LEARNED_NO_NET"])
+def test_auto_grow(pkg, oracle, scheduler):
+    """auto_grow: a context created with tiny front and pair capacities grows them on the frames
+    that overflow and redoes the front, so every frame's pairs still equal brute force and no
+    call reports a capacity error; the grown lists hold what the frames need."""
+    print(f"[gpu] auto grow {scheduler}", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=80, frames=300)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc, int_front_cap=64, ext_front_cap=64, pair_cap=64, auto_grow=True,
+               scheduler=getattr(pkg, scheduler))
+    for fr in range(150, 166):
+        v = sc.frame(fr)
+        ctx.update_vertices(v)
+        ctx.step_scheduled()
+        st = ctx.stats()
+        assert st["required_int_front"] >= st["int_front"] > 64 and st["ext_front"] > 64
+        for narrow in (False, True):
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(narrow)),
+                                          oracle.pair_set(oracle.bruteforce_pairs(f, v, narrow)),
+                                          err_msg=f"frame {fr} narrow {narrow}")
+    ctx.close()
+
+
 def test_full_spheres_sequence_digests(pkg, oracle):
     """Two icospheres (40,960 triangles), all 200 frames, plain schedule: per-frame digests equal."""
     print("[gpu] spheres 200 frames", flush=True)
