@@ -406,7 +406,11 @@ def main():
     del buf
     log(f"{rep_name} with device pair read-out: {ro['fps']:.1f} frames/s (up to {ro['peak_pairs']} pairs)")
 
-    # stage breakdown: the same frames again with per-stage events (launch shapes already tuned)
+    # stage breakdown: the same frames again with per-stage events (launch shapes already tuned),
+    # first under the plain schedule (its GENERATE / UPDATE / REORDER / ordering stages), then
+    # under the reported one (last, so that a profiler's last KEEP dispatches are its frames)
+    plain_inst = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_PLAIN, torch, instrument=True)
+    _, plain_stages = roofline(plain_inst, T, V)
     inst = run_gpu(pkg, sc, faces, frames_dev, warmup, steps, pkg.SCHED_LEARNED if weights is not None
                    else pkg.SCHED_PLAIN, torch, weights=weights, instrument=True)
     log(f"instrumented pass: {inst['fps']:.1f} frames/s ({inst['restructured']} restructure frames, "
@@ -497,6 +501,7 @@ def main():
         "peak_front_entries": int((cnt[:, 3] + cnt[:, 4]).max()),
         "roofline": roof,
         "stages": stages,
+        "plain_stages": plain_stages,
         "cpu_baseline": cpu,
         "other_configs": extras or None,
     }
