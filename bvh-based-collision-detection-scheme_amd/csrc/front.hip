@@ -1034,64 +1034,135 @@ void launch_generate(DevState& d, int32_t n, hipStream_t s, bool order) {
 }
 
 // ---------------------------------------------------------------- UPDATE (prune / sprout)
+// One entry per lane and loop iteration (grid-stride, so the wave's entries stay neighbours):
+// pruned and external entries are settled at once, an overlapping internal entry starts a sprout
+// walk (the stackless walk of its subtree, one node test per step, as in k_traverse), and the
+// wave steps its walks until all have ended, lanes without a walk taking over the tails of
+// others' (from an overlapping node's escape on): the wave no longer waits for its longest
+// sprout with the other lanes idle.
 __global__ void __launch_bounds__(kBlock) k_update(DevState d, FrontIn in, FrontSink fi, FrontSink fe) {
     __shared__ Stages st;
     stages_init(st);
     const Out out = make_out(st, fi, fe, d);
+    const int lane = lane_id();
     const uint32_t ni = min(*in.size[0], in.cap[0]), ne = min(*in.size[1], in.cap[1]), total = ni + ne;
+    const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t visits = 0;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+    const int64_t limit = 64 * (2 * d.maxT + 16);
+    // wave-uniform trip count: every lane of a wave takes part in the walk rounds
+    for (uint32_t k0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); k0 < total; k0 += stride) {
+        const uint32_t k = k0 + lane;
+        bool walk = false;
+        int32_t qi = 0, cur = -1, stop = -1, spine = -1;
+        Box qb{};
         if (k < ni) {
             const int32_t q = in.query[0][k];
             int32_t node = (int32_t)in.node[0][k];
             const float4 qlo = d.qleaf[q].lo, qhi = d.qleaf[q].hi;
-            const Box qb = rec_box(qlo, qhi);
+            const Box b = rec_box(qlo, qhi);
             const float4 lo = d.inode[node].lo, hi = d.inode[node].hi;
             ++visits;
-            if (!box_overlap(qb, lo, hi)) {  // prune: climb the left spine while not overlapping
+            if (!box_overlap(b, lo, hi)) {  // prune: climb the left spine while not overlapping
                 const int32_t t = max(d.leaf_lca[f2i(lo.w)] >> 1, 0);
                 for (node--; node >= t; node--) {
                     ++visits;
-                    if (node_overlap(d, node, qb)) break;
+                    if (node_overlap(d, node, b)) break;
                 }
                 bool keep = true;
                 if (node < t && node + 1 > 0) {
                     ++visits;
-                    keep = node_overlap(d, d.int_par[node + 1], qb);  // else a left sibling's entry covers it
+                    keep = node_overlap(d, d.int_par[node + 1], b);  // else a left sibling's entry covers it
                 }
                 if (keep) out.int_entry(q, node + 1);
-            } else {
-                const int32_t start = (d.int_lc[node] << 1) | (int32_t)(d.int_mark[node] & 1u);
-                visits += stackless<true>(d, out, q, qb, start, f2i(hi.w));  // sprout
+            } else {  // sprout: walk the subtree from its left child to its escape
+                qi = q;
+                qb = b;
+                cur = (d.int_lc[node] << 1) | (int32_t)(d.int_mark[node] & 1u);
+                stop = f2i(hi.w);
+                walk = cur != stop;
             }
-        } else {
+        } else if (k < total) {
             const int32_t q = in.query[1][k - ni];
             const int32_t j = (int32_t)in.node[1][k - ni];
             const float4 qlo = d.qleaf[q].lo, qhi = d.qleaf[q].hi;
-            const Box qb = rec_box(qlo, qhi);
+            const Box b = rec_box(qlo, qhi);
             const float4 llo = d.leaf[j].lo, lhi = d.leaf[j].hi;
             ++visits;
-            if (box_overlap(qb, llo, lhi)) {
+            if (box_overlap(b, llo, lhi)) {
                 out.ext_entry(q, j);
                 out.pair(q, j);
             } else {
                 const int32_t gfa = f2i(llo.w);
                 ++visits;
-                if (node_overlap(d, gfa, qb)) {
+                if (node_overlap(d, gfa, b)) {
                     out.ext_entry(q, j);
                 } else if (!(d.leaf_mark[j] & 4u)) {  // right child: the left sibling's entry climbs
                     const int32_t top = max(d.leaf_lca[j] >> 1, 0);
                     int32_t node;
                     for (node = gfa - 1; node >= top; node--) {
                         ++visits;
-                        if (node_overlap(d, node, qb)) break;
+                        if (node_overlap(d, node, b)) break;
                     }
                     bool keep = true;
                     if (node < top && node + 1 > 0) {
                         ++visits;
-                        keep = node_overlap(d, d.int_par[node + 1], qb);
+                        keep = node_overlap(d, d.int_par[node + 1], b);
                     }
                     if (keep) out.int_entry(q, node + 1);
+                }
+            }
+        }
+        // the sprout walks of this iteration's entries, shared by the wave
+        for (int64_t round = 0; __ballot(walk) != 0ull; ++round) {
+            if (round > limit) {  // only a corrupted tree gets here (flag 16, a device error)
+                if (walk) atomicOr(&d.ctr[C_OVF], 16u);
+                break;
+            }
+            bool can_give = false;
+            int32_t give = -1;
+            if (walk) {
+                ++visits;
+                if (!(cur & 1)) {
+                    const int32_t nd = cur >> 1;
+                    const float4 lo = d.inode[nd].lo, hi = d.inode[nd].hi;
+                    if (spine < 0) spine = d.leaf_par[f2i(lo.w)];
+                    if (!box_overlap(qb, lo, hi)) {
+                        out.int_entry(qi, nd);
+                        cur = f2i(hi.w);
+                        spine = -1;
+                    } else {
+                        give = f2i(hi.w);
+                        can_give = give != stop;
+                        cur = nd == spine ? ((f2i(lo.w) << 1) | 1) : cur + 2;
+                        if (nd == spine) spine = -1;
+                    }
+                } else {
+                    const int32_t j = cur >> 1;
+                    const float4 llo = d.leaf[j].lo, lhi = d.leaf[j].hi;
+                    if (box_overlap(qb, llo, lhi)) out.pair(qi, j);
+                    out.ext_entry(qi, j);
+                    cur = f2i(lhi.w);
+                    spine = -1;
+                }
+                walk = cur != stop;
+            }
+            const uint64_t idle_m = __ballot(!walk), donors = __ballot(can_give);
+            const uint32_t pairs = (uint32_t)min(__popcll(idle_m), __popcll(donors));
+            if (pairs >= 8u || (pairs > 0u && __popcll(idle_m) >= 32)) {
+                const uint32_t ri = (uint32_t)__popcll(idle_m & lanes_below());
+                const uint32_t rd = (uint32_t)__popcll(donors & lanes_below());
+                const int src = (!walk && ri < pairs) ? kth_set_lane(donors, ri) : lane;
+                const int32_t g_q = __shfl(qi, src), g_cur = __shfl(give, src), g_stop = __shfl(stop, src);
+                const float g0 = __shfl(qb.mnx, src), g1 = __shfl(qb.mny, src), g2 = __shfl(qb.mnz, src);
+                const float g3 = __shfl(qb.mxx, src), g4 = __shfl(qb.mxy, src), g5 = __shfl(qb.mxz, src);
+                if (can_give && rd < pairs) stop = give;
+                if (!walk && ri < pairs) {
+                    walk = true;
+                    qi = g_q;
+                    cur = g_cur;
+                    stop = g_stop;
+                    spine = -1;
+                    qb = Box{g0, g1, g2, g3, g4, g5};
                 }
             }
         }
