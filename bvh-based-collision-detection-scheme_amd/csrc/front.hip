@@ -246,9 +246,11 @@ struct Out {
     int skip_adj;   // drop pairs of triangles sharing a vertex
     // Broad-phase pairs are kept as leaf (sorted-slot) ids, query first (qi < j): the narrow
     // phase then reads leaf-ordered faces with locality, and maps to triangle ids itself.
+    __device__ void pair_push(bool pred, int2 v) const { stage_push(bp, ovf, sb[2], sf[2], scap, pred, v); }
+    __device__ void pair_final() const { stage_final(bp, ovf, sb[2], sf[2]); }
     __device__ void pair(int32_t qi, int32_t j) const {
         if (skip_adj && faces_share_vertex(lfaces, qi, j)) return;
-        stage_push(bp, ovf, sb[2], sf[2], scap, true, make_int2(qi, j));
+        pair_push(true, make_int2(qi, j));
     }
     __device__ void int_entry(int32_t qi, int32_t node) const {
         stage_push(fi, ovf, sb[0], sf[0], scap, true, make_int2(qi, node));
@@ -259,7 +261,7 @@ struct Out {
     __device__ void final_flush() const {
         stage_final(fi, ovf, sb[0], sf[0]);
         stage_final(fe, ovf, sb[1], sf[1]);
-        stage_final(bp, ovf, sb[2], sf[2]);
+        pair_final();
         wave_total(fi, sf[0]);
         wave_total(fe, sf[1]);
     }
@@ -719,7 +721,7 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int32_t n, DevState d, Fron
         }
         bool pair = emit_pair;
         if (out.skip_adj && pair) pair = !faces_share_vertex(out.lfaces, qi, node);
-        stage_push(out.bp, out.ovf, out.sb[2], out.sf[2], out.scap, pair, make_int2(qi, node));
+        out.pair_push(pair, make_int2(qi, node));
         // idle lanes take the rest of other lanes' walks
         const uint64_t idle = __ballot(!act), donors = __ballot(can_give);
         const uint32_t pairs = (uint32_t)min(__popcll(idle), __popcll(donors));
@@ -744,7 +746,7 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int32_t n, DevState d, Fron
     if (FRONT) {
         out.final_flush();
     } else {
-        stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
+        out.pair_final();
     }
     add_visits(d, v);
 }
@@ -786,7 +788,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_cd(int32_t n, DevState d, cons
             }
         }
     }
-    stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
+    out.pair_final();
     add_visits(d, v);
 }
 
@@ -1272,21 +1274,17 @@ __device__ inline uint32_t desc_test(const DevState& d, const Out& out, lds_int2
     }
     bool emit = leaf_hit;
     if (out.skip_adj && emit) emit = !faces_share_vertex(out.lfaces, q, nd);
-    stage_push(out.bp, out.ovf, out.sb[2], out.sf[2], out.scap, emit, make_int2(q, nd));
+    out.pair_push(emit, make_int2(q, nd));
     int4 tp = make_int4(0, 0, 0, 0);
     if (int_hit) tp = d.itopo[nd];
     const int32_t lc = (tp.x << 1) | (tp.w & 1), rc = (tp.y << 1) | ((tp.w >> 1) & 1);
     const bool ok_l = desc_push(stk, top, int_hit, q, lc);
     const bool ok_r = desc_push(stk, top, int_hit, q, rc);
-    if (int_hit && !(ok_l && ok_r)) {  // no room: walk what did not fit here
-        if (!ok_l && !ok_r) {
-            visits += stackless<false>(d, out, q, qb, lc, f2i(hi.w));
-        } else if (!ok_r) {
-            visits += stackless<false>(d, out, q, qb, rc, f2i(hi.w));
-        } else {  // the left child alone: from it to where its subtree ends
-            const int32_t stop = (lc & 1) ? d.leaf_lca[(lc >> 1) + 1] : f2i(d.inode[lc >> 1].hi.w);
-            visits += stackless<false>(d, out, q, qb, lc, stop);
-        }
+    if (int_hit && !(ok_l && ok_r)) {  // no room: walk what did not fit here (one call site)
+        int32_t start = lc, end = f2i(hi.w);  // both children: nd's subtree below nd
+        if (ok_l) start = rc;                 // the right child alone
+        else if (ok_r) end = (lc & 1) ? d.leaf_lca[(lc >> 1) + 1] : f2i(d.inode[lc >> 1].hi.w);  // the left alone
+        visits += stackless<false>(d, out, q, qb, start, end);
     }
     return visits;
 }
@@ -1378,7 +1376,8 @@ __global__ void __launch_bounds__(64 * W) k_keep(DevState d, FrontIn in, int imp
     // are in flight, so each iteration waits for one round trip instead of two
     int32_t qn[U], ndn[U];
     load_entries(it.begin, qn, ndn);
-    for (uint32_t k0 = it.begin; k0 < it.end; k0 += U * stride) {
+    // one loop iteration: test the entries, push the children of now-overlapping nodes, stage pairs
+    auto keep_step = [&](uint32_t k0) {
         int32_t q[U], nd[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1417,12 +1416,18 @@ __global__ void __launch_bounds__(64 * W) k_keep(DevState d, FrontIn in, int imp
             }
             bool emit = hit && !internal;
             if (out.skip_adj && emit) emit = !faces_share_vertex(out.lfaces, q[u], nd[u]);
-            stage_push(out.bp, out.ovf, out.sb[2], out.sf[2], out.scap, emit, make_int2(q[u], nd[u]));
+            out.pair_push(emit, make_int2(q[u], nd[u]));
         }
-        visits += desc_drain(d, out, stk, stop, 64u);
+    };
+    // The wave stays in the loop until all its lanes are past the end (their entries read as
+    // -1), so every drain of the stack runs with the whole wave, from one call site.
+    for (uint32_t k0 = it.begin;; k0 += U * stride) {
+        const bool last = __ballot(k0 < it.end) == 0;
+        if (!last) keep_step(k0);
+        visits += desc_drain(d, out, stk, stop, last ? 1u : 64u);
+        if (last) break;
     }
-    visits += desc_drain(d, out, stk, stop, 1u);
-    stage_final(out.bp, out.ovf, out.sb[2], out.sf[2]);
+    out.pair_final();
     add_visits(d, visits);
 }
 
@@ -1722,6 +1727,30 @@ __device__ bool sat_tri_tri(float3 P1, float3 P2, float3 P3, float3 Q1, float3 Q
     return true;
 }
 
+// Narrow-phase test of a self-CD broad-phase pair of leaves `lv`; `pr` receives the pair in
+// canonical order (the triangle with the lower id first, and as P of the test).
+__device__ inline bool narrow_self(const int32_t* __restrict__ prim, const int32_t* __restrict__ lfaces,
+                                   const uint8_t* __restrict__ finite, const float* __restrict__ verts, int2 lv,
+                                   int2& pr) {
+    int32_t ta = prim[lv.x], tb = prim[lv.y], la = lv.x, lb = lv.y;
+    if (ta > tb) {
+        const int32_t t = ta; ta = tb; tb = t;
+        const int32_t l = la; la = lb; lb = l;
+    }
+    pr = make_int2(ta, tb);
+    const int32_t* fa = lfaces + 3 * (int64_t)la;
+    const int32_t* fb = lfaces + 3 * (int64_t)lb;
+    const int32_t a0 = fa[0], a1 = fa[1], a2 = fa[2], b0 = fb[0], b1 = fb[1], b2 = fb[2];
+    // A shared vertex has bitwise-equal translated coordinates on both sides, so while every
+    // product stays finite (all |coordinates| <= 1e8) its projection lies in both intervals on
+    // every axis: the SAT always reports a hit, and the test is skipped.
+    const bool share = a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 ||
+                       a2 == b1 || a2 == b2;
+    if (share && finite[la] && finite[lb]) return true;
+    return sat_tri_tri(ld3(verts, a0), ld3(verts, a1), ld3(verts, a2), ld3(verts, b0), ld3(verts, b1),
+                       ld3(verts, b2));
+}
+
 template <int W>
 __global__ void __launch_bounds__(64 * W) k_narrow(DevState d) {
     __shared__ int2 sbuf[W][kStage];
@@ -1747,28 +1776,8 @@ __global__ void __launch_bounds__(64 * W) k_narrow(DevState d) {
                        make_int2(d.prim_idx[lv.y], d.qprim[lv.x]));
             continue;
         }
-        // canonical order: the triangle with the lower id is P
-        int32_t ta = d.prim_idx[lv.x], tb = d.prim_idx[lv.y], la = lv.x, lb = lv.y;
-        if (ta > tb) {
-            const int32_t t = ta; ta = tb; tb = t;
-            const int32_t l = la; la = lb; lb = l;
-        }
-        const int2 pr = make_int2(ta, tb);
-        const int32_t* fa = d.lfaces + 3 * (int64_t)la;
-        const int32_t* fb = d.lfaces + 3 * (int64_t)lb;
-        const int32_t a0 = fa[0], a1 = fa[1], a2 = fa[2], b0 = fb[0], b1 = fb[1], b2 = fb[2];
-        // A shared vertex has bitwise-equal translated coordinates on both sides, so while
-        // every product stays finite (all |coordinates| <= 1e8) its projection lies in both
-        // intervals on every axis: the SAT always reports a hit, and the test is skipped.
-        const bool share = a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 ||
-                           a2 == b1 || a2 == b2;
-        bool hit;
-        if (share && d.tri_finite[la] && d.tri_finite[lb]) {
-            hit = true;
-        } else {
-            hit = sat_tri_tri(ld3(d.verts, a0), ld3(d.verts, a1), ld3(d.verts, a2), ld3(d.verts, b0),
-                              ld3(d.verts, b1), ld3(d.verts, b2));
-        }
+        int2 pr;
+        const bool hit = narrow_self(d.prim_idx, d.lfaces, d.tri_finite, d.verts, lv, pr);
         stage_push(sink, &d.ctr[C_OVF], sb, sf, kStage, hit, pr);
     }
     stage_final(sink, &d.ctr[C_OVF], sb, sf);
