@@ -72,6 +72,7 @@ struct LearnedState {
     uint32_t* total = nullptr;
     uint32_t* cand_cnt = nullptr; // front entries per candidate (by the node's leftmost leaf)
     int32_t* cand_of_leaf = nullptr; // candidate holding each leaf, -1 none
+    int32_t* cand_of_int = nullptr;  // candidate holding each internal node's leftmost leaf, -1 none
     uint32_t* lcnt = nullptr;     // external-front entries per leaf
     bool by_query = true;         // the current fronts are ordered by query (else by node)
     int64_t counted_topo = -1;    // topology version the counts were made on
@@ -112,7 +113,7 @@ void learned_destroy(LearnedState* L) {
     for (void* p : {(void*)L->params, (void*)L->wbf, (void*)L->grad, (void*)L->m, (void*)L->v, (void*)L->loss,
                     (void*)L->x, (void*)L->y, (void*)L->t, (void*)L->roots, (void*)L->flags, (void*)L->pos,
                     (void*)L->total, (void*)L->feats, (void*)L->cand_cnt, (void*)L->cand_of_leaf, (void*)L->lcnt,
-                    (void*)L->gate})
+                    (void*)L->cand_of_int, (void*)L->gate})
         if (p) (void)hipFree(p);
     delete L;
 }
@@ -482,6 +483,13 @@ __global__ void __launch_bounds__(kBlock) k_cand_of_leaf(const int32_t* __restri
     const int32_t r = roots[blockIdx.x];
     for (int32_t j = d.int_rx[r] + threadIdx.x; j <= d.int_ry[r]; j += kBlock) cand_of_leaf[j] = blockIdx.x;
 }
+// The candidate of every internal node's leftmost leaf, so that counting an internal front entry
+// takes one lookup.
+__global__ void __launch_bounds__(kBlock) k_cand_of_int(int32_t I, DevState d, const int32_t* __restrict__ cand_of_leaf,
+                                                        int32_t* __restrict__ cand_of_int) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < I) cand_of_int[i] = cand_of_leaf[d.int_rx[i]];
+}
 
 // Front counts for the features, recomputed only when the front changed:
 //  * cand_cnt[c]: entries whose key leaf (an internal entry's node's leftmost leaf, an external
@@ -493,7 +501,30 @@ __global__ void __launch_bounds__(kBlock) k_cand_of_leaf(const int32_t* __restri
 // leaf (node-ordered fronts); keys outside go to global atomics. The tables are then added to
 // the global counts once per non-zero bin.
 constexpr int kCandLds = 4096, kLeafWin = 4096;
+// Workgroups of the count: smaller pieces keep more of a piece's external leaves inside its
+// LDS window (the rest take global atomics).
+constexpr int kCountGrid = 2048;
+
+// Adds one to ctr[key] for every lane where `valid` holds, with one atomic per run of equal keys
+// in neighbouring lanes: on query-ordered fronts a wave's entries mostly share their candidate,
+// and one atomic per lane on one word would serialise. Called by the whole wave.
+template <class T>
+__device__ inline void run_count(T* ctr, bool valid, uint32_t key) {
+    const int lane = lane_id();
+    const uint64_t vmask = __ballot(valid);
+    const uint32_t prev = __shfl_up(key, 1);
+    const bool prev_valid = lane > 0 && ((vmask >> (lane - 1)) & 1ull) != 0;
+    const bool start = valid && !(prev_valid && prev == key);
+    const uint64_t starts = __ballot(start);
+    if (start) {  // the run ends at the next start or the next invalid lane
+        const uint64_t le = (2ull << lane) - 1ull;
+        const uint64_t stop = (starts | ~vmask) & ~le;
+        const int e = stop ? __ffsll((long long)stop) - 1 : 64;
+        atomicAdd(ctr + key, (uint32_t)(e - lane));
+    }
+}
 __global__ void __launch_bounds__(kBlock) k_count_front(DevState d, const int32_t* __restrict__ cand_of_leaf,
+                                                        const int32_t* __restrict__ cand_of_int,
                                                         int32_t ncand, int by_query, uint32_t* __restrict__ cand_cnt,
                                                         uint32_t* __restrict__ lcnt) {
     __shared__ uint32_t hc[kCandLds];
@@ -514,18 +545,32 @@ __global__ void __launch_bounds__(kBlock) k_count_front(DevState d, const int32_
     }
     __syncthreads();
     const int32_t wb = wbase;
-    for (uint32_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
-        const bool ext = k >= ni;
-        const int32_t leaf = ext ? (int32_t)d.fnode[1][c1][k - ni] : d.int_rx[d.fnode[0][c0][k]];
-        const int32_t c = cand_of_leaf[leaf];
-        if (c >= 0) {
-            if (lds_cand) atomicAdd(&hc[c], 1u);
-            else atomicAdd(&cand_cnt[c], 1u);
+    // kCountU entries per thread and iteration (an entry, then its candidate: an external entry's
+    // leaf's, an internal entry's node's), their loads issued together
+    constexpr int kCountU = 4;
+    for (uint32_t kb = k0; kb < k1; kb += kCountU * kBlock) {  // wave-uniform trip count
+        int32_t nd[kCountU];  // the entry's node: an external entry's leaf
+#pragma unroll
+        for (int u = 0; u < kCountU; ++u) {
+            const uint32_t k = kb + u * kBlock + threadIdx.x;
+            nd[u] = k < k1 ? (int32_t)(k >= ni ? d.fnode[1][c1][k - ni] : d.fnode[0][c0][k]) : 0;
         }
-        if (ext) {
-            const uint32_t o = (uint32_t)(leaf - wb);
-            if (o < (uint32_t)kLeafWin) atomicAdd(&hl[o], 1u);
-            else atomicAdd(&lcnt[leaf], 1u);
+        int32_t c[kCountU];
+#pragma unroll
+        for (int u = 0; u < kCountU; ++u) {
+            const uint32_t k = kb + u * kBlock + threadIdx.x;
+            c[u] = k < k1 ? (k >= ni ? cand_of_leaf[nd[u]] : cand_of_int[nd[u]]) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kCountU; ++u) {
+            const uint32_t k = kb + u * kBlock + threadIdx.x;
+            const bool in = k < k1, ext = k >= ni;
+            if (lds_cand) run_count(hc, c[u] >= 0, (uint32_t)max(c[u], 0));
+            else run_count(cand_cnt, c[u] >= 0, (uint32_t)max(c[u], 0));
+            const uint32_t o = (uint32_t)(nd[u] - wb);
+            const bool win = in && ext && o < (uint32_t)kLeafWin;
+            run_count(hl, win, win ? o : 0u);
+            if (in && ext && !win) atomicAdd(&lcnt[nd[u]], 1u);
         }
     }
     __syncthreads();
@@ -591,12 +636,13 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
     if (I < 1) return BVHCD_OK;
     if (L->node_cap < n) {
         for (void* p : {(void*)L->roots, (void*)L->flags, (void*)L->pos, (void*)L->total, (void*)L->feats,
-                        (void*)L->cand_cnt, (void*)L->cand_of_leaf, (void*)L->lcnt})
+                        (void*)L->cand_cnt, (void*)L->cand_of_leaf, (void*)L->cand_of_int, (void*)L->lcnt})
             if (p) (void)hipFree(p);
         L->roots = nullptr; L->flags = nullptr; L->pos = nullptr; L->total = nullptr; L->feats = nullptr;
-        L->cand_cnt = nullptr; L->cand_of_leaf = nullptr; L->lcnt = nullptr;
+        L->cand_cnt = nullptr; L->cand_of_leaf = nullptr; L->cand_of_int = nullptr; L->lcnt = nullptr;
         if (hipMalloc(&L->cand_cnt, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->cand_of_leaf, sizeof(int32_t) * n) != hipSuccess ||
+            hipMalloc(&L->cand_of_int, sizeof(int32_t) * n) != hipSuccess ||
             hipMalloc(&L->lcnt, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->roots, sizeof(int32_t) * n) != hipSuccess ||
             hipMalloc(&L->flags, sizeof(uint32_t) * n) != hipSuccess ||
@@ -619,9 +665,12 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
         L->by_query = d.front_by_query != 0;
         fill_i32(L->cand_of_leaf, n, -1, s);
         hipLaunchKernelGGL(k_cand_of_leaf, dim3(cnt), dim3(kBlock), 0, s, L->roots, d, L->cand_of_leaf);
+        hipLaunchKernelGGL(k_cand_of_int, dim3(grid_for(I)), dim3(kBlock), 0, s, I, d, L->cand_of_leaf,
+                           L->cand_of_int);
         hipMemsetAsync(L->cand_cnt, 0, sizeof(uint32_t) * (size_t)cnt, s);
         hipMemsetAsync(L->lcnt, 0, sizeof(uint32_t) * (size_t)n, s);
-        hipLaunchKernelGGL(k_count_front, dim3(1024), dim3(kBlock), 0, s, d, L->cand_of_leaf, (int32_t)cnt,
+        hipLaunchKernelGGL(k_count_front, dim3(kCountGrid), dim3(kBlock), 0, s, d, L->cand_of_leaf, L->cand_of_int,
+                           (int32_t)cnt,
                            L->by_query ? 1 : 0, L->cand_cnt, L->lcnt);
         L->counted_version = d.front_version;
         L->counted_topo = d.topo_version;
