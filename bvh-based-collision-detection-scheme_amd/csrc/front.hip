@@ -873,21 +873,26 @@ __device__ inline uint32_t run_atomic(uint32_t* ctr, bool valid, uint32_t key) {
 // The same with every set of equal keys in the wave taken together, one round per distinct key:
 // internal fronts put nodes with large subtrees into the fronts of many queries, and their
 // counters would otherwise take one contended atomic per entry.
+// The rounds only group the lanes: each group's lowest lane issues its atomic and keeps the old
+// value, and one shuffle after the last round hands the bases out, so the atomics of all rounds
+// are in flight together instead of one round trip per distinct key.
 __device__ inline uint32_t match_atomic(uint32_t* ctr, bool valid, uint32_t key) {
     const int lane = lane_id();
     uint64_t todo = __ballot(valid);
-    uint32_t r = 0;
+    uint32_t base = 0, rank = 0;
+    int lead = lane;
     while (todo) {  // wave-uniform
         const int first = __ffsll((long long)todo) - 1;
         const uint32_t k = (uint32_t)__shfl((int)key, first);
         const uint64_t grp = __ballot(key == k) & todo;
-        uint32_t base = 0;
         if (lane == first) base = atomicAdd(ctr + k, (uint32_t)__popcll(grp));
-        base = (uint32_t)__shfl((int)base, first);
-        if ((grp >> lane) & 1ull) r = base + (uint32_t)__popcll(grp & lanes_below());
+        if ((grp >> lane) & 1ull) {
+            lead = first;
+            rank = (uint32_t)__popcll(grp & lanes_below());
+        }
         todo &= ~grp;
     }
-    return r;
+    return (uint32_t)__shfl((int)base, lead) + rank;
 }
 template <bool MATCH>
 __device__ inline uint32_t slot_atomic(uint32_t* ctr, bool valid, uint32_t key) {
