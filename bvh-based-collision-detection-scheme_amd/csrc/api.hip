@@ -834,6 +834,9 @@ int bvhcd_set_topology(bvhcd_ctx* c, const int32_t* faces, int64_t T, int64_t V,
     c->d.fcur[0] = c->d.fcur[1] = 0;
     HIPCHK(c, hipMemsetAsync(c->d.fsize, 0, 4 * sizeof(uint32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d.ctr, 0, 32 * sizeof(uint32_t), c->stream));
+    // the learned schedule's decision buffers are sized here, not on its first decision frame
+    if (c->learned && c->cfg.scheduler == BVHCD_SCHED_LEARNED && learned_reserve(c->learned, (int32_t)T))
+        return fail(c, BVHCD_E_OOM, "learned decision buffers");
     return BVHCD_OK;
 }
 
@@ -1255,7 +1258,11 @@ int bvhcd_mlp_set_weights(bvhcd_ctx* c, const float* p, int64_t n) {
     if (!c || !p || n != BVHCD_MLP_NPARAM) return BVHCD_E_INVALID;
     if (!c->learned) c->learned = learned_create(c->stream);
     if (!c->learned) return fail(c, BVHCD_E_OOM, "learned state");
-    return learned_set_weights(c->learned, p, c->stream, &c->err);
+    const int st = learned_set_weights(c->learned, p, c->stream, &c->err);
+    if (st) return st;
+    if (c->T > 0 && c->cfg.scheduler == BVHCD_SCHED_LEARNED && learned_reserve(c->learned, (int32_t)c->T))
+        return fail(c, BVHCD_E_OOM, "learned decision buffers");
+    return BVHCD_OK;
 }
 int bvhcd_mlp_get_weights(bvhcd_ctx* c, float* p, int64_t n) {
     if (!c || !p || n != BVHCD_MLP_NPARAM) return BVHCD_E_INVALID;

@@ -79,6 +79,7 @@ struct LearnedState {
     int64_t counted_version = -1; // front version the per-leaf counts belong to
     int64_t node_cap = 0;
     float* feats = nullptr;
+    float* score = nullptr;       // the network's score of each candidate (selection frames)
     float threshold = 0.0f;
     float repair_cost = kDefaultRepairCost;  // cost gate (0 = off), front entries' KEEP work per entry
     float* gate = nullptr;                   // device: predicted saving of the selection
@@ -113,7 +114,7 @@ void learned_destroy(LearnedState* L) {
     for (void* p : {(void*)L->params, (void*)L->wbf, (void*)L->grad, (void*)L->m, (void*)L->v, (void*)L->loss,
                     (void*)L->x, (void*)L->y, (void*)L->t, (void*)L->roots, (void*)L->flags, (void*)L->pos,
                     (void*)L->total, (void*)L->feats, (void*)L->cand_cnt, (void*)L->cand_of_leaf, (void*)L->lcnt,
-                    (void*)L->cand_of_int, (void*)L->gate})
+                    (void*)L->cand_of_int, (void*)L->gate, (void*)L->score})
         if (p) (void)hipFree(p);
     delete L;
 }
@@ -629,17 +630,20 @@ __global__ void __launch_bounds__(kBlock) k_features(const int32_t* __restrict__
     }
 }
 
-// candidate roots + features on the device; returns the row count (synchronises)
-static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t frame, hipStream_t s, int64_t* rows) {
-    const int32_t I = n - 1;
-    *rows = 0;
-    if (I < 1) return BVHCD_OK;
+int learned_reserve(LearnedState* L, int32_t n) {
+    if (!L) return BVHCD_E_INVALID;
+    if (L->node_cap >= n && L->gate) return BVHCD_OK;
     if (L->node_cap < n) {
         for (void* p : {(void*)L->roots, (void*)L->flags, (void*)L->pos, (void*)L->total, (void*)L->feats,
-                        (void*)L->cand_cnt, (void*)L->cand_of_leaf, (void*)L->cand_of_int, (void*)L->lcnt})
+                        (void*)L->cand_cnt, (void*)L->cand_of_leaf, (void*)L->cand_of_int, (void*)L->lcnt,
+                        (void*)L->score})
             if (p) (void)hipFree(p);
         L->roots = nullptr; L->flags = nullptr; L->pos = nullptr; L->total = nullptr; L->feats = nullptr;
         L->cand_cnt = nullptr; L->cand_of_leaf = nullptr; L->cand_of_int = nullptr; L->lcnt = nullptr;
+        L->score = nullptr;
+        L->node_cap = 0;
+        L->counted_version = -1;
+        // candidates are disjoint internal subtrees, so there are fewer than n of them
         if (hipMalloc(&L->cand_cnt, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->cand_of_leaf, sizeof(int32_t) * n) != hipSuccess ||
             hipMalloc(&L->cand_of_int, sizeof(int32_t) * n) != hipSuccess ||
@@ -648,11 +652,21 @@ static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t fra
             hipMalloc(&L->flags, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->pos, sizeof(uint32_t) * n) != hipSuccess ||
             hipMalloc(&L->total, sizeof(uint32_t) * 2) != hipSuccess ||
-            hipMalloc(&L->feats, sizeof(float) * kIn * (size_t)n) != hipSuccess)
+            hipMalloc(&L->feats, sizeof(float) * kIn * (size_t)n) != hipSuccess ||
+            hipMalloc(&L->score, sizeof(float) * (size_t)n) != hipSuccess)
             return BVHCD_E_OOM;
         L->node_cap = n;
-        L->counted_version = -1;
     }
+    if (!L->gate && hipMalloc(&L->gate, sizeof(float) * 2) != hipSuccess) return BVHCD_E_OOM;
+    return BVHCD_OK;
+}
+
+// candidate roots + features on the device; returns the row count (synchronises)
+static int compute_features(LearnedState* L, DevState& d, int32_t n, int64_t frame, hipStream_t s, int64_t* rows) {
+    const int32_t I = n - 1;
+    *rows = 0;
+    if (I < 1) return BVHCD_OK;
+    if (int st = learned_reserve(L, n)) return st;
     hipLaunchKernelGGL(k_treelet_flags, dim3(grid_for(I)), dim3(kBlock), 0, s, I, d, L->flags);
     launch_exscan(d, L->flags, L->pos, I, L->total, s);
     hipLaunchKernelGGL(k_treelet_compact, dim3(grid_for(I)), dim3(kBlock), 0, s, I, L->flags, L->pos, L->roots);
@@ -763,16 +777,15 @@ int learned_select(LearnedState* L, DevState& d, int32_t n, int64_t frame, int32
         if (err) *err = "learned selection: features failed";
         return st;
     }
-    if (!L->gate && hipMalloc(&L->gate, sizeof(float) * 2) != hipSuccess) return BVHCD_E_OOM;
+    if ((st = learned_reserve(L, n))) return st;  // allocated already unless n < 2 (no candidates)
     (void)hipMemsetAsync(d.ctr + C_RT_SUB, 0, sizeof(uint32_t) * 3, s);
     (void)hipMemsetAsync(d.r_leaf_lo, 0, sizeof(int32_t) * ((size_t)n + 1), s);
     (void)hipMemsetAsync(L->gate, 0, sizeof(float) * 2, s);
     if (rows > 0) {
-        if (stage_rows(L, rows)) return BVHCD_E_OOM;
-        launch_forward(L, L->feats, rows, L->y, s);
-        hipLaunchKernelGGL(k_mark_roots, dim3(grid_for(rows)), dim3(kBlock), 0, s, rows, L->roots, L->y, L->threshold,
-                           1, d);
-        hipLaunchKernelGGL(k_gate, dim3(1), dim3(kBlock), 0, s, rows, L->roots, L->y, L->threshold, L->feats, d,
+        launch_forward(L, L->feats, rows, L->score, s);
+        hipLaunchKernelGGL(k_mark_roots, dim3(grid_for(rows)), dim3(kBlock), 0, s, rows, L->roots, L->score,
+                           L->threshold, 1, d);
+        hipLaunchKernelGGL(k_gate, dim3(1), dim3(kBlock), 0, s, rows, L->roots, L->score, L->threshold, L->feats, d,
                            L->gate);
     }
     float saving = 0.f;

@@ -19,6 +19,9 @@ int learned_grad(LearnedState* L, const float* feats, const float* targets, int6
                  int where, hipStream_t s, std::string* err);
 // True once weights are loaded.
 bool learned_ready(const LearnedState* L);
+// Allocates the per-node decision buffers for a tree of n leaves (no-op when they fit), so that
+// decision frames allocate nothing.
+int learned_reserve(LearnedState* L, int32_t n);
 // Selects the subtrees to restructure with the MLP (difference marks in d.r_leaf_lo, as the
 // heuristic selection leaves them); returns subtree / internal-node / leaf counts.
 // Scores the candidate subtrees and marks those above the threshold; the selection is dropped
