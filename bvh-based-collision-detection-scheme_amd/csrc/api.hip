@@ -145,6 +145,7 @@ bool alloc_lists(bvhcd_ctx* c, int64_t ci, int64_t ce, int64_t cp) {
     // holds more than 128 entries; the last ones may all land in one segment)
     d.chunk_seg_cap = std::max(d.front_cap[0], d.front_cap[1]) / kPairSegs / 128 + d.wave_cap + 16;
     for (int k = 0; k < 2; ++k) ok = ok && lalloc(d.fchunk[k], (size_t)(kPairSegs * d.chunk_seg_cap));
+    ok = ok && lalloc(d.clist, (size_t)(kPairSegs * d.chunk_seg_cap));
     // the lists start empty
     d.fcur[0] = d.fcur[1] = 0;
     d.adj_version = -1;
@@ -261,6 +262,8 @@ int allocate(bvhcd_ctx* c) {
     d.wave_cap = (int64_t)grid_for(std::max<int64_t>(T - 1, 1)) * (kBlock / 64);
     for (int k = 0; k < 2; ++k) ok &= dalloc(c, d.wave_tot[k], (size_t)d.wave_cap);
     ok &= dalloc(c, d.wave_base, (size_t)d.wave_cap);
+    ok &= dalloc(c, d.wave_nch, (size_t)d.wave_cap);
+    ok &= dalloc(c, d.wave_cbase, (size_t)d.wave_cap);
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fixed arrays)");
     ok = alloc_lists(c, ci, ce, cp);
     if (!ok) return fail(c, BVHCD_E_OOM, "device allocation failed (fronts / pairs)");
@@ -1484,6 +1487,7 @@ DevState inter_view(bvhcd_inter* x) {
     v.fchunk[0] = v.fchunk[1] = nullptr;  // inter fronts are sorted by query after production
     v.wave_tot[0] = v.wave_tot[1] = nullptr;
     v.wave_base = nullptr;
+    v.wave_nch = v.wave_cbase = v.clist = nullptr;
     v.wave_cap = 0;
     v.qleaf = r.leaf;
     v.qprim = r.prim_idx;

@@ -231,6 +231,9 @@ struct DevState {
     int64_t chunk_seg_cap = 0;
     uint32_t* wave_tot[2] = {nullptr, nullptr};  // [kind][wave_cap]
     uint32_t* wave_base = nullptr;              // [wave_cap] exclusive scan of one kind's totals
+    uint32_t* wave_nch = nullptr;               // [wave_cap] chunk records per wave (list cursors)
+    uint32_t* wave_cbase = nullptr;             // [wave_cap] exclusive scan of wave_nch
+    uint32_t* clist = nullptr;                  // [kPairSegs * chunk_seg_cap] chunk slots by wave
     int64_t wave_cap = 0;
     uint32_t* intcnt = nullptr;     // T
     uint32_t* extcnt = nullptr;     // T+1
@@ -305,6 +308,7 @@ enum Counter : int {
                        // above 1e8 in magnitude (or NaN / inf), or boxes given by the caller
     C_BADFACE = 26,    // set_topology: a face vertex id outside its vertex range
     C_WAVE_TOTAL = 27, // wave-ordered production: entries of the list being gathered
+    C_WAVE_CHUNKS = 28,// wave-ordered production: chunk records of the list being gathered
     C_NUM = 32
 };
 enum Counter64 : int {

@@ -460,81 +460,98 @@ static void record_chunks(DevState& d, FrontSink& sk, int kind) {
     sk.chunk_seg_cap = (uint32_t)d.chunk_seg_cap;
 }
 
-// Gathers a wave-ordered production of `kind` into its current buffer: chunk c (segment g, seq)
-// of wave w goes to wave_base[w] + its wave offset, one wave per chunk. The size becomes the
-// wave totals' sum, clamped to the capacity (overflow bit otherwise; the clamped front is a
-// dense prefix).
-__global__ void __launch_bounds__(kBlock) k_chunk_gather(DevState d, int kind, const uint32_t* __restrict__ src_node,
-                                                         const int32_t* __restrict__ src_query,
-                                                         uint32_t* __restrict__ dst_node,
-                                                         int32_t* __restrict__ dst_query, uint32_t* dst_size) {
+// Orders a wave-ordered production of `kind` into its current buffer. The chunks are first
+// listed by producing wave (k_chunk_count, a scan of the counts, k_chunk_list); then one
+// workgroup per producing wave w loads its chunks straight from the segments (chunk c holds the
+// wave's entries c.z .. c.z + count - 1) and orders the run by query with a counting sort in LDS
+// on q - 64 w (the wave's queries are 64 w .. 64 w + 63), writing it once at the scan of the
+// wave totals. The order within a query is not kept (fronts are multisets). A run longer than
+// kWaveRunLds entries, or one past the capacity, is written in production order (clamped to the
+// capacity; the overflow bit is set and the front stays a dense prefix). The size becomes the
+// wave totals' sum, clamped to the capacity.
+__global__ void __launch_bounds__(kBlock) k_chunk_count(DevState d, int kind) {
     __shared__ uint32_t pre[kPairSegs + 1];
-    const int64_t seg_cap = d.front_cap[kind] / kPairSegs;
-    const uint32_t* cnt = d.fcnt + kind * kPairSegs * kCntStride;
-    const uint32_t nchunks = seg_prefix(cnt + 1, (uint32_t)d.chunk_seg_cap, pre);  // high words: chunk seqs
-    const uint32_t total = d.ctr[C_WAVE_TOTAL];
-    const uint32_t cap = (uint32_t)min(d.front_cap[kind], (int64_t)0xFFFFFFFF);
-    const uint4* chunks = d.fchunk[kind];
-    const int lane = lane_id();
-    const uint32_t nw = gridDim.x * kWaves;
-    for (uint32_t c = blockIdx.x * kWaves + (threadIdx.x >> 6); c < nchunks; c += nw) {
+    const uint32_t nchunks =
+        seg_prefix(d.fcnt + kind * kPairSegs * kCntStride + 1, (uint32_t)d.chunk_seg_cap, pre);  // chunk seqs
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x)
+        atomicAdd(&d.wave_nch[d.fchunk[kind][seg_slot(pre, c, d.chunk_seg_cap)].y], 1u);
+}
+__global__ void __launch_bounds__(kBlock) k_chunk_list(DevState d, int kind) {
+    __shared__ uint32_t pre[kPairSegs + 1];
+    const uint32_t nchunks =
+        seg_prefix(d.fcnt + kind * kPairSegs * kCntStride + 1, (uint32_t)d.chunk_seg_cap, pre);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
         const int64_t slot = seg_slot(pre, c, d.chunk_seg_cap);
-        const uint4 ch = chunks[slot];
-        const int64_t src = (slot / d.chunk_seg_cap) * seg_cap + ch.x;
-        const uint32_t dst0 = d.wave_base[ch.y] + ch.z;
-        for (uint32_t i = lane; i < ch.w; i += 64) {
-            const uint32_t o = dst0 + i;
-            if (o < cap) {
-                dst_node[o] = src_node[src + i];
-                dst_query[o] = src_query[src + i];
-            }
-        }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *dst_size = min(total, cap);
-        if (total > cap) atomicOr(&d.ctr[C_OVF], kind ? 2u : 1u);
-        uint32_t sum = cnt[kDroppedWord];
-        for (int g = 0; g < kPairSegs; ++g) sum += cnt[g * kCntStride];
-        d.ctr[kind ? C_FREQ_E : C_FREQ_I] = sum;
+        const uint32_t w = d.fchunk[kind][slot].y;
+        d.clist[d.wave_cbase[w] + atomicSub(&d.wave_nch[w], 1u) - 1u] = (uint32_t)slot;
     }
 }
-// Then each wave's run of entries (its 64 queries 64 w .. 64 w + 63) is ordered by query, one
-// workgroup per run: a counting sort on q - 64 w through LDS into the spare buffer, which
-// becomes the current one. The order within a query is not kept (fronts are multisets). A run
-// longer than kWaveRunLds entries is copied as it is.
 constexpr uint32_t kWaveRunLds = 4096;
-__global__ void __launch_bounds__(kBlock) k_wave_sort(DevState d, int kind, const uint32_t* __restrict__ src_node,
-                                                      const int32_t* __restrict__ src_query,
-                                                      uint32_t* __restrict__ dst_node, int32_t* __restrict__ dst_query,
-                                                      const uint32_t* src_size, uint32_t* dst_size) {
+__global__ void __launch_bounds__(kBlock) k_wave_order(DevState d, int kind, uint32_t waves,
+                                                       const uint32_t* __restrict__ src_node,
+                                                       const int32_t* __restrict__ src_query,
+                                                       uint32_t* __restrict__ dst_node,
+                                                       int32_t* __restrict__ dst_query, uint32_t* dst_size) {
     __shared__ uint32_t lnode[kWaveRunLds];
-    __shared__ int32_t lquery[kWaveRunLds];
+    __shared__ uint32_t lkey[kWaveRunLds];  // (q - 64 w) << 16 | rank among the run's entries of q
     __shared__ uint32_t cnt[64];
-    const uint32_t w = blockIdx.x, size = *src_size;
-    const uint32_t b0 = min(d.wave_base[w], size), b1 = min(d.wave_base[w] + d.wave_tot[kind][w], size);
-    const uint32_t m = b1 - b0;
-    const int32_t q0 = (int32_t)(w * 64u);
-    if (threadIdx.x == 0 && w == 0) *dst_size = size;
-    if (m > kWaveRunLds) {
-        for (uint32_t i = threadIdx.x; i < m; i += kBlock) {
-            dst_node[b0 + i] = src_node[b0 + i];
-            dst_query[b0 + i] = src_query[b0 + i];
+    const uint32_t w = blockIdx.x;
+    const int64_t seg_cap = d.front_cap[kind] / kPairSegs;
+    const uint32_t cap = (uint32_t)min(d.front_cap[kind], (int64_t)0xFFFFFFFF);
+    const uint32_t b0 = d.wave_base[w], m = d.wave_tot[kind][w];
+    const uint32_t c0 = d.wave_cbase[w], c1 = w + 1 < waves ? d.wave_cbase[w + 1] : d.ctr[C_WAVE_CHUNKS];
+    const uint4* chunks = d.fchunk[kind];
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    if (w == 0 && threadIdx.x == 0) {
+        const uint32_t total = d.ctr[C_WAVE_TOTAL];
+        *dst_size = min(total, cap);
+        if (total > cap) atomicOr(&d.ctr[C_OVF], kind ? 2u : 1u);
+        const uint32_t* fc = d.fcnt + kind * kPairSegs * kCntStride;
+        uint32_t sum = fc[kDroppedWord];
+        for (int g = 0; g < kPairSegs; ++g) sum += fc[g * kCntStride];
+        d.ctr[kind ? C_FREQ_E : C_FREQ_I] = sum;
+    }
+    if (m > kWaveRunLds || (uint64_t)b0 + m > cap) {
+        for (uint32_t c = c0 + wid; c < c1; c += kWaves) {
+            const int64_t slot = d.clist[c];
+            const uint4 ch = chunks[slot];
+            const int64_t src = (slot / d.chunk_seg_cap) * seg_cap + ch.x;
+            for (uint32_t i = lane; i < ch.w; i += 64) {
+                const uint32_t o = b0 + ch.z + i;
+                if (o < cap) {
+                    dst_node[o] = src_node[src + i];
+                    dst_query[o] = src_query[src + i];
+                }
+            }
         }
         return;
     }
     if (threadIdx.x < 64) cnt[threadIdx.x] = 0u;
     __syncthreads();
-    constexpr int kPer = kWaveRunLds / kBlock;
-    uint32_t rank[kPer];
+    const int32_t q0 = (int32_t)(w * 64u);
+    for (uint32_t c = c0 + wid; c < c1; c += kWaves) {
+        const int64_t slot = d.clist[c];
+        const uint4 ch = chunks[slot];
+        const int64_t src = (slot / d.chunk_seg_cap) * seg_cap + ch.x;
+        constexpr int kPerLane = kStage3 / 64;  // a chunk is one staging flush
+        uint32_t nd[kPerLane];
+        int32_t q[kPerLane];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const uint32_t i = threadIdx.x + k * kBlock;
-        if (i < m) {
-            const uint32_t nd = src_node[b0 + i];
-            const int32_t q = src_query[b0 + i];
-            lnode[i] = nd;
-            lquery[i] = q;
-            rank[k] = atomicAdd(&cnt[min((uint32_t)(q - q0), 63u)], 1u);
+        for (int k = 0; k < kPerLane; ++k) {
+            const uint32_t i = lane + 64u * k;
+            if (i < ch.w) {
+                nd[k] = src_node[src + i];
+                q[k] = src_query[src + i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            const uint32_t i = lane + 64u * k, j = ch.z + i;
+            if (i < ch.w && j < m) {
+                const uint32_t qo = min((uint32_t)(q[k] - q0), 63u);
+                lnode[j] = nd[k];
+                lkey[j] = qo << 16 | atomicAdd(&cnt[qo], 1u);
+            }
         }
     }
     __syncthreads();
@@ -548,27 +565,24 @@ __global__ void __launch_bounds__(kBlock) k_wave_sort(DevState d, int kind, cons
         cnt[threadIdx.x] = x - v;
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const uint32_t i = threadIdx.x + k * kBlock;
-        if (i < m) {
-            const int32_t q = lquery[i];
-            const uint32_t o = b0 + cnt[min((uint32_t)(q - q0), 63u)] + rank[k];
-            dst_node[o] = lnode[i];
-            dst_query[o] = q;
-        }
+    for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
+        const uint32_t key = lkey[j], qo = key >> 16;
+        const uint32_t o = b0 + cnt[qo] + (key & 0xFFFFu);
+        dst_node[o] = lnode[j];
+        dst_query[o] = q0 + (int32_t)qo;
     }
 }
 
 static void front_finish_waves(DevState& d, int kind, int64_t waves, hipStream_t s) {
-    const int c = d.fcur[kind], nb = c ^ 1;
+    const int c = d.fcur[kind], nb = c ^ 1;  // the production is in the spare buffer's segments
     launch_exscan(d, d.wave_tot[kind], d.wave_base, waves, d.ctr + C_WAVE_TOTAL, s);
-    hipLaunchKernelGGL(k_chunk_gather, dim3(2048), dim3(kBlock), 0, s, d, kind, d.fnode[kind][nb], d.fquery[kind][nb],
-                       d.fnode[kind][c], d.fquery[kind][c], d.fsize + 2 * kind + c);
-    hipLaunchKernelGGL(k_wave_sort, dim3((unsigned)waves), dim3(kBlock), 0, s, d, kind, d.fnode[kind][c],
-                       d.fquery[kind][c], d.fnode[kind][nb], d.fquery[kind][nb], d.fsize + 2 * kind + c,
-                       d.fsize + 2 * kind + nb);
-    d.fcur[kind] = nb;
+    fill_u32(d.wave_nch, waves, 0u, s);
+    hipLaunchKernelGGL(k_chunk_count, dim3(512), dim3(kBlock), 0, s, d, kind);
+    launch_exscan(d, d.wave_nch, d.wave_cbase, waves, d.ctr + C_WAVE_CHUNKS, s);
+    hipLaunchKernelGGL(k_chunk_list, dim3(512), dim3(kBlock), 0, s, d, kind);
+    hipLaunchKernelGGL(k_wave_order, dim3((unsigned)waves), dim3(kBlock), 0, s, d, kind, (uint32_t)waves,
+                       d.fnode[kind][nb], d.fquery[kind][nb], d.fnode[kind][c], d.fquery[kind][c],
+                       d.fsize + 2 * kind + c);
 }
 
 static FrontIn cur_front(DevState& d) {
