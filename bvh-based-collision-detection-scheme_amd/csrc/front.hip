@@ -487,7 +487,8 @@ __global__ void __launch_bounds__(kBlock) k_chunk_list(DevState d, int kind) {
     }
 }
 constexpr uint32_t kWaveRunLds = 4096;
-__global__ void __launch_bounds__(kBlock) k_wave_order(DevState d, int kind, uint32_t waves,
+constexpr int kOrderBlock = 512;  // LDS allows 4 workgroups per CU: 8 waves each fill it
+__global__ void __launch_bounds__(kOrderBlock) k_wave_order(DevState d, int kind, uint32_t waves,
                                                        const uint32_t* __restrict__ src_node,
                                                        const int32_t* __restrict__ src_query,
                                                        uint32_t* __restrict__ dst_node,
@@ -512,7 +513,7 @@ __global__ void __launch_bounds__(kBlock) k_wave_order(DevState d, int kind, uin
         d.ctr[kind ? C_FREQ_E : C_FREQ_I] = sum;
     }
     if (m > kWaveRunLds || (uint64_t)b0 + m > cap) {
-        for (uint32_t c = c0 + wid; c < c1; c += kWaves) {
+        for (uint32_t c = c0 + wid; c < c1; c += kOrderBlock / 64) {
             const int64_t slot = d.clist[c];
             const uint4 ch = chunks[slot];
             const int64_t src = (slot / d.chunk_seg_cap) * seg_cap + ch.x;
@@ -529,7 +530,7 @@ __global__ void __launch_bounds__(kBlock) k_wave_order(DevState d, int kind, uin
     if (threadIdx.x < 64) cnt[threadIdx.x] = 0u;
     __syncthreads();
     const int32_t q0 = (int32_t)(w * 64u);
-    for (uint32_t c = c0 + wid; c < c1; c += kWaves) {
+    for (uint32_t c = c0 + wid; c < c1; c += kOrderBlock / 64) {
         const int64_t slot = d.clist[c];
         const uint4 ch = chunks[slot];
         const int64_t src = (slot / d.chunk_seg_cap) * seg_cap + ch.x;
@@ -565,7 +566,7 @@ __global__ void __launch_bounds__(kBlock) k_wave_order(DevState d, int kind, uin
         cnt[threadIdx.x] = x - v;
     }
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
+    for (uint32_t j = threadIdx.x; j < m; j += kOrderBlock) {
         const uint32_t key = lkey[j], qo = key >> 16;
         const uint32_t o = b0 + cnt[qo] + (key & 0xFFFFu);
         dst_node[o] = lnode[j];
@@ -580,7 +581,7 @@ static void front_finish_waves(DevState& d, int kind, int64_t waves, hipStream_t
     hipLaunchKernelGGL(k_chunk_count, dim3(512), dim3(kBlock), 0, s, d, kind);
     launch_exscan(d, d.wave_nch, d.wave_cbase, waves, d.ctr + C_WAVE_CHUNKS, s);
     hipLaunchKernelGGL(k_chunk_list, dim3(512), dim3(kBlock), 0, s, d, kind);
-    hipLaunchKernelGGL(k_wave_order, dim3((unsigned)waves), dim3(kBlock), 0, s, d, kind, (uint32_t)waves,
+    hipLaunchKernelGGL(k_wave_order, dim3((unsigned)waves), dim3(kOrderBlock), 0, s, d, kind, (uint32_t)waves,
                        d.fnode[kind][nb], d.fquery[kind][nb], d.fnode[kind][c], d.fquery[kind][c],
                        d.fsize + 2 * kind + c);
 }
