@@ -1487,7 +1487,8 @@ DevState inter_view(bvhcd_inter* x) {
     v.fchunk[0] = v.fchunk[1] = nullptr;  // inter fronts are sorted by query after production
     v.wave_tot[0] = v.wave_tot[1] = nullptr;
     v.wave_base = nullptr;
-    v.wave_nch = v.wave_cbase = v.clist = nullptr;
+    v.wave_nch = v.wave_cbase = nullptr;
+    v.clist = nullptr;
     v.wave_cap = 0;
     v.qleaf = r.leaf;
     v.qprim = r.prim_idx;

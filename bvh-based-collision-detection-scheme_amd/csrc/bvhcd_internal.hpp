@@ -233,7 +233,8 @@ struct DevState {
     uint32_t* wave_base = nullptr;              // [wave_cap] exclusive scan of one kind's totals
     uint32_t* wave_nch = nullptr;               // [wave_cap] chunk records per wave (list cursors)
     uint32_t* wave_cbase = nullptr;             // [wave_cap] exclusive scan of wave_nch
-    uint32_t* clist = nullptr;                  // [kPairSegs * chunk_seg_cap] chunk slots by wave
+    uint4* clist = nullptr;                     // [kPairSegs * chunk_seg_cap] chunks grouped by wave:
+                                                // (offset in the segments: low, high word; c.z; count)
     int64_t wave_cap = 0;
     uint32_t* intcnt = nullptr;     // T
     uint32_t* extcnt = nullptr;     // T+1

@@ -460,8 +460,9 @@ static void record_chunks(DevState& d, FrontSink& sk, int kind) {
     sk.chunk_seg_cap = (uint32_t)d.chunk_seg_cap;
 }
 
-// Orders a wave-ordered production of `kind` into its current buffer. The chunks are first
-// listed by producing wave (k_chunk_count, a scan of the counts, k_chunk_list); then one
+// Orders a wave-ordered production of `kind` into its current buffer. The chunk records are
+// first copied into lists by producing wave (k_chunk_count, a scan of the counts, k_chunk_list,
+// each with its offset in the segments, so the next kernel finds the entries in one load); then one
 // workgroup per producing wave w loads its chunks straight from the segments (chunk c holds the
 // wave's entries c.z .. c.z + count - 1) and orders the run by query with a counting sort in LDS
 // on q - 64 w (the wave's queries are 64 w .. 64 w + 63), writing it once at the scan of the
@@ -480,10 +481,13 @@ __global__ void __launch_bounds__(kBlock) k_chunk_list(DevState d, int kind) {
     __shared__ uint32_t pre[kPairSegs + 1];
     const uint32_t nchunks =
         seg_prefix(d.fcnt + kind * kPairSegs * kCntStride + 1, (uint32_t)d.chunk_seg_cap, pre);
+    const int64_t seg_cap = d.front_cap[kind] / kPairSegs;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
         const int64_t slot = seg_slot(pre, c, d.chunk_seg_cap);
-        const uint32_t w = d.fchunk[kind][slot].y;
-        d.clist[d.wave_cbase[w] + atomicSub(&d.wave_nch[w], 1u) - 1u] = (uint32_t)slot;
+        const uint4 ch = d.fchunk[kind][slot];
+        const uint64_t src = (uint64_t)((slot / d.chunk_seg_cap) * seg_cap + ch.x);  // in the segments
+        d.clist[d.wave_cbase[ch.y] + atomicSub(&d.wave_nch[ch.y], 1u) - 1u] =
+            make_uint4((uint32_t)src, (uint32_t)(src >> 32), ch.z, ch.w);
     }
 }
 constexpr uint32_t kWaveRunLds = 4096;
@@ -497,11 +501,9 @@ __global__ void __launch_bounds__(kOrderBlock) k_wave_order(DevState d, int kind
     __shared__ uint32_t lkey[kWaveRunLds];  // (q - 64 w) << 16 | rank among the run's entries of q
     __shared__ uint32_t cnt[64];
     const uint32_t w = blockIdx.x;
-    const int64_t seg_cap = d.front_cap[kind] / kPairSegs;
     const uint32_t cap = (uint32_t)min(d.front_cap[kind], (int64_t)0xFFFFFFFF);
     const uint32_t b0 = d.wave_base[w], m = d.wave_tot[kind][w];
     const uint32_t c0 = d.wave_cbase[w], c1 = w + 1 < waves ? d.wave_cbase[w + 1] : d.ctr[C_WAVE_CHUNKS];
-    const uint4* chunks = d.fchunk[kind];
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     if (w == 0 && threadIdx.x == 0) {
         const uint32_t total = d.ctr[C_WAVE_TOTAL];
@@ -514,9 +516,8 @@ __global__ void __launch_bounds__(kOrderBlock) k_wave_order(DevState d, int kind
     }
     if (m > kWaveRunLds || (uint64_t)b0 + m > cap) {
         for (uint32_t c = c0 + wid; c < c1; c += kOrderBlock / 64) {
-            const int64_t slot = d.clist[c];
-            const uint4 ch = chunks[slot];
-            const int64_t src = (slot / d.chunk_seg_cap) * seg_cap + ch.x;
+            const uint4 ch = d.clist[c];
+            const int64_t src = (int64_t)((uint64_t)ch.x | (uint64_t)ch.y << 32);
             for (uint32_t i = lane; i < ch.w; i += 64) {
                 const uint32_t o = b0 + ch.z + i;
                 if (o < cap) {
@@ -531,9 +532,8 @@ __global__ void __launch_bounds__(kOrderBlock) k_wave_order(DevState d, int kind
     __syncthreads();
     const int32_t q0 = (int32_t)(w * 64u);
     for (uint32_t c = c0 + wid; c < c1; c += kOrderBlock / 64) {
-        const int64_t slot = d.clist[c];
-        const uint4 ch = chunks[slot];
-        const int64_t src = (slot / d.chunk_seg_cap) * seg_cap + ch.x;
+        const uint4 ch = d.clist[c];
+        const int64_t src = (int64_t)((uint64_t)ch.x | (uint64_t)ch.y << 32);
         constexpr int kPerLane = kStage3 / 64;  // a chunk is one staging flush
         uint32_t nd[kPerLane];
         int32_t q[kPerLane];
