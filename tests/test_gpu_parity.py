@@ -413,6 +413,52 @@ def test_keep_shared_vertex_pairs(pkg, oracle):
     ctx.close()
 
 
+@pytest.mark.skip(reason="faulted the GPU with a GENERATE that puts the shared-vertex head in place (code "
+                         "since taken out; this test still asserts that behaviour); its sequence includes a "
+                         "rebuild with one vertex at 2e8, where two Morton codes make an 18 045-deep chain and "
+                         "the fronts overflow about 100x; cause not found, not to be run until it is")
+@pytest.mark.parametrize("skip", [False, True])
+def test_generate_shared_vertex_head(pkg, oracle, skip):
+    """The query-ordered GENERATE of the learned schedules puts the external entries whose
+    triangles share a vertex at the front's head and, while every coordinate is finite, holds
+    their pairs implicitly there: the head is exactly those entries (each part still ordered by
+    query), and pair sets equal brute force on GENERATE frames, on the KEEP frames that reuse the
+    head, and on a GENERATE frame with a vertex beyond 1e8 (no implicit pairs). With
+    skip_adjacent the vertex-sharing pairs are dropped instead."""
+    print(f"[gpu] generate shared-vertex head skip={skip}", flush=True)
+    sc = pkg.Scene.preset("cloth256", n=96)
+    f = sc.faces()
+    ctx = _ctx(pkg, sc, scheduler=pkg.SCHED_LEARNED_NO_NET, skip_adjacent=skip)
+    for fr, gen, huge in ((100, True, False), (101, False, False), (102, True, True), (103, True, False),
+                          (104, False, False)):
+        v = sc.frame(fr)
+        if huge:
+            v[5] = np.float32(2e8)
+        ctx.update_vertices(v)
+        ctx.step(pkg.BVH_BUILD if gen else pkg.BVH_REFIT, pkg.FRONT_GENERATE if gen else pkg.FRONT_KEEP)
+        st = ctx.stats()
+        if skip or huge:
+            assert st["implicit_pairs"] == 0, fr
+        else:
+            assert 0 < st["implicit_pairs"] <= st["np_pairs"] <= st["bp_pairs"], (fr, st)
+        if gen:
+            prim = ctx.export_bvh()["prim_idx"]
+            fe = ctx.export_front(1)
+            adj = (f[prim[fe[:, 0]]][:, :, None] == f[prim[fe[:, 1]]][:, None, :]).any(axis=(1, 2))
+            h = int(adj.sum())
+            assert h > 0 and adj[:h].all() and not adj[h:].any(), f"frame {fr}: head is not the shared-vertex part"
+            if not (skip or huge):
+                assert st["implicit_pairs"] == h, (fr, st["implicit_pairs"], h)
+            for part in (fe[:h], fe[h:]):
+                assert np.all(np.diff(part[:, 0]) >= 0), f"frame {fr}: a part is not ordered by query"
+        for narrow in (False, True):
+            ref = oracle.pair_set(oracle.bruteforce_pairs(f, v, narrow))
+            if skip:
+                ref = ref[~_share_vertex(f, ref)]
+            np.testing.assert_array_equal(oracle.pair_set(ctx.pairs(narrow)), ref, err_msg=f"frame {fr} {narrow}")
+    ctx.close()
+
+
 def _share_vertex(faces, keys):
     """Which pair keys (oracle.pair_set: a << 32 | b) join triangles sharing a vertex (numpy covertex)."""
     if len(keys) == 0:
